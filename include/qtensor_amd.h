@@ -1,0 +1,129 @@
+/*
+ * qtensor_amd.h -- C ABI of the MI355X bucket-elimination engine.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses
+ * owned by the caller; the library never frees caller memory.  Every entry
+ * point returns QB_OK (0) or an error code; qb_last_error() gives the message
+ * of the last failure on the calling thread.
+ *
+ * Reference interface each entry point replaces (paths in DaniloZZZ/QTensor):
+ *   qb_step           -> BucketBackend.process_bucket   qtensor/ProcessingFrameworks.py:14-15
+ *                        (NumpyBackend :29-31, CMKLExtendedBackend :45-147)
+ *   qb_build_gates    -> the gate tensors that get_sliced_buckets materialises from
+ *                        data_dict (ProcessingFrameworks.py:17-18,33-34), built on the
+ *                        device in FP64 from (gamma, beta)
+ *   qb_execute        -> the per-edge loop QAOASimulator.energy_expectation drives
+ *                        (qtensor/QAOASimulator.py:44-66) around
+ *                        bucket_elimination(process_bucket) (qtensor/Simulate.py:90-93),
+ *                        one work item per (edge, slice) (FeynmanSimulator.py:49-58)
+ *   qb_reduce_terms   -> sum of slice partials (FeynmanSimulator.py:83) and of edge
+ *                        terms (QAOASimulator.py:60), in FP64
+ *   qb_init / qb_shutdown / qb_last_error -> backend construction and the Python
+ *                        exceptions of the reference (AssertionError, ValueError)
+ */
+#ifndef QTENSOR_AMD_H
+#define QTENSOR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QB_OK 0
+#define QB_ERR_INVALID 1      /* bad argument or descriptor */
+#define QB_ERR_HIP 2          /* HIP runtime error */
+#define QB_ERR_UNSUPPORTED 3  /* shape outside what the kernels support */
+#define QB_ERR_NOT_INIT 4     /* qb_init was not called */
+
+#define QB_C64 0              /* complex64: interleaved float (re, im) */
+#define QB_C128 1             /* complex128: interleaved double (re, im) */
+
+#define QB_MAX_STEP_INPUTS 16 /* tensors per bucket step */
+#define QB_MAX_RANK 31        /* size-2 axes in a step's index union */
+
+/* Words per work item in qb_exec_desc.items. */
+#define QB_ITEM_WORDS 4
+/* item[0] = offset of the item's plan in the program buffer (int32 words)
+ * item[1] = offset of the item's angle vector in params (doubles)
+ * item[2] = slice assignment: bit k = value of the plan's k-th sliced variable
+ * item[3] = reserved (0) */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int qb_version(void);
+
+/* Select device `device` and record its properties.  Idempotent. */
+int qb_init(int device);
+
+/* Release library state (device buffers it owns).  Idempotent. */
+int qb_shutdown(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* qb_last_error(void);
+
+/*
+ * One bucket step: multiply the step's inputs over the union U of their
+ * size-2 indices (ascending variable id, C order) and, if `sum`, sum over the
+ * first union axis (the bucket variable).
+ *
+ *   out[o] = sum_{x in {0,1} if sum} prod_j in_j[ pext(x*2^(rank_u-1) + o, masks[j]) ]
+ *
+ * where pext gathers the bits of the union offset selected by masks[j]:
+ * bit (rank_u-1-a) of masks[j] is set iff input j carries union axis a.
+ * Size-1 axes carry no data and appear in no mask.  `out` holds
+ * 2^(rank_u - sum) elements and must not alias an input.
+ */
+typedef struct qb_step_desc {
+    int32_t n_inputs;
+    int32_t rank_u;
+    int32_t sum;
+    int32_t dtype;                         /* QB_C64 or QB_C128 */
+    uint32_t masks[QB_MAX_STEP_INPUTS];
+} qb_step_desc;
+
+int qb_step(const qb_step_desc* desc, const void* const* inputs, void* out, void* stream);
+
+/*
+ * Batched executor: every work item contracts one compiled lightcone plan
+ * (one edge, or one slice of one edge) to a scalar.  A persistent grid pulls
+ * items from `queue` (one uint32, zeroed by the call); each item builds its
+ * gate tensors in LDS from its FP64 angles and runs the plan's steps with
+ * intermediates in LDS or, above the plan's LDS threshold, in its slot of
+ * `scratch`.  Writes one complex128 (re, im) per item to item_out.
+ */
+typedef struct qb_exec_desc {
+    const int32_t* program;      /* device: concatenated plans */
+    const int32_t* items;        /* device: n_items * QB_ITEM_WORDS */
+    int32_t n_items;
+    int32_t item_threads;        /* 64, 128, 256, 512 or 1024 */
+    int32_t lds_elems;           /* LDS elements reserved per item (>= every plan's need) */
+    int32_t dtype;               /* QB_C64 or QB_C128 */
+    const double* params;        /* device: angle vectors (FP64) */
+    const void* const_pool;      /* device: constant gate data, dtype elements */
+    void* scratch;               /* device: n_slots * scratch_elems elements */
+    int64_t scratch_elems;       /* elements per slot (>= every plan's global need) */
+    int32_t n_slots;             /* slots available in scratch */
+    int32_t reserved;
+    double* item_out;            /* device: 2 * n_items doubles */
+    uint32_t* queue;             /* device: one uint32 */
+} qb_exec_desc;
+
+/* Number of concurrently resident work items (= scratch slots needed) for a
+ * launch shape; qb_execute uses at most this many slots. */
+int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n_slots);
+
+int qb_execute(const qb_exec_desc* desc, void* stream);
+
+/*
+ * FP64 reductions: segment s covers items [seg[s], seg[s+1]); term_out[s] is
+ * the (Neumaier-compensated) complex sum of its items' values, and total[0..1]
+ * the complex sum of all terms.  n_segments <= 2^20.
+ */
+int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segments,
+                    double* term_out, double* total, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QTENSOR_AMD_H */

@@ -1,0 +1,96 @@
+"""ctypes binding of ``libqtensor_amd.so`` (the C ABI in ``include/qtensor_amd.h``).
+
+The library is built in-tree by ``qtensor_amd/build.py``.  Loading fails loudly
+when it is missing: there is no CPU fallback for any engine entry point.
+"""
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqtensor_amd.so")
+
+QB_OK = 0
+QB_ERR_INVALID = 1
+QB_ERR_HIP = 2
+QB_ERR_UNSUPPORTED = 3
+QB_ERR_NOT_INIT = 4
+QB_C64 = 0
+QB_C128 = 1
+QB_MAX_STEP_INPUTS = 16
+QB_MAX_RANK = 31
+QB_ITEM_WORDS = 4
+
+EXPORTED = ("qb_version", "qb_init", "qb_shutdown", "qb_last_error", "qb_step",
+            "qb_exec_slots", "qb_execute", "qb_reduce_terms")
+
+
+class StepDesc(ctypes.Structure):
+    _fields_ = [("n_inputs", ctypes.c_int32), ("rank_u", ctypes.c_int32),
+                ("sum", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("masks", ctypes.c_uint32 * QB_MAX_STEP_INPUTS)]
+
+
+class ExecDesc(ctypes.Structure):
+    _fields_ = [("program", ctypes.c_void_p), ("items", ctypes.c_void_p),
+                ("n_items", ctypes.c_int32), ("item_threads", ctypes.c_int32),
+                ("lds_elems", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("params", ctypes.c_void_p), ("const_pool", ctypes.c_void_p),
+                ("scratch", ctypes.c_void_p), ("scratch_elems", ctypes.c_int64),
+                ("n_slots", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("item_out", ctypes.c_void_p), ("queue", ctypes.c_void_p)]
+
+
+class QBError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qtensor_amd error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load the shared library (no device work)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing; build it with `python -m qtensor_amd.build` "
+                "(the engine has no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.qb_version.restype = ctypes.c_int
+        lib.qb_init.argtypes = [ctypes.c_int]
+        lib.qb_last_error.restype = ctypes.c_char_p
+        lib.qb_step.argtypes = [ctypes.POINTER(StepDesc), ctypes.POINTER(ctypes.c_void_p),
+                                ctypes.c_void_p, ctypes.c_void_p]
+        lib.qb_exec_slots.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_int32)]
+        lib.qb_execute.argtypes = [ctypes.POINTER(ExecDesc), ctypes.c_void_p]
+        lib.qb_reduce_terms.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib = lib
+        return lib
+
+
+def check(code):
+    if code != QB_OK:
+        msg = load().qb_last_error().decode(errors="replace")
+        if code == QB_ERR_INVALID:
+            raise ValueError(f"qtensor_amd: {msg}")
+        raise QBError(code, msg)
+    return code
+
+
+_inited = set()
+
+
+def init(device=0):
+    lib = load()
+    if device not in _inited:
+        check(lib.qb_init(int(device)))
+        _inited.add(device)
+    return lib

@@ -1,0 +1,308 @@
+"""Batched QAOA energy engine: every edge lightcone of a graph in one persistent launch.
+
+This is the batched counterpart of the per-edge loop of
+``QAOASimulator.energy_expectation`` (``qtensor/QAOASimulator.py:44-66``): the
+same per-edge ``<Z_u Z_v>`` networks (composer + greedy ordering +
+bucket elimination), compiled once per canonical lightcone into static plans
+(``plan.py``) and executed by ``qb_execute`` -- a persistent kernel whose
+work items are (edge, slice) pairs -- followed by ``qb_reduce_terms`` (FP64,
+compensated) for the per-edge terms and the total.
+
+Gate tensors are rebuilt on the device from the request's (gamma, beta) in
+FP64, so a compiled batch can be re-run with new angles without any host
+work (``Batch.set_params`` + ``Batch.run``), and the launch sequence can be
+captured once in a HIP graph (``Batch.capture``).
+"""
+import ctypes
+import math
+import threading
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .composer import QtreeQAOAComposer
+from .lightcone import canonical_graph, canonical_lightcone
+from .ordering import OrderingOptimizer, SlicesOptimizer
+from .plan import PlanCompiler
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass
+class LaunchShape:
+    item_threads: int = 64
+    lds_capacity: int = 5120          # elements per item (complex64: 40 KiB)
+    lds_tensor_limit: int = 2048      # largest intermediate kept in LDS
+
+    def key(self):
+        return (self.item_threads, self.lds_capacity, self.lds_tensor_limit)
+
+
+DEFAULT_SHAPES = {
+    "small": LaunchShape(64, 5120, 2048),
+    "large": LaunchShape(256, 10240, 4096),
+}
+
+
+@dataclass
+class EdgeWork:
+    edge: tuple
+    key: tuple
+    n_slices: int
+
+
+class GraphWorkload:
+    """Plans for every edge of one graph at depth p (host side, reusable)."""
+
+    def __init__(self, G, p, edges, works, plans):
+        self.G = G
+        self.p = p
+        self.edges = edges
+        self.works = works
+        self.plans = plans      # key -> Plan
+
+    @property
+    def n_terms(self):
+        return len(self.works)
+
+    def step_model_bytes(self):
+        return sum(self.plans[w.key].bytes_step_model * w.n_slices for w in self.works)
+
+    def flops(self):
+        return sum(self.plans[w.key].flops * w.n_slices for w in self.works)
+
+    def max_treewidth(self):
+        return max(self.plans[w.key].treewidth for w in self.works)
+
+
+class LightconeEngine:
+    """Compile lightcones and run them in batches on one GPU.
+
+    ``composer``: QAOA composer class (default ``QtreeQAOAComposer``, the one
+    ``qtensor.QAOA_energy`` uses).  ``max_tw``: lightcones whose greedy
+    treewidth reaches it are sliced (``SlicesOptimizer``) into 2^s work items.
+    """
+
+    def __init__(self, composer=QtreeQAOAComposer, device=0, dtype="complex64", shape=None,
+                 max_tw=None, tw_bias=0):
+        self.composer = composer
+        self.device_index = device
+        self.dtype = dtype
+        self.shape = shape
+        self.max_tw = max_tw
+        self.tw_bias = tw_bias
+        self._plans = {}
+        self._lock = threading.Lock()
+
+    # -- host planning -----------------------------------------------------
+    def _optimizer(self):
+        if self.max_tw is None:
+            return OrderingOptimizer()
+        return SlicesOptimizer(max_tw=self.max_tw, tw_bias=self.tw_bias)
+
+    def shape_for(self, p):
+        if self.shape is not None:
+            return self.shape
+        return DEFAULT_SHAPES["small"] if p <= 2 else DEFAULT_SHAPES["large"]
+
+    def plan(self, key, n, edges, i, j, p, shape):
+        ck = (key, p, shape.key(), self.composer.__name__, self.max_tw)
+        with self._lock:
+            hit = self._plans.get(ck)
+        if hit is not None:
+            return hit
+        g = canonical_graph(n, edges)
+        comp = self.composer(g, gamma=[0.0] * p, beta=[0.0] * p)
+        comp.energy_expectation(i, j)
+        pc = PlanCompiler(lds_tensor_limit=shape.lds_tensor_limit, lds_capacity=shape.lds_capacity)
+        plan = pc.compile_circuit(comp.circuit, self._optimizer())
+        if plan.lds_need > shape.lds_capacity:
+            raise ValueError(f"plan needs {plan.lds_need} LDS elements (> {shape.lds_capacity}); "
+                             "its gate tensors alone do not fit the launch shape")
+        with self._lock:
+            self._plans[ck] = plan
+        return plan
+
+    def workload(self, G, p, edges=None):
+        shape = self.shape_for(p)
+        edges = list(G.edges()) if edges is None else list(edges)
+        works, plans = [], {}
+        for e in edges:
+            key, n, ce, i, j = canonical_lightcone(G, e, p)
+            plan = self.plan(key, n, ce, i, j, p, shape)
+            pk = (key, p)
+            plans[pk] = plan
+            works.append(EdgeWork(e, pk, 1 << plan.n_sliced))
+        return GraphWorkload(G, p, edges, works, plans)
+
+    # -- device batches -----------------------------------------------------
+    def batch(self, requests):
+        """Device batch for ``requests = [(workload, gamma, beta), ...]``."""
+        return Batch(self, requests)
+
+    def energy(self, G, gamma, beta, edges=None):
+        """MaxCut expectation and per-edge <Z_u Z_v> terms (complex) of one graph."""
+        p = len(gamma)
+        wl = self.workload(G, p, edges)
+        b = self.batch([(wl, gamma, beta)])
+        b.run()
+        terms = b.terms()
+        zz = terms[0]
+        C = (G.number_of_edges() - math.fsum(z.real for z in zz)) / 2
+        return C, zz
+
+
+class Batch:
+    """Device buffers and launch of one set of (workload, angles) requests."""
+
+    def __init__(self, engine, requests):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("the batched engine needs a ROCm GPU")
+        self.engine = engine
+        self.lib = _lib.init(engine.device_index)
+        dev = torch.device("cuda", engine.device_index)
+        self.device = dev
+        self.qb_dtype = _lib.QB_C64 if engine.dtype == "complex64" else _lib.QB_C128
+        elem = 8 if self.qb_dtype == _lib.QB_C64 else 16
+        shapes = {engine.shape_for(wl.p).key() for wl, _, _ in requests}
+        if len(shapes) != 1:
+            raise ValueError("all requests of a batch must use one launch shape")
+        self.shape = engine.shape_for(requests[0][0].p)
+        # program: every distinct plan once
+        plan_off, words = {}, []
+        pos = 0
+        for wl, _, _ in requests:
+            for pk, plan in wl.plans.items():
+                ident = id(plan)
+                if ident not in plan_off:
+                    plan_off[ident] = pos
+                    words.append(plan.words)
+                    pos += plan.words.size
+        program = np.concatenate(words).astype(np.int32)
+        # params, items, segments (one segment per edge term)
+        params, items, seg = [], [], [0]
+        self.term_slices = []
+        order = []
+        lds_need, glb_need = 0, 0
+        self.bytes_step_model = 0
+        self.flops = 0
+        for r, (wl, gamma, beta) in enumerate(requests):
+            if len(gamma) != wl.p or len(beta) != wl.p:
+                raise ValueError("gamma and beta must have p entries")
+            poff = len(params)
+            params += [float(x) for x in gamma] + [float(x) for x in beta]
+            start = len(seg) - 1
+            for w in wl.works:
+                plan = wl.plans[w.key]
+                lds_need = max(lds_need, plan.lds_need)
+                glb_need = max(glb_need, plan.glb_need)
+                self.bytes_step_model += plan.bytes_step_model * w.n_slices
+                self.flops += plan.flops * w.n_slices
+                for s in range(w.n_slices):
+                    items.append((plan_off[id(plan)], poff, s, 0))
+                    order.append(plan.bytes_step_model)
+                seg.append(len(items))
+            self.term_slices.append((start, len(seg) - 1))
+        self.n_items = len(items)
+        self.n_terms = len(seg) - 1
+        self.lds_elems = max(lds_need, 2)
+        self.scratch_elems = int(glb_need)
+        slots = ctypes.c_int32(0)
+        _lib.check(self.lib.qb_exec_slots(self.shape.item_threads, self.lds_elems, self.qb_dtype,
+                                          ctypes.byref(slots)))
+        self.n_slots = slots.value
+        items_np = np.asarray(items, dtype=np.int32).reshape(-1, _lib.QB_ITEM_WORDS)
+        self.program = torch.from_numpy(program).to(dev)
+        self.items = torch.from_numpy(np.ascontiguousarray(items_np)).to(dev)
+        self.params = torch.tensor(params, dtype=torch.float64, device=dev)
+        self.seg = torch.tensor(seg, dtype=torch.int32, device=dev)
+        self.item_out = torch.zeros(2 * self.n_items, dtype=torch.float64, device=dev)
+        self.term_out = torch.zeros(2 * self.n_terms, dtype=torch.float64, device=dev)
+        self.total = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.queue = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.pool = torch.zeros(2, dtype=torch.float64, device=dev)
+        it = self.shape.item_threads
+        groups = 256 // it if it < 256 else 1
+        launched = -(-max(self.n_items, 1) // groups) * groups
+        self.scratch_slots = min(self.n_slots, launched) if self.scratch_elems else 0
+        n_scr = self.scratch_elems * self.scratch_slots
+        self.scratch = torch.empty(max(n_scr, 1) * (elem // 8), dtype=torch.complex64, device=dev)
+        self.graph = None
+        self._desc = self._make_desc()
+
+    def _make_desc(self):
+        d = _lib.ExecDesc()
+        d.program = self.program.data_ptr()
+        d.items = self.items.data_ptr()
+        d.n_items = self.n_items
+        d.item_threads = self.shape.item_threads
+        d.lds_elems = self.lds_elems
+        d.dtype = self.qb_dtype
+        d.params = self.params.data_ptr()
+        d.const_pool = self.pool.data_ptr()
+        d.scratch = self.scratch.data_ptr()
+        d.scratch_elems = self.scratch_elems
+        d.n_slots = self.scratch_slots if self.scratch_elems else self.n_slots
+        d.item_out = self.item_out.data_ptr()
+        d.queue = self.queue.data_ptr()
+        return d
+
+    def set_params(self, requests_params):
+        """New (gamma, beta) per request, same structure; no host planning."""
+        flat = []
+        for gamma, beta in requests_params:
+            flat += [float(x) for x in gamma] + [float(x) for x in beta]
+        if len(flat) != self.params.numel():
+            raise ValueError("parameter count does not match the batch")
+        self.params.copy_(_torch().tensor(flat, dtype=_torch().float64))
+
+    def launch_exec(self, stream=None):
+        """Launch only the executor (queue reset + persistent kernel)."""
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        _lib.check(self.lib.qb_execute(ctypes.byref(self._desc), ctypes.c_void_p(st.cuda_stream)))
+
+    def launch(self, stream=None):
+        torch = _torch()
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        h = ctypes.c_void_p(st.cuda_stream)
+        _lib.check(self.lib.qb_execute(ctypes.byref(self._desc), h))
+        _lib.check(self.lib.qb_reduce_terms(ctypes.c_void_p(self.item_out.data_ptr()),
+                                            ctypes.c_void_p(self.seg.data_ptr()), self.n_terms,
+                                            ctypes.c_void_p(self.term_out.data_ptr()),
+                                            ctypes.c_void_p(self.total.data_ptr()), h))
+
+    def capture(self):
+        """Capture the launch sequence in a HIP graph (replayed by ``run``)."""
+        torch = _torch()
+        self.launch()     # warm up outside capture
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.launch()
+        self.graph = g
+        return g
+
+    def run(self, sync=True):
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self.launch()
+        if sync:
+            _torch().cuda.synchronize(self.device)
+
+    def terms(self):
+        """Per request: list of complex edge terms (FP64)."""
+        t = self.term_out.cpu().numpy().reshape(-1, 2)
+        out = []
+        for s, e in self.term_slices:
+            out.append([complex(a, b) for a, b in t[s:e]])
+        return out
+
+    def item_values(self):
+        return self.item_out.cpu().numpy().reshape(-1, 2)

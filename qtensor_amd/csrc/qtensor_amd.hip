@@ -1,0 +1,647 @@
+// qtensor_amd.hip -- CDNA4 (gfx950) kernels and the C ABI of the bucket-elimination engine.
+//
+// Two execution forms of the same bucket step:
+//
+//  * qb_step     -- one launch per bucket (the plain backend behind process_bucket).
+//                   Grid-stride over output elements; every input offset is a bit
+//                   gather ("pext") of the union offset, evaluated from precomputed
+//                   runs of contiguous mask bits with one bit-field extract per run.
+//  * qb_execute  -- persistent batched executor.  One work item = one compiled
+//                   lightcone plan (one edge or one slice of an edge).  Items are
+//                   pulled from an atomic queue; each item builds its gate tensors
+//                   in LDS from FP64 angles, then runs its steps with intermediate
+//                   tensors in LDS (small) or in a private slot of global scratch
+//                   (large), synchronising only where the plan marks a hazard.
+//
+// Index convention (shared with the host plan compiler): a tensor over variables
+// a_0 < a_1 < ... < a_{r-1} (all of size 2) is stored in C order, so a_0 is the most
+// significant bit of the element offset.  In a step over union U, input j's offset
+// for union offset u is pext(u, mask_j).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/qtensor_amd.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+int g_device = -1;
+int g_num_cus = 0;
+std::mutex g_mutex;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define QB_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return set_error(QB_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename R> struct Cx;
+template <> struct Cx<float> { using V = float2; };
+template <> struct Cx<double> { using V = double2; };
+
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) {
+    V r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) {
+    V r;
+    r.x = a.x + b.x;
+    r.y = a.y + b.y;
+    return r;
+}
+
+__device__ __forceinline__ uint32_t pext_loop(uint32_t x, uint32_t m) {
+    uint32_t r = 0, bit = 1;
+    while (m) {
+        uint32_t low = m & (0u - m);
+        if (x & low) r |= bit;
+        bit <<= 1;
+        m &= m - 1;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t bfe(uint32_t x, uint32_t off, uint32_t w) {
+    return __builtin_amdgcn_ubfe(x, off, w);
+}
+
+// Host-side pext-run decomposition: runs of contiguous set bits of `mask`,
+// listed from the least significant; run r copies bits [src, src+w) of the
+// union offset to bits [dst, dst+w) of the input offset.
+int mask_runs(uint32_t mask, uint8_t* src, uint8_t* w, uint8_t* dst, int max_runs) {
+    int n = 0;
+    uint32_t d = 0;
+    for (uint32_t b = 0; b < 32;) {
+        if (!((mask >> b) & 1u)) { ++b; continue; }
+        uint32_t s = b;
+        while (b < 32 && ((mask >> b) & 1u)) ++b;
+        if (n < max_runs) { src[n] = (uint8_t)s; w[n] = (uint8_t)(b - s); dst[n] = (uint8_t)d; }
+        d += b - s;
+        ++n;
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// Plain path: one bucket step per launch
+// ---------------------------------------------------------------------------
+
+constexpr int kStepMaxRuns = 6;
+
+struct StepInput {
+    const void* ptr;
+    uint32_t top;      // offset contributed by the summed variable = 1
+    uint32_t mask;     // mask over the output bits (summed bit removed)
+    uint32_t n_runs;   // > kStepMaxRuns: evaluate with pext_loop(mask)
+    uint8_t src[kStepMaxRuns], w[kStepMaxRuns], dst[kStepMaxRuns];
+};
+
+struct StepArgs {
+    StepInput in[QB_MAX_STEP_INPUTS];
+    void* out;
+    uint32_t n_in;
+    uint32_t rank_out;
+    uint32_t n_sum;    // 1 or 2
+};
+
+template <typename R>
+__device__ __forceinline__ uint32_t step_offset(const StepInput& d, uint32_t o) {
+    if (d.n_runs > (uint32_t)kStepMaxRuns) return pext_loop(o, d.mask);
+    uint32_t off = 0;
+#pragma unroll
+    for (int r = 0; r < kStepMaxRuns; ++r)
+        if ((uint32_t)r < d.n_runs) off |= bfe(o, d.src[r], d.w[r]) << d.dst[r];
+    return off;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
+    using V = typename Cx<R>::V;
+    const uint64_t n_out = 1ull << a.rank_out;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t oo = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; oo < n_out; oo += stride) {
+        const uint32_t o = (uint32_t)oo;
+        V acc;
+        acc.x = 0;
+        acc.y = 0;
+        for (uint32_t sv = 0; sv < a.n_sum; ++sv) {
+            V prod;
+            prod.x = 1;
+            prod.y = 0;
+#pragma unroll
+            for (int j = 0; j < QB_MAX_STEP_INPUTS; ++j) {
+                if ((uint32_t)j < a.n_in) {
+                    const uint32_t off = step_offset<R>(a.in[j], o) + (sv ? a.in[j].top : 0u);
+                    const V v = reinterpret_cast<const V*>(a.in[j].ptr)[off];
+                    prod = j == 0 ? v : cmul(prod, v);
+                }
+            }
+            acc = sv ? cadd(acc, prod) : prod;
+        }
+        reinterpret_cast<V*>(a.out)[o] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched executor
+// ---------------------------------------------------------------------------
+
+// Plan layout (int32 words; offsets relative to the plan start):
+//   header[kPlanHdr]: n_gates, n_steps, gates_off, steps_off, result_loc,
+//                     lds_need, glb_need, n_sliced
+//   gates: n_gates * kGateWords
+//   steps: variable length, 4 + n_in * kInWords each
+constexpr int kPlanHdr = 8;
+constexpr int kGateWords = 16;
+constexpr int kInWords = 8;
+constexpr int kExecMaxIn = 8;
+constexpr int kExecMaxRuns = 4;
+constexpr uint32_t kGlobal = 0x80000000u;
+
+// gate kinds (shared with qtensor_amd/gates.py and qtensor_amd/plan.py)
+enum GateKind : int32_t {
+    kConst = 0, kZPhase = 1, kXPhase = 2, kZZ = 3, kYPhase = 4,
+    kH = 5, kCX = 6, kZ = 7, kM = 8,
+};
+
+// Gate entry words:
+//  0 kind, 1 dst (LDS element offset), 2 n_elems, 3 src_map (4 bits per element),
+//  4 j0, 5 c0 (float bits), 6 j1, 7 c1 (float bits), 8-9 alpha0 (double bits),
+//  10 const_off, 11 n_slice | var0 << 8 | var1 << 16, 12 stride0, 13 stride1
+__device__ void natural_gate(int32_t kind, double alpha, double* re, double* im) {
+    for (int e = 0; e < 8; ++e) { re[e] = 0.0; im[e] = 0.0; }
+    const double h = 0.70710678118654752440;
+    switch (kind) {
+    case kZPhase: {
+        double s, c;
+        sincospi(alpha, &s, &c);
+        re[0] = 1.0; re[1] = c; im[1] = s;
+        break;
+    }
+    case kXPhase:
+    case kYPhase: {
+        // e^{i pi a/2} [[c, -i s], [-i s, c]]  /  e^{i pi a/2} [[c, -s], [s, c]]
+        double s, c, gs, gc;
+        sincospi(alpha * 0.5, &s, &c);
+        gs = s; gc = c;  // global phase e^{i pi a/2} = gc + i gs
+        re[0] = gc * c; im[0] = gs * c;
+        re[3] = gc * c; im[3] = gs * c;
+        if (kind == kXPhase) {
+            // (gc + i gs)(-i s) = gs s - i gc s
+            re[1] = gs * s; im[1] = -gc * s;
+            re[2] = gs * s; im[2] = -gc * s;
+        } else {
+            re[1] = -gc * s; im[1] = -gs * s;
+            re[2] = gc * s; im[2] = gs * s;
+        }
+        break;
+    }
+    case kZZ: {
+        double s, c;
+        sincospi(alpha * 0.5, &s, &c);
+        re[0] = c; im[0] = -s;  // m
+        re[1] = c; im[1] = s;   // p
+        re[2] = c; im[2] = s;
+        re[3] = c; im[3] = -s;
+        break;
+    }
+    case kH:
+        re[0] = h; re[1] = h; re[2] = h; re[3] = -h;
+        break;
+    case kCX:  // [c][t_out][t_in] = (t_out == t_in ^ c)
+        re[0] = 1.0; re[3] = 1.0; re[5] = 1.0; re[6] = 1.0;
+        break;
+    case kZ:
+        re[0] = 1.0; re[1] = -1.0;
+        break;
+    case kM:
+        re[0] = 1.0; re[3] = 1.0;
+        break;
+    default:
+        break;
+    }
+}
+
+template <typename R>
+__device__ void build_gate(const int32_t* g, const double* params, uint32_t slice_bits,
+                           typename Cx<R>::V* lds, const typename Cx<R>::V* pool) {
+    using V = typename Cx<R>::V;
+    const int32_t kind = g[0];
+    const int32_t dst = g[1];
+    const int32_t n = g[2];
+    const uint32_t src_map = (uint32_t)g[3];
+    uint32_t shift = 0;
+    const int32_t sl = g[11];
+    const int32_t n_slice = sl & 0xff;
+    if (n_slice > 0) shift += ((slice_bits >> ((sl >> 8) & 0xff)) & 1u) * (uint32_t)g[12];
+    if (n_slice > 1) shift += ((slice_bits >> ((sl >> 16) & 0xff)) & 1u) * (uint32_t)g[13];
+    if (kind == kConst) {
+        for (int e = 0; e < n; ++e) lds[dst + e] = pool[g[10] + ((src_map >> (4 * e)) & 15u) + shift];
+        return;
+    }
+    double alpha = __hiloint2double(g[9], g[8]);
+    if (g[4] >= 0) alpha += (double)__int_as_float(g[5]) * params[g[4]];
+    if (g[6] >= 0) alpha += (double)__int_as_float(g[7]) * params[g[6]];
+    double re[8], im[8];
+    natural_gate(kind, alpha, re, im);
+    for (int e = 0; e < n; ++e) {
+        const uint32_t s = ((src_map >> (4 * e)) & 15u) + shift;
+        V v;
+        v.x = (R)re[s & 7];
+        v.y = (R)im[s & 7];
+        lds[dst + e] = v;
+    }
+}
+
+struct ExecArgs {
+    const int32_t* program;
+    const int32_t* items;
+    int32_t n_items;
+    int32_t lds_elems;
+    const double* params;
+    const void* pool;
+    void* scratch;
+    int64_t scratch_elems;
+    double* item_out;
+    uint32_t* queue;
+};
+
+template <int ITEM_THREADS>
+__device__ __forceinline__ void group_sync() {
+    if constexpr (ITEM_THREADS == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <typename V>
+__device__ __forceinline__ V load_loc(uint32_t loc, uint32_t off, const V* lds, const V* glb) {
+    return (loc & kGlobal) ? glb[(loc & ~kGlobal) + off] : lds[loc + off];
+}
+
+template <typename R, int ITEM_THREADS>
+__device__ void run_steps(const int32_t* st, int32_t n_steps, typename Cx<R>::V* lds,
+                          typename Cx<R>::V* glb, int lane) {
+    using V = typename Cx<R>::V;
+    for (int32_t s = 0; s < n_steps; ++s) {
+        const int32_t n_in = st[0];
+        const uint32_t rank_out = (uint32_t)st[1];
+        const int32_t flags = st[2];
+        const uint32_t out_loc = (uint32_t)st[3];
+        if (flags & 2) group_sync<ITEM_THREADS>();
+        const uint32_t n_sum = (flags & 1) ? 2u : 1u;
+        uint32_t loc[kExecMaxIn], top[kExecMaxIn], nr[kExecMaxIn], mask[kExecMaxIn];
+        uint32_t rs[kExecMaxIn][kExecMaxRuns], rw[kExecMaxIn][kExecMaxRuns],
+            rd[kExecMaxIn][kExecMaxRuns];
+#pragma unroll
+        for (int j = 0; j < kExecMaxIn; ++j) {
+            if (j < n_in) {
+                const int32_t* d = st + 4 + j * kInWords;
+                loc[j] = (uint32_t)d[0];
+                top[j] = (uint32_t)d[1];
+                nr[j] = (uint32_t)d[2];
+                mask[j] = (uint32_t)d[3];
+#pragma unroll
+                for (int r = 0; r < kExecMaxRuns; ++r) {
+                    const uint32_t word = (uint32_t)d[4 + r];
+                    rs[j][r] = word & 31u;
+                    rw[j][r] = (word >> 5) & 63u;
+                    rd[j][r] = (word >> 11) & 31u;
+                }
+            }
+        }
+        const uint32_t n_out = 1u << rank_out;
+        for (uint32_t o = lane; o < n_out; o += ITEM_THREADS) {
+            V acc;
+            acc.x = 0;
+            acc.y = 0;
+            for (uint32_t sv = 0; sv < n_sum; ++sv) {
+                V prod;
+                prod.x = 1;
+                prod.y = 0;
+#pragma unroll
+                for (int j = 0; j < kExecMaxIn; ++j) {
+                    if (j < n_in) {
+                        uint32_t off;
+                        if (nr[j] > (uint32_t)kExecMaxRuns) {
+                            off = pext_loop(o, mask[j]);
+                        } else {
+                            off = 0;
+#pragma unroll
+                            for (int r = 0; r < kExecMaxRuns; ++r)
+                                if ((uint32_t)r < nr[j]) off |= bfe(o, rs[j][r], rw[j][r]) << rd[j][r];
+                        }
+                        if (sv) off += top[j];
+                        const V v = load_loc(loc[j], off, lds, glb);
+                        prod = j == 0 ? v : cmul(prod, v);
+                    }
+                }
+                acc = sv ? cadd(acc, prod) : prod;
+            }
+            if (out_loc & kGlobal)
+                glb[(out_loc & ~kGlobal) + o] = acc;
+            else
+                lds[out_loc + o] = acc;
+        }
+        st += 4 + n_in * kInWords;
+    }
+}
+
+template <typename R, int ITEM_THREADS>
+__global__ __launch_bounds__(ITEM_THREADS < 256 ? 256 : ITEM_THREADS) void exec_kernel(const ExecArgs a) {
+    using V = typename Cx<R>::V;
+    constexpr int kWG = ITEM_THREADS < 256 ? 256 : ITEM_THREADS;
+    constexpr int kGroups = kWG / ITEM_THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int32_t item_sh[kGroups];
+    const int group = threadIdx.x / ITEM_THREADS;
+    const int lane = threadIdx.x % ITEM_THREADS;
+    V* lds = reinterpret_cast<V*>(smem_raw) + (size_t)group * a.lds_elems;
+    const int64_t slot = (int64_t)blockIdx.x * kGroups + group;
+    V* glb = reinterpret_cast<V*>(a.scratch) + slot * a.scratch_elems;
+    const V* pool = reinterpret_cast<const V*>(a.pool);
+    for (;;) {
+        if (lane == 0) item_sh[group] = (int32_t)atomicAdd(a.queue, 1u);
+        group_sync<ITEM_THREADS>();
+        const int32_t item = item_sh[group];
+        if (item >= a.n_items) break;
+        const int32_t* it = a.items + (size_t)item * QB_ITEM_WORDS;
+        const int32_t* plan = a.program + it[0];
+        const double* params = a.params + it[1];
+        const uint32_t slice_bits = (uint32_t)it[2];
+        const int32_t n_gates = plan[0];
+        const int32_t* gates = plan + plan[2];
+        for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
+            build_gate<R>(gates + (size_t)g * kGateWords, params, slice_bits, lds, pool);
+        group_sync<ITEM_THREADS>();
+        run_steps<R, ITEM_THREADS>(plan + plan[3], plan[1], lds, glb, lane);
+        group_sync<ITEM_THREADS>();
+        if (lane == 0) {
+            const V r = load_loc((uint32_t)plan[4], 0u, lds, glb);
+            a.item_out[2 * (size_t)item] = (double)r.x;
+            a.item_out[2 * (size_t)item + 1] = (double)r.y;
+        }
+        // item_sh is rewritten only after every lane has passed this point
+        group_sync<ITEM_THREADS>();
+    }
+}
+
+__global__ void zero_u32(uint32_t* p) { *p = 0u; }
+
+// Neumaier-compensated FP64 complex sums: one thread per segment, then one
+// block for the total (segment count is small: edges of one batch).
+__device__ __forceinline__ void neumaier(double& s, double& c, double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+}
+
+__global__ void reduce_segments(const double* item_out, const int32_t* seg, int32_t n_seg,
+                                double* term_out) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_seg) return;
+    double sr = 0, cr = 0, si = 0, ci = 0;
+    for (int32_t i = seg[s]; i < seg[s + 1]; ++i) {
+        neumaier(sr, cr, item_out[2 * (size_t)i]);
+        neumaier(si, ci, item_out[2 * (size_t)i + 1]);
+    }
+    term_out[2 * (size_t)s] = sr + cr;
+    term_out[2 * (size_t)s + 1] = si + ci;
+}
+
+__global__ __launch_bounds__(256) void reduce_total(const double* term_out, int32_t n_seg, double* total) {
+    __shared__ double sh[4][256];
+    double sr = 0, cr = 0, si = 0, ci = 0;
+    for (int32_t s = threadIdx.x; s < n_seg; s += blockDim.x) {
+        neumaier(sr, cr, term_out[2 * (size_t)s]);
+        neumaier(si, ci, term_out[2 * (size_t)s + 1]);
+    }
+    sh[0][threadIdx.x] = sr;
+    sh[1][threadIdx.x] = cr;
+    sh[2][threadIdx.x] = si;
+    sh[3][threadIdx.x] = ci;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ar = 0, acr = 0, ai = 0, aci = 0;
+        for (int t = 0; t < 256; ++t) {
+            neumaier(ar, acr, sh[0][t]);
+            acr += sh[1][t];
+            neumaier(ai, aci, sh[2][t]);
+            aci += sh[3][t];
+        }
+        total[0] = ar + acr;
+        total[1] = ai + aci;
+    }
+}
+
+template <typename R, int T>
+int exec_occupancy(int lds_bytes, int* blocks) {
+    constexpr int kWG = T < 256 ? 256 : T;
+    QB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, exec_kernel<R, T>, kWG, lds_bytes));
+    return QB_OK;
+}
+
+template <typename R>
+int exec_occupancy_dispatch(int item_threads, int lds_bytes, int* blocks) {
+    switch (item_threads) {
+    case 64: return exec_occupancy<R, 64>(lds_bytes, blocks);
+    case 128: return exec_occupancy<R, 128>(lds_bytes, blocks);
+    case 256: return exec_occupancy<R, 256>(lds_bytes, blocks);
+    case 512: return exec_occupancy<R, 512>(lds_bytes, blocks);
+    case 1024: return exec_occupancy<R, 1024>(lds_bytes, blocks);
+    default: return set_error(QB_ERR_INVALID, "item_threads %d not in {64,128,256,512,1024}", item_threads);
+    }
+}
+
+template <typename R, int T>
+int exec_launch(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
+    constexpr int kWG = T < 256 ? 256 : T;
+    hipLaunchKernelGGL((exec_kernel<R, T>), dim3(grid), dim3(kWG), lds_bytes, st, a);
+    QB_HIP(hipGetLastError());
+    return QB_OK;
+}
+
+template <typename R>
+int exec_launch_dispatch(int item_threads, const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
+    switch (item_threads) {
+    case 64: return exec_launch<R, 64>(a, grid, lds_bytes, st);
+    case 128: return exec_launch<R, 128>(a, grid, lds_bytes, st);
+    case 256: return exec_launch<R, 256>(a, grid, lds_bytes, st);
+    case 512: return exec_launch<R, 512>(a, grid, lds_bytes, st);
+    case 1024: return exec_launch<R, 1024>(a, grid, lds_bytes, st);
+    default: return set_error(QB_ERR_INVALID, "item_threads %d not in {64,128,256,512,1024}", item_threads);
+    }
+}
+
+int slots_for(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n_slots, int* grid_per_cu) {
+    if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
+    if (lds_elems < 0) return set_error(QB_ERR_INVALID, "lds_elems < 0");
+    const int elem = dtype == QB_C128 ? 16 : 8;
+    const int groups = item_threads < 256 ? 256 / item_threads : 1;
+    const int64_t lds_bytes = (int64_t)lds_elems * elem * groups;
+    if (lds_bytes > 160 * 1024)
+        return set_error(QB_ERR_UNSUPPORTED, "LDS request %lld bytes exceeds 160 KiB", (long long)lds_bytes);
+    int blocks = 0;
+    int rc = dtype == QB_C128 ? exec_occupancy_dispatch<double>(item_threads, (int)lds_bytes, &blocks)
+                              : exec_occupancy_dispatch<float>(item_threads, (int)lds_bytes, &blocks);
+    if (rc) return rc;
+    if (blocks < 1) return set_error(QB_ERR_UNSUPPORTED, "executor shape does not fit on a CU");
+    *grid_per_cu = blocks;
+    *n_slots = blocks * g_num_cus * groups;
+    return QB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qb_version(void) { return 100; }
+
+const char* qb_last_error(void) { return g_last_error.c_str(); }
+
+int qb_init(int device) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    int n = 0;
+    QB_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_error(QB_ERR_INVALID, "device %d of %d", device, n);
+    QB_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    QB_HIP(hipGetDeviceProperties(&prop, device));
+    g_num_cus = prop.multiProcessorCount;
+    g_device = device;
+    return QB_OK;
+}
+
+int qb_shutdown(void) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    g_device = -1;
+    return QB_OK;
+}
+
+int qb_step(const qb_step_desc* d, const void* const* inputs, void* out, void* stream) {
+    if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
+    if (!d || !inputs || !out) return set_error(QB_ERR_INVALID, "null descriptor or pointer");
+    if (d->n_inputs < 1 || d->n_inputs > QB_MAX_STEP_INPUTS)
+        return set_error(QB_ERR_INVALID, "n_inputs=%d outside [1,%d]", d->n_inputs, QB_MAX_STEP_INPUTS);
+    if (d->rank_u < 0 || d->rank_u > QB_MAX_RANK)
+        return set_error(QB_ERR_UNSUPPORTED, "rank_u=%d outside [0,%d]", d->rank_u, QB_MAX_RANK);
+    if (d->sum && d->rank_u < 1) return set_error(QB_ERR_INVALID, "sum over an empty union");
+    if (d->dtype != QB_C64 && d->dtype != QB_C128) return set_error(QB_ERR_INVALID, "dtype %d", d->dtype);
+    const uint32_t full = d->rank_u >= 32 ? 0xffffffffu : ((1u << d->rank_u) - 1u);
+    StepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n_in = (uint32_t)d->n_inputs;
+    a.n_sum = d->sum ? 2u : 1u;
+    a.rank_out = (uint32_t)(d->rank_u - (d->sum ? 1 : 0));
+    a.out = out;
+    const uint32_t top_bit = d->sum ? (1u << (d->rank_u - 1)) : 0u;
+    for (int j = 0; j < d->n_inputs; ++j) {
+        const uint32_t m = d->masks[j];
+        if (m & ~full) return set_error(QB_ERR_INVALID, "mask %d has bits outside rank %d", j, d->rank_u);
+        if (!inputs[j]) return set_error(QB_ERR_INVALID, "input %d is null", j);
+        StepInput& in = a.in[j];
+        in.ptr = inputs[j];
+        const uint32_t low = m & ~top_bit;
+        in.mask = low;
+        in.top = (m & top_bit) ? (1u << __builtin_popcount(low)) : 0u;
+        in.n_runs = (uint32_t)mask_runs(low, in.src, in.w, in.dst, kStepMaxRuns);
+    }
+    const uint64_t n_out = 1ull << a.rank_out;
+    const uint64_t want = (n_out + 255) / 256;
+    const uint64_t cap = (uint64_t)g_num_cus * 8;
+    const int grid = (int)(want < cap ? (want > 0 ? want : 1) : cap);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (d->dtype == QB_C128)
+        hipLaunchKernelGGL(step_kernel<double>, dim3(grid), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(step_kernel<float>, dim3(grid), dim3(256), 0, st, a);
+    QB_HIP(hipGetLastError());
+    return QB_OK;
+}
+
+int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n_slots) {
+    if (!n_slots) return set_error(QB_ERR_INVALID, "n_slots is null");
+    int per_cu = 0;
+    return slots_for(item_threads, lds_elems, dtype, n_slots, &per_cu);
+}
+
+int qb_execute(const qb_exec_desc* d, void* stream) {
+    if (!d) return set_error(QB_ERR_INVALID, "null descriptor");
+    if (d->dtype != QB_C64 && d->dtype != QB_C128) return set_error(QB_ERR_INVALID, "dtype %d", d->dtype);
+    if (d->n_items < 0) return set_error(QB_ERR_INVALID, "n_items < 0");
+    if (!d->program || !d->items || !d->params || !d->item_out || !d->queue)
+        return set_error(QB_ERR_INVALID, "null device pointer in descriptor");
+    int32_t n_slots = 0;
+    int per_cu = 0;
+    int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, &n_slots, &per_cu);
+    if (rc) return rc;
+    const int groups = d->item_threads < 256 ? 256 / d->item_threads : 1;
+    int grid = per_cu * g_num_cus;
+    const int need = (d->n_items + groups - 1) / groups;
+    if (need < grid) grid = need;
+    if (grid < 1) grid = 1;
+    if ((int64_t)grid * groups > d->n_slots && d->scratch_elems > 0)
+        return set_error(QB_ERR_INVALID, "scratch holds %d slots, launch needs %d", d->n_slots, grid * groups);
+    if (d->scratch_elems > 0 && !d->scratch) return set_error(QB_ERR_INVALID, "scratch is null");
+    ExecArgs a;
+    a.program = d->program;
+    a.items = d->items;
+    a.n_items = d->n_items;
+    a.lds_elems = d->lds_elems;
+    a.params = d->params;
+    a.pool = d->const_pool;
+    a.scratch = d->scratch;
+    a.scratch_elems = d->scratch_elems;
+    a.item_out = d->item_out;
+    a.queue = d->queue;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, st, d->queue);
+    QB_HIP(hipGetLastError());
+    const int elem = d->dtype == QB_C128 ? 16 : 8;
+    const int lds_bytes = d->lds_elems * elem * groups;
+    return d->dtype == QB_C128 ? exec_launch_dispatch<double>(d->item_threads, a, grid, lds_bytes, st)
+                               : exec_launch_dispatch<float>(d->item_threads, a, grid, lds_bytes, st);
+}
+
+int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segments,
+                    double* term_out, double* total, void* stream) {
+    if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
+    if (!item_out || !seg || !term_out || !total) return set_error(QB_ERR_INVALID, "null pointer");
+    if (n_segments < 1 || n_segments > (1 << 20))
+        return set_error(QB_ERR_INVALID, "n_segments=%d outside [1, 2^20]", n_segments);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(reduce_segments, dim3((n_segments + 255) / 256), dim3(256), 0, st, item_out, seg,
+                       n_segments, term_out);
+    QB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_total, dim3(1), dim3(256), 0, st, term_out, n_segments, total);
+    QB_HIP(hipGetLastError());
+    return QB_OK;
+}
+
+}  // extern "C"

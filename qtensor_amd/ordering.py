@@ -1,0 +1,235 @@
+"""Elimination-order optimizers (host side).
+
+* ``greedy_peo``         -- min-degree greedy ordering: repeatedly eliminate the
+  node with the fewest distinct neighbours (first such node in graph order)
+  and join its neighbours into a clique (``qtensor/utils.py:12-44,77-86``).
+  The treewidth reported is the largest neighbour count at elimination.
+* ``OrderingOptimizer``  -- ``qtensor/optimisation/Optimizer.py:46-88``:
+  order the line graph without bra/ket variables, then prepend ket+bra.
+* ``SlicesOptimizer``    -- ``Optimizer.py:90-159`` with ``GreedyParvars``
+  (``qtensor/optimisation/Greedy.py:78-92``): remove the highest-degree
+  variable until the greedy treewidth drops below ``max_tw - tw_bias``; the
+  removed variables are sliced and go last in the order.
+* ``TamakiOptimizer``    -- ``Optimizer.py:161-176``: PACE-2017 heuristic via an
+  external ``tw-heuristic`` executable (Java), used when it is on ``PATH``.
+"""
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+
+from .model import Var
+
+
+def _adjacency(graph):
+    return {n: set(graph[n]) - {n} for n in graph.nodes()}
+
+
+def greedy_peo(graph):
+    """Return ``(peo, degrees)`` of the min-degree greedy elimination."""
+    adj = _adjacency(graph)
+    peo, degs = [], []
+    while adj:
+        best = min(adj, key=lambda n: len(adj[n]))
+        nb = adj.pop(best)
+        peo.append(best)
+        degs.append(len(nb))
+        for u in nb:
+            s = adj[u]
+            s.discard(best)
+            s.update(nb)
+            s.discard(u)
+    return peo, degs
+
+
+def elimination_widths(graph, peo):
+    """Neighbour count of each node when eliminated in ``peo`` order."""
+    adj = _adjacency(graph)
+    degs = []
+    for n in peo:
+        nb = adj.pop(n)
+        degs.append(len(nb))
+        for u in nb:
+            s = adj[u]
+            s.discard(n)
+            s.update(nb)
+            s.discard(u)
+    return degs
+
+
+def _vars(graph, ids):
+    return [Var(v, size=graph.nodes[v]["size"], name=graph.nodes[v]["name"]) for v in ids]
+
+
+def _with_clique(graph, nodes):
+    g = graph.copy()
+    ids = [int(v) for v in nodes]
+    for a in range(len(ids)):
+        for b in range(a + 1, len(ids)):
+            g.add_edge(ids[a], ids[b])
+    return g
+
+
+class Optimizer:
+    def optimize(self, tensor_net):
+        raise NotImplementedError
+
+
+class OrderingOptimizer(Optimizer):
+    def _get_ordering_ints(self, graph):
+        return greedy_peo(graph)
+
+    def _get_ordering(self, graph):
+        peo, path = self._get_ordering_ints(graph)
+        return _vars(graph, peo), path
+
+    def optimize(self, tensor_net):
+        graph = tensor_net.get_line_graph()
+        free_vars = list(tensor_net.free_vars)
+        ignored = list(tensor_net.ket_vars) + list(tensor_net.bra_vars)
+        if free_vars:
+            graph = _with_clique(graph, free_vars)
+        self.graph = graph
+        peo, path = self._get_ordering(graph)
+        self.treewidth = max(path) if path else 0
+        if free_vars:
+            free_ids = set(int(v) for v in free_vars)
+            peo = [v for v in peo if int(v) not in free_ids] + free_vars
+        self.peo_ints = [int(v) for v in peo]
+        peo = ignored + peo
+        self.peo = peo
+        self.ignored_vars = ignored
+        return peo, tensor_net
+
+
+DefaultOptimizer = OrderingOptimizer
+
+
+class GreedyParvars:
+    """Pick slice variables greedily by highest degree; removes them from ``graph``."""
+
+    def __init__(self, graph):
+        self.graph = graph
+        self.result = []
+
+    def step(self):
+        nodes = list(self.graph.nodes)
+        if not nodes:
+            return 1
+        costs = [-self.graph.degree(n) for n in nodes]
+        best = nodes[int(np.argmin(costs))]
+        self.result.append(best)
+        self.graph.remove_node(best)
+        return 0
+
+
+class SlicesOptimizer(OrderingOptimizer):
+    def __init__(self, tw_bias=2, max_tw=None, **kwargs):
+        self.tw_bias = tw_bias
+        self.max_tw = kwargs.get("target_tw") or max_tw
+
+    def _get_max_tw(self):
+        if self.max_tw is not None:
+            return self.max_tw
+        import psutil
+        return int(np.log2(psutil.virtual_memory().available)) - 4
+
+    def _split_graph(self, p_graph, max_tw):
+        searcher = GreedyParvars(p_graph)
+        peo_ints, path = self._get_ordering_ints(p_graph)
+        while True:
+            if self.treewidth < max_tw:
+                break
+            if searcher.step():
+                raise MemoryError("Estimated OOM")
+            peo_ints, path = self._get_ordering_ints(p_graph)
+            self.treewidth = max(path) if path else 0
+        return peo_ints, searcher.result
+
+    def optimize(self, tensor_net):
+        super().optimize(tensor_net)
+        graph = self.graph
+        p_graph = graph.copy()
+        max_tw = self._get_max_tw() - self.tw_bias
+        peo_ints, par_ints = self._split_graph(p_graph, max_tw)
+        self.parallel_vars = _vars(graph, par_ints)
+        peo = _vars(graph, peo_ints)
+        free_vars = list(tensor_net.free_vars)
+        if free_vars:
+            free_ids = set(int(v) for v in free_vars)
+            peo = [v for v in peo if int(v) not in free_ids] + free_vars
+        self.peo = self.ignored_vars + peo + self.parallel_vars
+        return self.peo, self.parallel_vars, tensor_net
+
+
+class TamakiOptimizer(OrderingOptimizer):
+    """Ordering from the PACE-2017 ``tw-heuristic`` solver run for ``wait_time`` s."""
+
+    def __init__(self, *args, wait_time=5, **kwargs):
+        self.wait_time = wait_time
+
+    def _get_ordering_ints(self, graph):
+        exe = shutil.which("tw-heuristic")
+        if exe is None:
+            raise RuntimeError("TamakiOptimizer needs the 'tw-heuristic' executable on PATH")
+        nodes = list(graph.nodes())
+        pos = {n: k + 1 for k, n in enumerate(nodes)}
+        lines = [f"p tw {len(nodes)} {graph.number_of_edges()}"]
+        lines += [f"{pos[a]} {pos[b]}" for a, b in graph.edges() if a != b]
+        with tempfile.NamedTemporaryFile("w", suffix=".gr", delete=False) as f:
+            f.write("\n".join(lines) + "\n")
+            path = f.name
+        try:
+            with open(path) as fin:
+                proc = subprocess.Popen([exe], stdin=fin, stdout=subprocess.PIPE, text=True)
+                try:
+                    out, _ = proc.communicate(timeout=self.wait_time)
+                except subprocess.TimeoutExpired:
+                    proc.terminate()
+                    out, _ = proc.communicate()
+        finally:
+            os.unlink(path)
+        return _peo_from_td(out, nodes, graph)
+
+
+def _peo_from_td(text, nodes, graph):
+    """Elimination order from a PACE ``.td`` tree decomposition: eliminate
+    leaves of the decomposition tree first, each bag's vertices not shared
+    with its parent."""
+    bags, tree = {}, {}
+    for line in text.splitlines():
+        parts = line.split()
+        if not parts or parts[0] == "c":
+            continue
+        if parts[0] == "s":
+            continue
+        if parts[0] == "b":
+            bags[int(parts[1])] = [nodes[int(x) - 1] for x in parts[2:]]
+        else:
+            a, b = int(parts[0]), int(parts[1])
+            tree.setdefault(a, set()).add(b)
+            tree.setdefault(b, set()).add(a)
+    if not bags:
+        raise RuntimeError("tw-heuristic returned no decomposition")
+    root = next(iter(bags))
+    order, seen, stack = [], {root}, [(root, None)]
+    parent = {}
+    while stack:
+        node, par = stack.pop()
+        parent[node] = par
+        order.append(node)
+        for nb in tree.get(node, ()):
+            if nb not in seen:
+                seen.add(nb)
+                stack.append((nb, node))
+    peo, done = [], set()
+    for b in reversed(order):
+        keep = set(bags[parent[b]]) if parent[b] is not None else set()
+        for v in bags[b]:
+            if v not in keep and v not in done:
+                peo.append(v)
+                done.add(v)
+    peo += [n for n in nodes if n not in done]
+    return peo, elimination_widths(graph, peo)

@@ -1,0 +1,111 @@
+"""Batched persistent executor against the oracle and the statevector."""
+import networkx as nx
+import numpy as np
+import pytest
+
+from oracle import closed_form, statevector
+from oracle import np_framework as npf
+from qtensor_amd.batched import LaunchShape, LightconeEngine
+from qtensor_amd.composer import QtreeQAOAComposer, SimpZZQtreeComposer, ZZQtreeQAOAComposer
+from qtensor_amd.simulator import QAOAQtreeSimulator
+
+pytestmark = pytest.mark.gpu
+
+
+def _tol(ref):
+    return 1e-5 * max(1.0, abs(ref))
+
+
+def test_config1_batched(gpu):
+    G = nx.random_regular_graph(3, 10, seed=42)
+    g, b = [np.pi / 3], [np.pi / 2]
+    C, zz = LightconeEngine().energy(G, g, b)
+    for e, z in zip(G.edges(), zz):
+        ref = closed_form.zz_p1(G, e, g[0], b[0])
+        assert abs(z - ref) <= _tol(ref)
+    assert abs(C - statevector.maxcut_energy(G, g, b)) < 1e-4
+
+
+SHAPES = [LaunchShape(64, 5120, 2048), LaunchShape(64, 5120, 8), LaunchShape(256, 10240, 4096),
+          LaunchShape(128, 4096, 64), LaunchShape(512, 16384, 16384)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"t{s.item_threads}_c{s.lds_capacity}_l{s.lds_tensor_limit}")
+@pytest.mark.parametrize("Comp", [QtreeQAOAComposer, SimpZZQtreeComposer])
+def test_p2_vs_statevector(gpu, shape, Comp):
+    G = nx.random_regular_graph(3, 16, seed=3)
+    rng = np.random.default_rng(5)
+    g, b = list(rng.uniform(0, np.pi, 2)), list(rng.uniform(0, np.pi / 2, 2))
+    C, zz = LightconeEngine(composer=Comp, shape=shape).energy(G, g, b)
+    sv = statevector.zz_expectations(G, g, b)
+    for e, z in zip(G.edges(), zz):
+        assert abs(z - sv[e]) <= _tol(sv[e])
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex64", 1e-5), ("complex128", 1e-11)])
+def test_p3_vs_oracle(gpu, dtype, tol):
+    G = nx.random_regular_graph(3, 64, seed=1)
+    rng = np.random.default_rng(2)
+    g, b = list(rng.uniform(0, np.pi, 3)), list(rng.uniform(0, np.pi / 2, 3))
+    edges = list(G.edges())[:6]
+    eng = LightconeEngine(dtype=dtype, shape=LaunchShape(256, 8192, 1024))
+    _, zz = eng.energy(G, g, b, edges=edges)
+    sim = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend())
+    for e, z in zip(edges, zz):
+        ref = sim._get_edge_energy(G, g, b, e)[0]
+        assert abs(z - ref) <= tol * max(1.0, abs(ref))
+
+
+def test_sliced_items_sum_to_unsliced(gpu):
+    G = nx.random_regular_graph(3, 40, seed=7)
+    g, b = [0.3, 0.7, 0.2], [0.5, 0.1, 0.9]
+    edges = list(G.edges())[:4]
+    shape = LaunchShape(256, 8192, 512)
+    _, ref = LightconeEngine(shape=shape, dtype="complex128").energy(G, g, b, edges=edges)
+    eng = LightconeEngine(shape=shape, max_tw=9, dtype="complex128")
+    wl = eng.workload(G, 3, edges)
+    assert any(w.n_slices > 1 for w in wl.works)
+    batch = eng.batch([(wl, g, b)])
+    batch.run()
+    for z, r in zip(batch.terms()[0], ref):
+        assert abs(z - r) < 1e-11
+
+
+def test_reparameterise_and_graph_replay(gpu):
+    import torch
+    G = nx.random_regular_graph(3, 24, seed=9)
+    eng = LightconeEngine()
+    wl = eng.workload(G, 2)
+    g0, b0 = [0.1, 0.2], [0.3, 0.4]
+    batch = eng.batch([(wl, g0, b0)])
+    batch.capture()
+    for k in range(3):
+        rng = np.random.default_rng(k)
+        g, b = list(rng.uniform(0, np.pi, 2)), list(rng.uniform(0, np.pi / 2, 2))
+        batch.set_params([(g, b)])
+        batch.run()
+        sv = statevector.zz_expectations(G, g, b) if G.number_of_nodes() <= 20 else None
+        _, eager = LightconeEngine().energy(G, g, b)
+        torch.cuda.synchronize()
+        for z, r in zip(batch.terms()[0], eager):
+            assert abs(z - r) < 1e-6
+        if sv:
+            for e, z in zip(G.edges(), batch.terms()[0]):
+                assert abs(z - sv[e]) < 1e-5
+
+
+def test_mixed_requests_match_alone(gpu):
+    """Each request's terms are the same alone and inside a mixed batch."""
+    eng = LightconeEngine()
+    reqs = []
+    for k, (n, seed) in enumerate([(12, 1), (20, 2), (16, 3)]):
+        G = nx.random_regular_graph(3, n, seed=seed)
+        rng = np.random.default_rng(10 + k)
+        reqs.append((eng.workload(G, 2), list(rng.uniform(0, np.pi, 2)), list(rng.uniform(0, np.pi / 2, 2))))
+    mixed = eng.batch(reqs)
+    mixed.run()
+    for r, terms in zip(reqs, mixed.terms()):
+        alone = eng.batch([r])
+        alone.run()
+        for a, m in zip(alone.terms()[0], terms):
+            assert abs(a - m) <= 1e-12

@@ -1,0 +1,173 @@
+"""Host-side logic: composers, ordering, canonical lightcones, the plan compiler
+(through a numpy interpreter of the device program) and the C-ABI library
+surface.  CPU only."""
+import ctypes
+import os
+import pickle
+import re
+
+import networkx as nx
+import numpy as np
+import pytest
+
+from oracle import statevector
+from oracle import np_framework as npf
+from plan_interp import run_plan
+from qtensor_amd import _lib
+from qtensor_amd.composer import (QtreeQAOAComposer, SimpZZQtreeComposer, ZZQtreeQAOAComposer,
+                                  get_edge_subgraph, simplify_circuit)
+from qtensor_amd.lightcone import canonical_graph, canonical_lightcone
+from qtensor_amd.ordering import OrderingOptimizer, SlicesOptimizer, elimination_widths, greedy_peo
+from qtensor_amd.plan import PlanCompiler, mask_runs
+from qtensor_amd.simulator import QAOAQtreeSimulator, QtreeSimulator
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_greedy_peo_path_and_clique():
+    path = nx.path_graph(6)
+    peo, degs = greedy_peo(path)
+    assert max(degs) == 1 and sorted(peo) == list(range(6))
+    K = nx.complete_graph(5)
+    assert max(greedy_peo(K)[1]) == 4
+    g = nx.random_regular_graph(3, 20, seed=1)
+    peo, degs = greedy_peo(g)
+    assert elimination_widths(g, peo) == degs
+
+
+def test_greedy_tie_break_first_in_order():
+    g = nx.Graph()
+    g.add_nodes_from([5, 3, 9])
+    g.add_edges_from([(5, 3), (3, 9)])
+    peo, _ = greedy_peo(g)
+    assert peo[0] == 5
+
+
+def test_canonical_lightcone_invariant_under_relabel():
+    G = nx.random_regular_graph(3, 60, seed=5)
+    perm = np.random.default_rng(0).permutation(60)
+    H = nx.relabel_nodes(G, {v: int(perm[v]) for v in G.nodes()})
+    for e in list(G.edges())[:20]:
+        k1 = canonical_lightcone(G, e, 2)[0]
+        k2 = canonical_lightcone(H, (int(perm[e[0]]), int(perm[e[1]])), 2)[0]
+        k3 = canonical_lightcone(G, (e[1], e[0]), 2)[0]
+        assert k1 == k2 == k3
+
+
+def test_canonical_graph_is_isomorphic_to_lightcone():
+    G = nx.random_regular_graph(3, 30, seed=2)
+    for e in list(G.edges())[:10]:
+        key, n, edges, i, j = canonical_lightcone(G, e, 2)
+        sub = get_edge_subgraph(G, e, 2)
+        assert nx.is_isomorphic(sub, canonical_graph(n, edges))
+
+
+def test_simplify_preserves_energy():
+    G = nx.random_regular_graph(3, 10, seed=3)
+    g, b = [0.3, 0.1], [0.2, 0.4]
+    plain = QAOAQtreeSimulator(ZZQtreeQAOAComposer, bucket_backend=npf.NumpyBackend())
+    simp = QAOAQtreeSimulator(SimpZZQtreeComposer, bucket_backend=npf.NumpyBackend())
+    for e in list(G.edges())[:4]:
+        c1 = ZZQtreeQAOAComposer(G, gamma=g, beta=b)
+        c1.energy_expectation_lightcone(e)
+        c2 = SimpZZQtreeComposer(G, gamma=g, beta=b)
+        c2.energy_expectation_lightcone(e)
+        assert len(c2.circuit) < len(c1.circuit)
+        assert abs(plain._get_edge_energy(G, g, b, e)[0] - simp._get_edge_energy(G, g, b, e)[0]) < 1e-10
+
+
+def test_max_tw_guard():
+    G = nx.random_regular_graph(3, 10, seed=1)
+    comp = QtreeQAOAComposer(G, gamma=[0.1, 0.2], beta=[0.3, 0.1])
+    comp.ansatz_state()
+    with pytest.raises(ValueError):
+        QtreeSimulator(bucket_backend=npf.NumpyBackend(), max_tw=2).simulate(comp.circuit)
+
+
+def test_mask_runs():
+    assert mask_runs(0b1101100) == [(2, 2, 0), (5, 2, 2)]
+    for m in (0, 1, 0xFFFF, 0xA5A5):
+        x = 0x1234 & 0xFFFF
+        off = 0
+        for s, w, d in mask_runs(m):
+            off |= ((x >> s) & ((1 << w) - 1)) << d
+        ref, bit, mm = 0, 1, m
+        while mm:
+            low = mm & -mm
+            if x & low:
+                ref |= bit
+            bit <<= 1
+            mm &= mm - 1
+        assert off == ref
+
+
+def _edge_plan(G, e, p, Comp=QtreeQAOAComposer, **kw):
+    key, n, ce, i, j = canonical_lightcone(G, e, p)
+    comp = Comp(canonical_graph(n, ce), gamma=[0.0] * p, beta=[0.0] * p)
+    comp.energy_expectation(i, j)
+    opt = kw.pop("optimizer", None) or OrderingOptimizer()
+    return PlanCompiler(**kw).compile_circuit(comp.circuit, opt)
+
+
+@pytest.mark.parametrize("Comp", [QtreeQAOAComposer, ZZQtreeQAOAComposer, SimpZZQtreeComposer])
+@pytest.mark.parametrize("p,lim,cap", [(1, 4, None), (2, 1 << 20, 4096), (2, 16, 2048)])
+def test_plan_program_vs_oracle(Comp, p, lim, cap):
+    G = nx.random_regular_graph(3, 16, seed=p)
+    rng = np.random.default_rng(p)
+    g, b = list(rng.uniform(0, np.pi, p)), list(rng.uniform(0, np.pi / 2, p))
+    sv = statevector.zz_expectations(G, g, b)
+    for e in list(G.edges())[:4]:
+        plan = _edge_plan(G, e, p, Comp, lds_tensor_limit=lim, lds_capacity=cap)
+        assert abs(run_plan(plan.words, g + b) - sv[e]) < 1e-10
+        if cap is not None:
+            assert plan.lds_need <= cap
+
+
+def test_plan_p3_and_slices_vs_oracle():
+    G = nx.random_regular_graph(3, 40, seed=1)
+    g, b = [0.3, 0.7, 0.2], [0.5, 0.1, 0.9]
+    sim = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend())
+    e = list(G.edges())[0]
+    ref = sim._get_edge_energy(G, g, b, e)[0]
+    plan = _edge_plan(G, e, 3, lds_tensor_limit=256, lds_capacity=4096)
+    assert abs(run_plan(plan.words, g + b) - ref) < 1e-10
+    sl = _edge_plan(G, e, 3, lds_tensor_limit=256, lds_capacity=4096,
+                    optimizer=SlicesOptimizer(max_tw=plan.treewidth - 2, tw_bias=0))
+    assert sl.n_sliced >= 1 and sl.max_rank < plan.max_rank
+    total = sum(run_plan(sl.words, g + b, slice_bits=s) for s in range(1 << sl.n_sliced))
+    assert abs(total - ref) < 1e-10
+
+
+def test_plan_reparameterise_without_recompile():
+    G = nx.random_regular_graph(3, 12, seed=4)
+    e = list(G.edges())[2]
+    plan = _edge_plan(G, e, 2, lds_tensor_limit=64)
+    for seed in range(3):
+        rng = np.random.default_rng(seed)
+        g, b = list(rng.uniform(0, np.pi, 2)), list(rng.uniform(0, np.pi / 2, 2))
+        assert abs(run_plan(plan.words, g + b) - statevector.zz_expectations(G, g, b)[e]) < 1e-10
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "qtensor_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(qb_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_header_symbols():
+    lib = _lib.load()
+    syms = _header_symbols()
+    assert syms and set(syms) == set(_lib.EXPORTED)
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert lib.qb_version() >= 100
+    # calls that need no device report "not initialised" cleanly
+    n = ctypes.c_int32(0)
+    assert lib.qb_exec_slots(64, 16, 0, ctypes.byref(n)) == _lib.QB_ERR_NOT_INIT
+    assert b"qb_init" in lib.qb_last_error()
+
+
+def test_backend_pickles_without_device_state():
+    from qtensor_amd.backend import HipBackend
+    be = HipBackend()
+    be2 = pickle.loads(pickle.dumps(be))
+    assert be2.dtype == "complex64" and not be2._ready

@@ -1,0 +1,175 @@
+"""The oracle is pinned before it is trusted: golden fixtures F1-F4 from the
+reference's notebooks/tests, brute force on tiny buckets, an independent
+statevector and the p=1 closed form.  CPU only."""
+import json
+import os
+
+import networkx as nx
+import numpy as np
+import pytest
+
+from oracle import closed_form, statevector
+from oracle import np_framework as npf
+from qtensor_amd.circuit import (bucket_elimination, circ2buckets, reorder_buckets,
+                                 slice_from_bits)
+from qtensor_amd.composer import (QtreeQAOAComposer, SimpZZQtreeComposer, ZZQtreeQAOAComposer)
+from qtensor_amd.gates import H
+from qtensor_amd.model import Tensor, Var
+from qtensor_amd.ordering import SlicesOptimizer
+from qtensor_amd.simulator import FeynmanSimulator, QAOAQtreeSimulator, QtreeSimulator
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixtures.json")))
+
+
+def _graph(edges):
+    G = nx.Graph()
+    G.add_edges_from([tuple(e) for e in edges])
+    return G
+
+
+def test_f1_amplitude():
+    f = FIX["F1"]
+    G = _graph(f["edges"])
+    comp = QtreeQAOAComposer(G, gamma=f["gamma"], beta=f["beta"])
+    comp.ansatz_state()
+    amp = QtreeSimulator(bucket_backend=npf.NumpyBackend()).simulate(comp.circuit)[0]
+    ref = complex(*f["amplitude"])
+    assert abs(amp - ref) < 1e-7
+    sv = statevector.qaoa_state(4, [tuple(e) for e in G.edges()], f["gamma"], f["beta"])
+    assert abs(sv[0] - ref) < 1e-7
+
+
+@pytest.mark.parametrize("Comp", [QtreeQAOAComposer, ZZQtreeQAOAComposer, SimpZZQtreeComposer])
+def test_f2_energy(Comp):
+    f = FIX["F2"]
+    G = _graph(f["edges"])
+    E = QAOAQtreeSimulator(Comp, bucket_backend=npf.NumpyBackend()).energy_expectation(
+        G, f["gamma"], f["beta"])
+    assert abs(E[0] - f["energy"]) < 1e-8
+    assert abs(statevector.maxcut_energy(G, f["gamma"], f["beta"]) - f["energy"]) < 1e-8
+
+
+class _MyGate(H):
+    name = "MyGate"
+
+
+def test_f3_hadamard_network():
+    f = FIX["F3"]
+    buckets, data_dict, bra, ket = circ2buckets(1, [[_MyGate(0)]])
+    assert str(buckets) == f["buckets"]
+    be = npf.NumpyBackend()
+    full = bucket_elimination(be.get_sliced_buckets(buckets, data_dict, {}), be.process_bucket)
+    assert abs(full.data - f["full"]) < 1e-12
+    sd = slice_from_bits(0, ket)
+    r = bucket_elimination(be.get_sliced_buckets(buckets, data_dict, sd), be.process_bucket)
+    assert abs(r.data - f["ket_sliced"]) < 1e-12
+    sd.update(slice_from_bits(0, bra))
+    r = bucket_elimination(be.get_sliced_buckets(buckets, data_dict, sd), be.process_bucket)
+    assert abs(r.data - f["amplitude"]) < 1e-15
+    # partial contraction keeping the output wire: reorder with bra last
+    all_vars = {v for b in buckets for t in b for v in t.indices}
+    peo = sorted(all_vars - set(bra), key=int) + list(bra)
+    perm_buckets, perm = reorder_buckets(buckets, peo)
+    kv = [perm[v] for v in ket]
+    sliced = be.get_sliced_buckets(perm_buckets, data_dict, slice_from_bits(0, kv))
+    r = bucket_elimination(sliced, be.process_bucket, n_var_nosum=1)
+    assert np.allclose(np.asarray(r.data).ravel(), f["partial"], atol=1e-8)
+
+
+def test_f4_qiskit_cut():
+    f = FIX["F4"]
+    G = _graph(f["edges"])
+    gamma = [t / (2 * np.pi) for t in f["qiskit_rzz"]]
+    beta = [b / np.pi for b in f["qiskit_beta"]]
+    cut = statevector.maxcut_energy(G, gamma, beta)
+    assert abs(cut - f["cut"]) < f["tol"]
+    sim = QAOAQtreeSimulator(ZZQtreeQAOAComposer, bucket_backend=npf.NumpyBackend())
+    sv = statevector.zz_expectations(G, gamma, beta)
+    for e in list(G.edges())[:2]:
+        assert abs(sim._get_edge_energy(G, gamma, beta, e)[0] - sv[e]) < 1e-9
+
+
+def _rand_bucket(rng, n_vars=6, k=3, size1=()):
+    vs = [Var(i, size=(1 if i in size1 else 2)) for i in range(n_vars)]
+    bucket = []
+    for j in range(k):
+        ids = sorted({0} | set(rng.choice(np.arange(1, n_vars), size=rng.integers(0, n_vars),
+                                          replace=False).tolist()))
+        idx = [vs[i] for i in ids]
+        data = rng.normal(size=[v.size for v in idx]) + 1j * rng.normal(size=[v.size for v in idx])
+        bucket.append(Tensor(f"T{j}", idx, data=data))
+    return bucket
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("no_sum", [False, True])
+def test_process_bucket_vs_brute_force(seed, no_sum):
+    rng = np.random.default_rng(seed)
+    bucket = _rand_bucket(rng, n_vars=5, k=int(rng.integers(1, 5)),
+                          size1=(3,) if seed % 2 else ())
+    got = npf.process_bucket(bucket, no_sum=no_sum)
+    ref = npf.brute_force_bucket(bucket, no_sum=no_sum)
+    assert np.allclose(np.asarray(got.data), ref, atol=1e-12)
+    ids = sorted({int(v) for t in bucket for v in t.indices})
+    assert [int(v) for v in got.indices] == (ids if no_sum else ids[1:])
+    assert got.name == f"E{ids[0]}"
+
+
+@pytest.mark.parametrize("d,n,seed", [(3, 10, 1), (4, 9, 2), (5, 8, 3)])
+def test_closed_form_p1_vs_statevector(d, n, seed):
+    G = nx.random_regular_graph(d, n, seed=seed)
+    G.add_edge(0, 1)
+    rng = np.random.default_rng(seed)
+    for _ in range(2):
+        g, b = float(rng.uniform(0, np.pi)), float(rng.uniform(0, np.pi / 2))
+        sv = statevector.zz_expectations(G, [g], [b])
+        for e in G.edges():
+            assert abs(closed_form.zz_p1(G, e, g, b) - sv[e]) < 1e-10
+
+
+@pytest.mark.parametrize("Comp", [QtreeQAOAComposer, ZZQtreeQAOAComposer, SimpZZQtreeComposer])
+@pytest.mark.parametrize("p", [1, 2])
+def test_edge_terms_vs_statevector(Comp, p):
+    G = nx.random_regular_graph(3, 10, seed=7)
+    rng = np.random.default_rng(p)
+    g, b = list(rng.uniform(0, np.pi, p)), list(rng.uniform(0, np.pi / 2, p))
+    sim = QAOAQtreeSimulator(Comp, bucket_backend=npf.NumpyBackend())
+    E = sim.energy_expectation(G, g, b)
+    sv = statevector.zz_expectations(G, g, b)
+    for e, z in zip(G.edges(), sim.edge_terms):
+        assert abs(z - sv[e]) < 1e-10
+    assert abs(E[0] - statevector.maxcut_energy(G, g, b)) < 1e-9
+
+
+class FeynmanQAOASimulator(QAOAQtreeSimulator, FeynmanSimulator):
+    pass
+
+
+def test_sliced_equals_unsliced():
+    G = nx.random_regular_graph(3, 10, seed=11)
+    g, b = [np.pi / 5] * 2, [np.pi / 2] * 2
+    ref = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend()).energy_expectation(G, g, b)
+    sim = FeynmanQAOASimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend())
+    sim.optimizer = SlicesOptimizer(max_tw=7, tw_bias=0)
+    res = sim.energy_expectation(G, g, b)
+    assert np.isclose(res, ref, rtol=1e-10)
+    assert len(sim.optimizer.parallel_vars) >= 1
+
+
+def test_planted_mistake_is_caught(monkeypatch):
+    """A dropped term (the last tensor of each bucket replaced by ones) must break F2."""
+    f = FIX["F2"]
+    G = _graph(f["edges"])
+    right = npf.process_bucket
+
+    def wrong(bucket, no_sum=False, dtype=np.complex128):
+        if len(bucket) > 1:
+            last = bucket[-1]
+            bucket = list(bucket[:-1]) + [Tensor(last.name, last.indices,
+                                                 data=np.ones_like(np.asarray(last.data)))]
+        return right(bucket, no_sum=no_sum, dtype=dtype)
+
+    monkeypatch.setattr(npf, "process_bucket", wrong)
+    E = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend()).energy_expectation(
+        G, f["gamma"], f["beta"])
+    assert abs(E[0] - f["energy"]) > 1e-3
