@@ -24,7 +24,7 @@ from . import _lib
 from .composer import QtreeQAOAComposer
 from .lightcone import canonical_graph, canonical_lightcone
 from .ordering import OrderingOptimizer, SlicesOptimizer
-from .plan import PlanCompiler
+from .plan import PROGRAM_PAD, PlanCompiler
 
 
 def _torch():
@@ -35,7 +35,7 @@ def _torch():
 @dataclass
 class LaunchShape:
     item_threads: int = 64
-    lds_capacity: int = 5120          # elements per item (complex64: 40 KiB)
+    lds_capacity: int = 4096          # elements per item (complex64: 32 KiB)
     lds_tensor_limit: int = 2048      # largest intermediate kept in LDS
 
     def key(self):
@@ -43,8 +43,8 @@ class LaunchShape:
 
 
 DEFAULT_SHAPES = {
-    "small": LaunchShape(64, 5120, 2048),
-    "large": LaunchShape(256, 10240, 4096),
+    "small": LaunchShape(64, 4096, 2048),
+    "large": LaunchShape(64, 4096, 2048),
 }
 
 
@@ -118,11 +118,24 @@ class LightconeEngine:
         g = canonical_graph(n, edges)
         comp = self.composer(g, gamma=[0.0] * p, beta=[0.0] * p)
         comp.energy_expectation(i, j)
-        pc = PlanCompiler(lds_tensor_limit=shape.lds_tensor_limit, lds_capacity=shape.lds_capacity)
-        plan = pc.compile_circuit(comp.circuit, self._optimizer())
+        lim = shape.lds_tensor_limit
+        while True:
+            pc = PlanCompiler(lds_tensor_limit=lim, lds_capacity=shape.lds_capacity,
+                              lanes=shape.item_threads)
+            try:
+                plan = pc.compile_circuit(comp.circuit, self._optimizer())
+                break
+            except ValueError as exc:
+                # LDS arena too small: keep fewer tensors in LDS, spill the rest
+                if "LDS capacity" not in str(exc) or lim <= shape.item_threads:
+                    raise
+                lim //= 2
         if plan.lds_need > shape.lds_capacity:
             raise ValueError(f"plan needs {plan.lds_need} LDS elements (> {shape.lds_capacity}); "
                              "its gate tensors alone do not fit the launch shape")
+        if plan.lds_elems(16 if self.dtype == "complex128" else 8) * (16 if self.dtype == "complex128" else 8) \
+                > 160 * 1024:
+            raise ValueError("plan data plus staged step records exceed 160 KiB of LDS")
         with self._lock:
             self._plans[ck] = plan
         return plan
@@ -183,6 +196,7 @@ class Batch:
                     plan_off[ident] = pos
                     words.append(plan.words)
                     pos += plan.words.size
+        words.append(np.zeros(PROGRAM_PAD, dtype=np.int32))   # record prefetch reads ahead
         program = np.concatenate(words).astype(np.int32)
         # params, items, segments (one segment per edge term)
         params, items, seg = [], [], [0]
@@ -199,7 +213,7 @@ class Batch:
             start = len(seg) - 1
             for w in wl.works:
                 plan = wl.plans[w.key]
-                lds_need = max(lds_need, plan.lds_need)
+                lds_need = max(lds_need, plan.lds_elems(elem))
                 glb_need = max(glb_need, plan.glb_need)
                 self.bytes_step_model += plan.bytes_step_model * w.n_slices
                 self.flops += plan.flops * w.n_slices
@@ -226,9 +240,7 @@ class Batch:
         self.total = torch.zeros(2, dtype=torch.float64, device=dev)
         self.queue = torch.zeros(1, dtype=torch.int32, device=dev)
         self.pool = torch.zeros(2, dtype=torch.float64, device=dev)
-        it = self.shape.item_threads
-        groups = 256 // it if it < 256 else 1
-        launched = -(-max(self.n_items, 1) // groups) * groups
+        launched = max(self.n_items, 1)
         self.scratch_slots = min(self.n_slots, launched) if self.scratch_elems else 0
         n_scr = self.scratch_elems * self.scratch_slots
         self.scratch = torch.empty(max(n_scr, 1) * (elem // 8), dtype=torch.complex64, device=dev)

@@ -53,9 +53,19 @@ int set_error(int code, const char* fmt, ...) {
             return set_error(QB_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+// Complex values as clang extended vectors (re, im): builtin loads and stores in
+// every address space.
+typedef float cf32 __attribute__((ext_vector_type(2)));
+typedef double cf64 __attribute__((ext_vector_type(2)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 template <typename R> struct Cx;
-template <> struct Cx<float> { using V = float2; };
-template <> struct Cx<double> { using V = double2; };
+template <> struct Cx<float> { using V = cf32; };
+template <> struct Cx<double> { using V = cf64; };
+
+// Pointers into LDS keep their address space so that LDS and global accesses are
+// never merged into generic (flat) accesses.
+#define QB_LDS __attribute__((address_space(3)))
+#define QB_GLB __attribute__((address_space(1)))
 
 template <typename V>
 __device__ __forceinline__ V cmul(V a, V b) {
@@ -170,17 +180,33 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
 // ---------------------------------------------------------------------------
 
 // Plan layout (int32 words; offsets relative to the plan start):
-//   header[kPlanHdr]: n_gates, n_steps, gates_off, steps_off, result_loc,
-//                     lds_need, glb_need, n_sliced
-//   gates: n_gates * kGateWords
-//   steps: variable length, 4 + n_in * kInWords each
-constexpr int kPlanHdr = 8;
+//   header[kPlanHdr]: n_gates, n_ops, gates_off, ops_off, result_loc,
+//                     lds_need, glb_need, n_sliced, ops_words, 0, 0, 0
+//   gates: n_gates * kGateWords (distinct gate tensors only; equal leaves share one)
+//   ops:   one record per op; the item stages them in LDS after its data
+// LDS element 0 holds 1+0i, element 1 holds 0+0i, element 2 is a write sink;
+// gate tensors start at element 4.
+//
+// Step op (one bucket step, all lanes of the item):
+//   w0 = n_in | rank_out << 4 | sum << 9 | sync_glb << 11 | all_lds << 12 | len << 16
+//   w1 = 0, w2 = out_loc
+//   per input j, 4 words at 3 + 4 j: loc, top, mask, runs
+//     runs = n_runs << 30 | run0 | run1 << 15, run = src | width << 5 | dst << 10;
+//     n_runs == 3 means "more than two runs": the offset is pext(o, mask).
+// Gather op (several small steps, one output element per lane, all in LDS):
+//   w0 = kOpGather | n_steps | len << 16, w1 = active lanes, w2 = descriptor offset
+//   w4...: lane -> step table, one byte per lane
+//   per step, 12 words at w2 + 12 s: out_loc, first_lane | n_in << 8 | sum << 12,
+//     then per input (loc | top << 16, runs); unused inputs read element 0 (1+0i)
+constexpr int kPlanHdr = 12;
 constexpr int kGateWords = 16;
-constexpr int kInWords = 8;
-constexpr int kExecMaxIn = 8;
-constexpr int kExecMaxRuns = 4;
+constexpr int kInWords = 4;
+constexpr int kStepHdr = 3;
+constexpr int kExecMaxIn = 4;
 constexpr uint32_t kGlobal = 0x80000000u;
-
+constexpr uint32_t kOpGather = 1u << 15;
+constexpr int kGatherHdr = 4;
+constexpr int kGatherDesc = 12;
 // gate kinds (shared with qtensor_amd/gates.py and qtensor_amd/plan.py)
 enum GateKind : int32_t {
     kConst = 0, kZPhase = 1, kXPhase = 2, kZZ = 3, kYPhase = 4,
@@ -247,7 +273,7 @@ __device__ void natural_gate(int32_t kind, double alpha, double* re, double* im)
 
 template <typename R>
 __device__ void build_gate(const int32_t* g, const double* params, uint32_t slice_bits,
-                           typename Cx<R>::V* lds, const typename Cx<R>::V* pool) {
+                           QB_LDS typename Cx<R>::V* lds, const typename Cx<R>::V* pool) {
     using V = typename Cx<R>::V;
     const int32_t kind = g[0];
     const int32_t dst = g[1];
@@ -301,63 +327,135 @@ __device__ __forceinline__ void group_sync() {
 }
 
 template <typename V>
-__device__ __forceinline__ V load_loc(uint32_t loc, uint32_t off, const V* lds, const V* glb) {
-    return (loc & kGlobal) ? glb[(loc & ~kGlobal) + off] : lds[loc + off];
+__device__ __attribute__((noinline)) V load_glb(const QB_GLB V* glb, uint32_t idx) {
+    return glb[idx];
 }
 
-template <typename R, int ITEM_THREADS>
-__device__ void run_steps(const int32_t* st, int32_t n_steps, typename Cx<R>::V* lds,
-                          typename Cx<R>::V* glb, int lane) {
+template <typename V>
+__device__ __attribute__((noinline)) void store_glb(QB_GLB V* glb, uint32_t idx, V v) {
+    glb[idx] = v;
+}
+
+// The global paths are out-of-line calls so that the optimiser cannot fold an
+// LDS access and a global access into one generic (flat) access.
+template <typename V>
+__device__ __forceinline__ V load_loc(uint32_t loc, uint32_t off, const QB_LDS V* lds, const QB_GLB V* glb) {
+    if (loc & kGlobal) return load_glb(glb, (loc & ~kGlobal) + off);
+    return lds[loc + off];
+}
+
+__device__ __forceinline__ uint32_t in_offset(uint32_t o, uint32_t mask, uint32_t runs) {
+    const uint32_t n = runs >> 30;
+    if (n == 3u) return pext_loop(o, mask);
+    uint32_t off = 0;
+    if (n > 0) off = bfe(o, runs & 31u, (runs >> 5) & 31u) << ((runs >> 10) & 31u);
+    if (n > 1) off |= bfe(o, (runs >> 15) & 31u, (runs >> 20) & 31u) << ((runs >> 25) & 31u);
+    return off;
+}
+
+#define QB_RL(r, k) ((uint32_t)__builtin_amdgcn_readlane((r), (k)))
+
+template <typename V>
+__device__ __forceinline__ V gather_in(const QB_LDS V* lds, uint32_t locw, uint32_t runs, uint32_t o,
+                                       uint32_t sum, V* y) {
+    const uint32_t a = (locw & 0xffffu) + in_offset(o, 0u, runs);
+    *y = lds[a + (sum ? (locw >> 16) : 0u)];
+    return lds[a];
+}
+
+// One gather op: lane l computes element o = l - first of step table[l].
+template <typename R>
+__device__ __forceinline__ void gather_op(const QB_LDS int32_t* op, QB_LDS typename Cx<R>::V* lds,
+                                          int lane) {
+    static_assert(sizeof(i32x4) == 16, "descriptor rows are 16 bytes");
     using V = typename Cx<R>::V;
-    for (int32_t s = 0; s < n_steps; ++s) {
-        const int32_t n_in = st[0];
-        const uint32_t rank_out = (uint32_t)st[1];
-        const int32_t flags = st[2];
-        const uint32_t out_loc = (uint32_t)st[3];
-        if (flags & 2) group_sync<ITEM_THREADS>();
-        const uint32_t n_sum = (flags & 1) ? 2u : 1u;
-        uint32_t loc[kExecMaxIn], top[kExecMaxIn], nr[kExecMaxIn], mask[kExecMaxIn];
-        uint32_t rs[kExecMaxIn][kExecMaxRuns], rw[kExecMaxIn][kExecMaxRuns],
-            rd[kExecMaxIn][kExecMaxRuns];
+    const uint32_t n_act = (uint32_t)op[1];
+    if ((uint32_t)lane >= n_act) return;
+    const uint32_t sidx = ((const QB_LDS uint8_t*)(op + kGatherHdr))[lane];
+    const QB_LDS i32x4* d = (const QB_LDS i32x4*)(op + (uint32_t)op[2] + kGatherDesc * sidx);
+    const i32x4 d0 = d[0], d1 = d[1], d2 = d[2];
+    const uint32_t meta = (uint32_t)d0.y;
+    const uint32_t o = (uint32_t)lane - (meta & 255u);
+    const uint32_t sum = (meta >> 12) & 1u;
+    V y0, y1, y2, y3;
+    const V x0 = gather_in(lds, (uint32_t)d0.z, (uint32_t)d0.w, o, sum, &y0);
+    const V x1 = gather_in(lds, (uint32_t)d1.x, (uint32_t)d1.y, o, sum, &y1);
+    const V x2 = gather_in(lds, (uint32_t)d1.z, (uint32_t)d1.w, o, sum, &y2);
+    const V x3 = gather_in(lds, (uint32_t)d2.x, (uint32_t)d2.y, o, sum, &y3);
+    V p = cmul(cmul(x0, x1), cmul(x2, x3));
+    if (sum) p = cadd(p, cmul(cmul(y0, y1), cmul(y2, y3)));
+    lds[(uint32_t)d0.x + o] = p;
+}
+
+// One step op: all lanes of the item cover the step's output elements.  Lane l
+// of each wave holds word l of the record; fields reach scalar registers by
+// readlane.
+template <typename R, int ITEM_THREADS>
+__device__ __forceinline__ void step_op(const QB_LDS int32_t* st, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
+                                        QB_GLB typename Cx<R>::V* glb, int lane) {
+    using V = typename Cx<R>::V;
+    const int32_t r_cur = st[lane & 63];
+    const uint32_t out_loc = QB_RL(r_cur, 2);
+    const uint32_t n_in = w0 & 15u;
+    const uint32_t rank_out = (w0 >> 4) & 31u;
+    const uint32_t n_sum = ((w0 >> 9) & 1u) + 1u;
+    // per input: location, summed-variable offset, mask, and this lane's offset
+    // for element o = lane; element o = base + lane adds pext(base, mask), which
+    // is uniform across the item (base is a multiple of the item size)
+    uint32_t loc[kExecMaxIn], top[kExecMaxIn], mask[kExecMaxIn], off0[kExecMaxIn];
 #pragma unroll
-        for (int j = 0; j < kExecMaxIn; ++j) {
-            if (j < n_in) {
-                const int32_t* d = st + 4 + j * kInWords;
-                loc[j] = (uint32_t)d[0];
-                top[j] = (uint32_t)d[1];
-                nr[j] = (uint32_t)d[2];
-                mask[j] = (uint32_t)d[3];
-#pragma unroll
-                for (int r = 0; r < kExecMaxRuns; ++r) {
-                    const uint32_t word = (uint32_t)d[4 + r];
-                    rs[j][r] = word & 31u;
-                    rw[j][r] = (word >> 5) & 63u;
-                    rd[j][r] = (word >> 11) & 31u;
-                }
-            }
+    for (int j = 0; j < kExecMaxIn; ++j) {
+        if ((uint32_t)j < n_in) {
+            loc[j] = QB_RL(r_cur, 3 + 4 * j);
+            top[j] = QB_RL(r_cur, 4 + 4 * j);
+            mask[j] = QB_RL(r_cur, 5 + 4 * j);
+            off0[j] = in_offset((uint32_t)lane, mask[j], QB_RL(r_cur, 6 + 4 * j));
         }
-        const uint32_t n_out = 1u << rank_out;
-        for (uint32_t o = lane; o < n_out; o += ITEM_THREADS) {
-            V acc;
-            acc.x = 0;
-            acc.y = 0;
+    }
+    const uint32_t n_out = 1u << rank_out;
+    if (w0 & (1u << 12)) {
+        // all-LDS: every read of the step is issued before the first use;
+        // unused input slots read element 0 and are dropped by selects
+        uint32_t a0[kExecMaxIn];
+#pragma unroll
+        for (int j = 0; j < kExecMaxIn; ++j) a0[j] = (uint32_t)j < n_in ? loc[j] + off0[j] : 0u;
+        for (uint32_t base = 0; base < n_out; base += ITEM_THREADS) {
+            uint32_t ad[kExecMaxIn];
+#pragma unroll
+            for (int j = 0; j < kExecMaxIn; ++j)
+                ad[j] = a0[j] + ((base && (uint32_t)j < n_in) ? pext_loop(base, mask[j]) : 0u);
+            V x[kExecMaxIn], y[kExecMaxIn];
+#pragma unroll
+            for (int j = 0; j < kExecMaxIn; ++j) x[j] = lds[ad[j]];
+            if (n_sum == 2) {
+#pragma unroll
+                for (int j = 0; j < kExecMaxIn; ++j) y[j] = lds[ad[j] + ((uint32_t)j < n_in ? top[j] : 0u)];
+            }
+            V p = x[0];
+#pragma unroll
+            for (int j = 1; j < kExecMaxIn; ++j) p = (uint32_t)j < n_in ? cmul(p, x[j]) : p;
+            if (n_sum == 2) {
+                V q = y[0];
+#pragma unroll
+                for (int j = 1; j < kExecMaxIn; ++j) q = (uint32_t)j < n_in ? cmul(q, y[j]) : q;
+                p = cadd(p, q);
+            }
+            const uint32_t o = base + (uint32_t)lane;
+            if (o < n_out) lds[out_loc + o] = p;
+        }
+        return;
+    }
+    for (uint32_t base = 0; base < n_out; base += ITEM_THREADS) {
+        const uint32_t o = base + (uint32_t)lane;
+        if (o < n_out) {
+            V acc = {0, 0};
             for (uint32_t sv = 0; sv < n_sum; ++sv) {
-                V prod;
-                prod.x = 1;
-                prod.y = 0;
+                V prod = {1, 0};
 #pragma unroll
                 for (int j = 0; j < kExecMaxIn; ++j) {
-                    if (j < n_in) {
-                        uint32_t off;
-                        if (nr[j] > (uint32_t)kExecMaxRuns) {
-                            off = pext_loop(o, mask[j]);
-                        } else {
-                            off = 0;
-#pragma unroll
-                            for (int r = 0; r < kExecMaxRuns; ++r)
-                                if ((uint32_t)r < nr[j]) off |= bfe(o, rs[j][r], rw[j][r]) << rd[j][r];
-                        }
-                        if (sv) off += top[j];
+                    if ((uint32_t)j < n_in) {
+                        uint32_t off = off0[j] + (sv ? top[j] : 0u);
+                        if (base) off += pext_loop(base, mask[j]);
                         const V v = load_loc(loc[j], off, lds, glb);
                         prod = j == 0 ? v : cmul(prod, v);
                     }
@@ -365,47 +463,82 @@ __device__ void run_steps(const int32_t* st, int32_t n_steps, typename Cx<R>::V*
                 acc = sv ? cadd(acc, prod) : prod;
             }
             if (out_loc & kGlobal)
-                glb[(out_loc & ~kGlobal) + o] = acc;
+                store_glb(glb, (out_loc & ~kGlobal) + o, acc);
             else
                 lds[out_loc + o] = acc;
         }
-        st += 4 + n_in * kInWords;
+    }
+}
+
+// All ops of one plan, from the copy staged in LDS.
+template <typename R, int ITEM_THREADS>
+__device__ void run_ops(const QB_LDS int32_t* st, int32_t n_ops, QB_LDS typename Cx<R>::V* lds,
+                        QB_GLB typename Cx<R>::V* glb, int lane) {
+    for (int32_t k = 0; k < n_ops; ++k) {
+        const uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(st[0]);
+        if constexpr (ITEM_THREADS == 64) {
+            // one wave: LDS operations complete in issue order, so the previous op's
+            // LDS results are visible once the compiler keeps memory operations on
+            // their side of this point; global data needs a real fence
+            if (w0 & (1u << 11)) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (w0 & (1u << 11)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        } else {
+            __syncthreads();
+        }
+        if (w0 & kOpGather)
+            gather_op<R>(st, lds, lane);
+        else
+            step_op<R, ITEM_THREADS>(st, w0, lds, glb, lane);
+        st += w0 >> 16;
     }
 }
 
 template <typename R, int ITEM_THREADS>
-__global__ __launch_bounds__(ITEM_THREADS < 256 ? 256 : ITEM_THREADS) void exec_kernel(const ExecArgs a) {
+__global__ __launch_bounds__(ITEM_THREADS) void exec_kernel(
+    const int32_t* __restrict__ program, const int32_t* __restrict__ items, int32_t n_items,
+    int32_t lds_elems, const double* __restrict__ params, const void* __restrict__ pool_v,
+    void* __restrict__ scratch, int64_t scratch_elems, double* __restrict__ item_out,
+    uint32_t* __restrict__ queue) {
     using V = typename Cx<R>::V;
-    constexpr int kWG = ITEM_THREADS < 256 ? 256 : ITEM_THREADS;
-    constexpr int kGroups = kWG / ITEM_THREADS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ int32_t item_sh[kGroups];
-    const int group = threadIdx.x / ITEM_THREADS;
-    const int lane = threadIdx.x % ITEM_THREADS;
-    V* lds = reinterpret_cast<V*>(smem_raw) + (size_t)group * a.lds_elems;
-    const int64_t slot = (int64_t)blockIdx.x * kGroups + group;
-    V* glb = reinterpret_cast<V*>(a.scratch) + slot * a.scratch_elems;
-    const V* pool = reinterpret_cast<const V*>(a.pool);
+    __shared__ int32_t item_sh;
+    const int lane = threadIdx.x;
+    QB_LDS V* lds = (QB_LDS V*)(smem_raw);
+    QB_GLB V* glb = (QB_GLB V*)(scratch) + (int64_t)blockIdx.x * scratch_elems;
+    const V* pool = reinterpret_cast<const V*>(pool_v);
+    (void)lds_elems;
     for (;;) {
-        if (lane == 0) item_sh[group] = (int32_t)atomicAdd(a.queue, 1u);
+        if (lane == 0) item_sh = (int32_t)atomicAdd(queue, 1u);
         group_sync<ITEM_THREADS>();
-        const int32_t item = item_sh[group];
-        if (item >= a.n_items) break;
-        const int32_t* it = a.items + (size_t)item * QB_ITEM_WORDS;
-        const int32_t* plan = a.program + it[0];
-        const double* params = a.params + it[1];
+        const int32_t item = __builtin_amdgcn_readfirstlane(item_sh);
+        if (item >= n_items) break;
+        const int32_t* __restrict__ it = items + (size_t)item * QB_ITEM_WORDS;
+        const int32_t* __restrict__ plan = program + it[0];
+        const double* __restrict__ prm = params + it[1];
         const uint32_t slice_bits = (uint32_t)it[2];
         const int32_t n_gates = plan[0];
-        const int32_t* gates = plan + plan[2];
+        const int32_t* __restrict__ gates = plan + plan[2];
+        // stage the op records in LDS, after the plan's data (plus 64 words of
+        // padding read by the lane-distributed record loads); constants 1, 0
+        const int32_t rec_words = plan[8];
+        QB_LDS int32_t* recs = (QB_LDS int32_t*)(lds + plan[5]);
+        const int32_t* __restrict__ src = plan + plan[3];
+        for (int32_t w = lane; w < rec_words + 64; w += ITEM_THREADS) recs[w] = w < rec_words ? src[w] : 0;
+        if (lane == 0) {
+            lds[0] = V{1, 0};
+            lds[1] = V{0, 0};
+        }
         for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
-            build_gate<R>(gates + (size_t)g * kGateWords, params, slice_bits, lds, pool);
+            build_gate<R>(gates + (size_t)g * kGateWords, prm, slice_bits, lds, pool);
         group_sync<ITEM_THREADS>();
-        run_steps<R, ITEM_THREADS>(plan + plan[3], plan[1], lds, glb, lane);
+        run_ops<R, ITEM_THREADS>(recs, plan[1], lds, glb, lane);
         group_sync<ITEM_THREADS>();
         if (lane == 0) {
             const V r = load_loc((uint32_t)plan[4], 0u, lds, glb);
-            a.item_out[2 * (size_t)item] = (double)r.x;
-            a.item_out[2 * (size_t)item + 1] = (double)r.y;
+            item_out[2 * (size_t)item] = (double)r.x;
+            item_out[2 * (size_t)item + 1] = (double)r.y;
         }
         // item_sh is rewritten only after every lane has passed this point
         group_sync<ITEM_THREADS>();
@@ -462,8 +595,7 @@ __global__ __launch_bounds__(256) void reduce_total(const double* term_out, int3
 
 template <typename R, int T>
 int exec_occupancy(int lds_bytes, int* blocks) {
-    constexpr int kWG = T < 256 ? 256 : T;
-    QB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, exec_kernel<R, T>, kWG, lds_bytes));
+    QB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, exec_kernel<R, T>, T, lds_bytes));
     return QB_OK;
 }
 
@@ -481,8 +613,9 @@ int exec_occupancy_dispatch(int item_threads, int lds_bytes, int* blocks) {
 
 template <typename R, int T>
 int exec_launch(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
-    constexpr int kWG = T < 256 ? 256 : T;
-    hipLaunchKernelGGL((exec_kernel<R, T>), dim3(grid), dim3(kWG), lds_bytes, st, a);
+    hipLaunchKernelGGL((exec_kernel<R, T>), dim3(grid), dim3(T), lds_bytes, st, a.program, a.items,
+                       a.n_items, a.lds_elems, a.params, a.pool, a.scratch, a.scratch_elems, a.item_out,
+                       a.queue);
     QB_HIP(hipGetLastError());
     return QB_OK;
 }
@@ -503,8 +636,8 @@ int slots_for(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n
     if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
     if (lds_elems < 0) return set_error(QB_ERR_INVALID, "lds_elems < 0");
     const int elem = dtype == QB_C128 ? 16 : 8;
-    const int groups = item_threads < 256 ? 256 / item_threads : 1;
-    const int64_t lds_bytes = (int64_t)lds_elems * elem * groups;
+    const int groups = 1;
+    const int64_t lds_bytes = (int64_t)lds_elems * elem;
     if (lds_bytes > 160 * 1024)
         return set_error(QB_ERR_UNSUPPORTED, "LDS request %lld bytes exceeds 160 KiB", (long long)lds_bytes);
     int blocks = 0;
@@ -601,7 +734,7 @@ int qb_execute(const qb_exec_desc* d, void* stream) {
     int per_cu = 0;
     int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, &n_slots, &per_cu);
     if (rc) return rc;
-    const int groups = d->item_threads < 256 ? 256 / d->item_threads : 1;
+    const int groups = 1;
     int grid = per_cu * g_num_cus;
     const int need = (d->n_items + groups - 1) / groups;
     if (need < grid) grid = need;

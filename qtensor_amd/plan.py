@@ -1,4 +1,4 @@
-"""Plan compiler: one lightcone tensor network -> a static step program.
+"""Plan compiler: one lightcone tensor network -> a static op program.
 
 A plan is the symbolic run of ``bucket_elimination`` with the backend's
 ``process_bucket`` (``qtensor/Simulate.py:84-94``,
@@ -15,16 +15,22 @@ the same bucket as in the reference loop.  Variables sliced to an integer
 (Feynman slicing, ``qtensor/FeynmanSimulator.py:18-25,49-58``) disappear from
 the network; each work item selects their values through ``slice_bits``.
 
-Leaf tensors are gates.  Each leaf records its gate kind and angle as a
-linear form of the request's (gamma, beta) so the device rebuilds it in FP64,
-plus a map from its device layout to the gate's natural layout (transposed
-to ascending variable id and sliced).
+Leaf tensors are gates.  Each distinct leaf records its gate kind and angle
+as a linear form of the request's (gamma, beta) so the device rebuilds it in
+FP64, plus a map from its device layout to the gate's natural layout
+(transposed to ascending variable id and sliced).
 
-Memory: leaves live in an LDS gate area; an intermediate goes to the LDS
-arena when it has at most ``lds_tensor_limit`` elements and fits, otherwise to
-the item's global scratch slot.  Both arenas are first-fit with reuse after
-the consuming step.  A barrier is emitted before a step only if it reads
-data written, or writes memory read or written, since the previous barrier.
+Scheduling: steps form a DAG (each result feeds one later bucket).  Steps
+with at most one wave of outputs, at most four inputs, at most two bit-runs
+per input offset and all data in LDS are packed, dependency-respecting, into
+"gather" ops whose lanes each compute one output element of one step; every
+other step is a "step" op executed by all lanes of the item.
+
+Memory: leaves live in an LDS gate area (plus constants ONE, ZERO and a sink
+element); an intermediate goes to the LDS arena when it has at most
+``lds_tensor_limit`` elements, otherwise to the item's global scratch slot.
+Both arenas are first-fit; an op's outputs are placed before its inputs are
+released, so no op writes memory it also reads.
 
 The program format is shared with ``csrc/qtensor_amd.hip`` (constants below).
 """
@@ -37,12 +43,21 @@ from .circuit import reorder_buckets
 from .gates import (KIND_CONST, KIND_XPHASE, KIND_YPHASE, KIND_ZPHASE, KIND_ZZ)
 from .tensornet import QtreeTensorNet
 
-PLAN_HDR = 8
+PLAN_HDR = 12
 GATE_WORDS = 16
-IN_WORDS = 8
-MAX_IN = 8
-MAX_RUNS = 4
+IN_WORDS = 4
+STEP_HDR = 3
+MAX_IN = 4
+PROGRAM_PAD = 64
+MAX_RUNS = 2
 GLOBAL = 0x80000000
+OP_GATHER = 1 << 15
+GATHER_HDR = 4
+GATHER_DESC = 12
+GATHER_MAX_STEPS = 16
+# LDS element offsets of the constants every plan reserves ahead of its gates
+LOC_ONE, LOC_ZERO, LOC_SINK = 0, 1, 2
+GATE_BASE = 4
 
 K_H, K_CX, K_Z, K_M = 5, 6, 7, 8
 _KIND_BY_NAME = {"H": K_H, "cX": K_CX, "Z": K_Z, "M": K_M, "ZPhase": KIND_ZPHASE,
@@ -78,14 +93,35 @@ def mask_runs(mask):
     return runs
 
 
+def _runs_word(runs):
+    if len(runs) > MAX_RUNS:
+        return 3 << 30
+    packed = [src | (width << 5) | (dst << 10) for src, width, dst in runs] + [0, 0]
+    return (len(runs) << 30) | packed[0] | (packed[1] << 15)
+
+
+class Buf:
+    """Storage of a tensor; pass-through aliases share one Buf."""
+    __slots__ = ("n", "loc", "leaf", "last_op")
+
+    def __init__(self, n, loc=-1, leaf=False):
+        self.n = n
+        self.loc = loc
+        self.leaf = leaf
+        self.last_op = -1
+
+    @property
+    def is_global(self):
+        return bool(self.loc & GLOBAL) if self.loc >= 0 else False
+
+
 @dataclass
 class Node:
     """A tensor of the symbolic elimination."""
     indices: tuple            # full symbolic indices (Var), ascending id
     dev: tuple                # ids of size-2 variables = device layout axes
-    loc: int = -1             # element offset; GLOBAL bit for the global arena
-    leaf: int = -1            # gate index for leaves
-    step: int = -1            # producing step for intermediates
+    buf: Buf = None
+    step: int = -1            # producing step (-1 for leaves)
 
     @property
     def n(self):
@@ -93,9 +129,28 @@ class Node:
 
 
 @dataclass
+class SymStep:
+    inputs: list
+    out: Node
+    rank_out: int
+    sum_flag: int
+    lows: list                # per input: mask of output bits
+    tops: list                # per input: offset of the summed variable = 1
+    global_res: bool = False  # output too large for LDS
+
+    def gatherable(self, lanes, lds_limit):
+        if self.out.n > lanes or len(self.inputs) > MAX_IN:
+            return False
+        if any(nd.n > lds_limit and not nd.buf.leaf for nd in self.inputs):
+            return False
+        return all(len(mask_runs(m)) <= MAX_RUNS for m in self.lows)
+
+
+@dataclass
 class Plan:
     words: np.ndarray
     n_steps: int
+    n_ops: int
     n_gates: int
     treewidth: int
     max_rank: int
@@ -105,7 +160,15 @@ class Plan:
     bytes_step_model: int     # algorithmic bytes (complex64) summed over steps
     flops: int
     n_barriers: int
+    n_gather: int = 0
     step_sizes: list = field(default_factory=list)
+    steps_words: int = 0
+    dag: list = field(default_factory=list)   # per step: (producer steps of inputs, n_out, n_in, sum)
+
+    def lds_elems(self, elem_bytes=8):
+        """LDS elements one item needs: data, then the staged op records
+        (plus 64 words of padding)."""
+        return self.lds_need + -(-(self.steps_words + PROGRAM_PAD) * 4 // elem_bytes)
 
 
 class _Arena:
@@ -151,17 +214,22 @@ class _Arena:
         self.free = merged
 
 
-def _overlap(a, b):
-    return a[0] < b[1] and b[0] < a[1]
-
-
 class PlanCompiler:
-    """Compile a (reordered) bucket network into a device plan."""
+    """Compile a (reordered) bucket network into a device plan.
 
-    def __init__(self, lds_tensor_limit=1 << 12, lds_capacity=None, elem_bytes=8):
+    ``lanes``: threads per work item (a gather op holds at most this many
+    output elements); ``gather=False`` emits one step op per bucket.
+    """
+
+    def __init__(self, lds_tensor_limit=1 << 12, lds_capacity=None, elem_bytes=8, lanes=64,
+                 gather=True, window=None):
         self.lds_tensor_limit = lds_tensor_limit
         self.lds_capacity = lds_capacity      # elements available per item, or None
         self.elem_bytes = elem_bytes
+        self.lanes = lanes
+        self.gather = gather
+        # a step may join one of the last `window` ops (None: any op after its inputs)
+        self.window = window
 
     # -- front end: circuit -> reordered symbolic buckets
     @staticmethod
@@ -193,40 +261,29 @@ class PlanCompiler:
         if len(par_index) > 24:
             raise ValueError("more than 24 sliced variables")
 
-        gates = []          # gate entries (lists of words)
-        gate_area = 0
-        sym = []            # symbolic buckets of Nodes
+        # leaves: distinct gate tensors in the LDS gate area
+        gates, gate_buf = [], {}
+        gate_area = GATE_BASE
+        sym = []
         for bucket in buckets:
             nb = []
             for t in bucket:
                 node, entry, size = self._leaf(t, data_dict, fixed, par_index, gate_area)
-                node.leaf = len(gates)
-                gates.append(entry)
-                gate_area += size
+                sig = tuple(entry[:1] + entry[2:])
+                if sig not in gate_buf:
+                    gate_buf[sig] = Buf(size, loc=gate_area, leaf=True)
+                    gates.append(entry)
+                    gate_area += size
+                node.buf = gate_buf[sig]
                 nb.append(node)
             sym.append(nb)
 
-        lds = _Arena(base=gate_area, align=2)
-        glb = _Arena(base=0, align=64)
+        # phase 1: symbolic elimination -> steps
         steps = []
-        barriers = 0
-        since_w = []    # intervals written since last barrier: (space, start, end)
-        since_r = []
-        n_nosum = len(par_index)
-        n_contract = len(sym) - n_nosum
-        scalars = []
-        total_bytes = 0
-        total_flops = 0
-        step_sizes = []
-        max_rank = 0
-
-        def interval(node):
-            sp = 1 if node.loc & GLOBAL else 0
-            s = node.loc & ~GLOBAL
-            return (sp, s, s + node.n)
 
         def emit(inputs, no_sum):
-            nonlocal barriers, total_bytes, total_flops, max_rank
+            if not 1 <= len(inputs) <= MAX_IN:
+                raise AssertionError(f"step with {len(inputs)} inputs (kernel takes 1..{MAX_IN})")
             union = {}
             for nd in inputs:
                 for v in nd.indices:
@@ -238,57 +295,20 @@ class PlanCompiler:
             out_indices = tuple(U[1:]) if do_sum else tuple(U)
             out_dev = tuple(U2[1:]) if sum_flag else tuple(U2)
             rank_u = len(U2)
-            rank_out = rank_u - sum_flag
-            max_rank = max(max_rank, rank_u)
             if rank_u > 31:
                 raise ValueError(f"step rank {rank_u} exceeds 31")
             pos = {vid: rank_u - 1 - a for a, vid in enumerate(U2)}
             top_bit = (1 << (rank_u - 1)) if sum_flag else 0
-            out = Node(out_indices, out_dev, step=len(steps))
-            # allocate output before releasing inputs (no aliasing)
-            n_out = out.n
-            s = lds.alloc(n_out, self.lds_capacity) if n_out <= self.lds_tensor_limit else None
-            out.loc = s if s is not None else (GLOBAL | glb.alloc(n_out))
-            words = [len(inputs), rank_out, sum_flag, _i32(out.loc)]
+            lows, tops = [], []
             for nd in inputs:
                 m = 0
                 for vid in nd.dev:
                     m |= 1 << pos[vid]
                 low = m & ~top_bit
-                top = (1 << bin(low).count("1")) if (m & top_bit) else 0
-                runs = mask_runs(low)
-                w = [_i32(nd.loc), top, len(runs) if len(runs) <= MAX_RUNS else 255, low]
-                for r in range(MAX_RUNS):
-                    if r < len(runs) and len(runs) <= MAX_RUNS:
-                        src, width, dst = runs[r]
-                        w.append(src | (width << 5) | (dst << 11))
-                    else:
-                        w.append(0)
-                words += w
-            # hazards since the last barrier
-            reads = [interval(nd) for nd in inputs]
-            wr = interval(out)
-            need = any(_overlap(r[1:], x[1:]) and r[0] == x[0] for r in reads for x in since_w) or \
-                any(wr[0] == x[0] and _overlap(wr[1:], x[1:]) for x in since_w + since_r)
-            if need:
-                words[2] |= 2
-                barriers += 1
-                since_w.clear()
-                since_r.clear()
-            since_w.append(wr)
-            since_r.extend(reads)
-            steps.append(words)
-            n_in_elems = sum(nd.n for nd in inputs)
-            total_bytes += 8 * (n_in_elems + n_out)
-            n_union = 1 << rank_u
-            total_flops += 6 * (len(inputs) - 1) * n_union + 2 * (n_union - n_out)
-            step_sizes.append(rank_u)
-            for nd in inputs:
-                if nd.step >= 0:
-                    if nd.loc & GLOBAL:
-                        glb.release(nd.loc & ~GLOBAL, nd.n)
-                    else:
-                        lds.release(nd.loc, nd.n)
+                lows.append(low)
+                tops.append((1 << bin(low).count("1")) if (m & top_bit) else 0)
+            out = Node(out_indices, out_dev, buf=Buf(1 << len(out_dev)), step=len(steps))
+            steps.append(SymStep(list(inputs), out, rank_u - sum_flag, sum_flag, lows, tops))
             return out
 
         def process(nodes, no_sum=False):
@@ -300,9 +320,12 @@ class PlanCompiler:
             if len(nodes) == 1 and nodes[0].indices and nodes[0].indices[0].size == 1 \
                     and not no_sum:
                 nd = nodes[0]
-                return Node(nd.indices[1:], nd.dev, loc=nd.loc, leaf=nd.leaf, step=nd.step)
+                return Node(nd.indices[1:], nd.dev, buf=nd.buf, step=nd.step)
             return emit(nodes, no_sum)
 
+        n_nosum = len(par_index)
+        n_contract = len(sym) - n_nosum
+        scalars = []
         for b in range(n_contract):
             if not sym[b]:
                 continue
@@ -319,26 +342,160 @@ class PlanCompiler:
             scalars.append(res)
         if not scalars:
             raise ValueError("network has no scalar result")
-        if len(scalars) == 1:
-            result = scalars[0]
-        else:
-            result = emit(scalars, no_sum=True)
+        result = scalars[0] if len(scalars) == 1 else process(scalars, no_sum=True)
         if result.dev:
             raise ValueError("network result is not a scalar")
 
-        n_steps = len(steps)
+        # phase 2: schedule steps into ops
+        lanes = self.lanes
+        lim = self.lds_tensor_limit
+        ops = []          # list of lists of step indices
+        op_kind = []      # True: gather op
+        op_fill = []
+        op_of = [0] * len(steps)
+        for i, st in enumerate(steps):
+            ready = max([op_of[nd.step] + 1 for nd in st.inputs if nd.step >= 0] + [0])
+            if self.window is not None:
+                ready = max(ready, len(ops) - self.window)
+            if self.gather and st.out.n <= lim and st.gatherable(lanes, lim):
+                k = ready
+                while k < len(ops) and not (op_kind[k] and op_fill[k] + st.out.n <= lanes
+                                            and len(ops[k]) < GATHER_MAX_STEPS):
+                    k += 1
+                if k == len(ops):
+                    ops.append([])
+                    op_kind.append(True)
+                    op_fill.append(0)
+                ops[k].append(i)
+                op_fill[k] += st.out.n
+            else:
+                k = len(ops) if ready >= len(ops) else max(ready, len(ops))
+                ops.append([i])
+                op_kind.append(False)
+                op_fill.append(st.out.n)
+            op_of[i] = k
+
+        # phase 3: allocation in op order
+        for i, st in enumerate(steps):
+            for nd in st.inputs:
+                nd.buf.last_op = max(nd.buf.last_op, op_of[i])
+        lds = _Arena(base=gate_area, align=2)
+        glb = _Arena(base=0, align=64)
+        by_last = {}
+        for i, st in enumerate(steps):
+            for nd in st.inputs:
+                if not nd.buf.leaf:
+                    by_last.setdefault(nd.buf.last_op, set()).add(nd.buf)
+        for k, members in enumerate(ops):
+            for i in members:
+                b = steps[i].out.buf
+                if b.n <= lim:
+                    s = lds.alloc(b.n, self.lds_capacity)
+                    if s is None:
+                        raise ValueError("LDS capacity exhausted; use a larger capacity, a smaller "
+                                         "LDS tensor limit or more slicing")
+                    b.loc = s
+                else:
+                    b.loc = GLOBAL | glb.alloc(b.n)
+            for b in by_last.get(k, ()):
+                if b.loc & GLOBAL:
+                    glb.release(b.loc & ~GLOBAL, b.n)
+                else:
+                    lds.release(b.loc, b.n)
+
+        # phase 4: encode ops; barriers only matter for global data
+        op_words = []
+        glb_written_since = set()
+        barriers = 0
+        n_gather = 0
+        for k, members in enumerate(ops):
+            reads_glb = any(nd.buf.is_global for i in members for nd in steps[i].inputs)
+            writes_glb = any(steps[i].out.buf.is_global for i in members)
+            sync_glb = 0
+            if (reads_glb or writes_glb) and glb_written_since:
+                sync_glb = 1
+                barriers += 1
+                glb_written_since = set()
+            if writes_glb:
+                glb_written_since.add(k)
+            if op_kind[k]:
+                op_words.append(self._encode_gather([steps[i] for i in members]))
+                n_gather += 1
+            else:
+                op_words.append(self._encode_step(steps[members[0]], sync_glb))
+
+        total_bytes, total_flops, max_rank = 0, 0, 0
+        step_sizes, dag = [], []
+        for st in steps:
+            n_out = st.out.n
+            rank_u = st.rank_out + st.sum_flag
+            max_rank = max(max_rank, rank_u)
+            total_bytes += 8 * (sum(nd.n for nd in st.inputs) + n_out)
+            n_union = 1 << rank_u
+            total_flops += 6 * (len(st.inputs) - 1) * n_union + 2 * (n_union - n_out)
+            step_sizes.append(rank_u)
+            dag.append(([nd.step for nd in st.inputs], n_out, len(st.inputs), st.sum_flag))
+
+        n_ops = len(op_words)
         gates_off = PLAN_HDR
         steps_off = gates_off + GATE_WORDS * len(gates)
-        words = [len(gates), n_steps, gates_off, steps_off, _i32(result.loc),
-                 lds.peak, glb.peak, len(par_index)]
+        steps_words = sum(len(w) for w in op_words)
+        words = [len(gates), n_ops, gates_off, steps_off, _i32(result.buf.loc),
+                 lds.peak, glb.peak, len(par_index), steps_words, 0, 0, 0]
         for g in gates:
             words += g
-        for s in steps:
-            words += s
-        return Plan(words=np.asarray(words, dtype=np.int32), n_steps=n_steps, n_gates=len(gates),
-                    treewidth=treewidth, max_rank=max_rank, lds_need=lds.peak, glb_need=glb.peak,
-                    n_sliced=len(par_index), bytes_step_model=total_bytes, flops=total_flops,
-                    n_barriers=barriers, step_sizes=step_sizes)
+        for w in op_words:
+            words += w
+        return Plan(words=np.asarray(words, dtype=np.int32), n_steps=len(steps), n_ops=n_ops,
+                    n_gates=len(gates), treewidth=treewidth, max_rank=max_rank,
+                    lds_need=lds.peak, glb_need=glb.peak, n_sliced=len(par_index),
+                    bytes_step_model=total_bytes, flops=total_flops, n_barriers=barriers,
+                    n_gather=n_gather, step_sizes=step_sizes, steps_words=steps_words, dag=dag)
+
+    @staticmethod
+    def _encode_step(st, sync_glb):
+        rec_len = STEP_HDR + IN_WORDS * len(st.inputs)
+        all_lds = not st.out.buf.is_global and not any(nd.buf.is_global for nd in st.inputs)
+        w0 = len(st.inputs) | (st.rank_out << 4) | (st.sum_flag << 9) | (sync_glb << 11) | \
+            ((1 if all_lds else 0) << 12) | (rec_len << 16)
+        words = [w0, 0, _i32(st.out.buf.loc)]
+        for nd, low, top in zip(st.inputs, st.lows, st.tops):
+            words += [_i32(nd.buf.loc), top, _i32(low), _i32(_runs_word(mask_runs(low)))]
+        return words
+
+    def _encode_gather(self, members):
+        """Gather op: header, lane -> step table, one 12-word descriptor per step."""
+        lanes = self.lanes
+        table_words = -(-lanes // 4)
+        n = len(members)
+        first = []
+        lane_step = [255] * (table_words * 4)
+        pos = 0
+        for s_idx, st in enumerate(members):
+            first.append(pos)
+            for o in range(st.out.n):
+                lane_step[pos + o] = s_idx
+            pos += st.out.n
+        hdr_len = GATHER_HDR + table_words
+        desc_off = -(-hdr_len // 4) * 4      # 16-byte aligned descriptors
+        length = desc_off + GATHER_DESC * n
+        words = [OP_GATHER | n | (length << 16), pos, desc_off, 0]
+        for t in range(table_words):
+            b = lane_step[4 * t: 4 * t + 4]
+            words.append(_i32(b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24)))
+        words += [0] * (desc_off - len(words))
+        for s_idx, st in enumerate(members):
+            d = [_i32(st.out.buf.loc), first[s_idx] | (len(st.inputs) << 8) | (st.sum_flag << 12)]
+            for j in range(MAX_IN):
+                if j < len(st.inputs):
+                    loc, top, low = st.inputs[j].buf.loc, st.tops[j], st.lows[j]
+                    assert 0 <= loc < (1 << 16) and 0 <= top < (1 << 16)
+                    d += [_i32(loc | (top << 16)), _i32(_runs_word(mask_runs(low)))]
+                else:
+                    d += [_i32(LOC_ONE), 0]
+            d += [0, 0]
+            words += d
+        return words
 
     def _leaf(self, t, data_dict, fixed, par_index, dst):
         """Leaf node and its gate entry for tensor ``t`` (indices in data order)."""
@@ -400,5 +557,5 @@ class PlanCompiler:
             raise ValueError("a gate touches more than 2 sliced variables")
         entry = [kind, dst, n, _i32(src_map), int(j0), _f32_bits(c0), int(j1), _f32_bits(c1), lo, hi,
                  0, slw, slice_axes[0][1] if sl > 0 else 0, slice_axes[1][1] if sl > 1 else 0, 0, 0]
-        node = Node(tuple(kept), tuple(int(v) for v in kept if v.size == 2), loc=dst)
+        node = Node(tuple(kept), tuple(int(v) for v in kept if v.size == 2))
         return node, entry, n

@@ -55,11 +55,27 @@ def pext(x, m):
     return r
 
 
-def run_plan(words, params, slice_bits=0, dtype=np.complex128):
+def _runs_offset(o, rw, mask):
+    nr = rw >> 30
+    if nr == 3:
+        return np.array([pext(int(x), mask) for x in np.atleast_1d(o)], dtype=np.int64)
+    off = np.zeros_like(np.atleast_1d(o))
+    for r in range(nr):
+        word = (rw >> (15 * r)) & 0x7FFF
+        src, wd, dst = word & 31, (word >> 5) & 31, (word >> 10) & 31
+        off = off | (((np.atleast_1d(o) >> src) & ((1 << wd) - 1)) << dst)
+    return off
+
+
+def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
+    """Execute a plan; ``lanes`` must be the item size the plan was compiled for."""
     w = [int(x) for x in np.asarray(words)]
-    n_gates, n_steps, goff, soff, result_loc, lds_need, glb_need = w[:7]
-    lds = np.full(max(lds_need, 1), np.nan + 0j, dtype=dtype)
+    n_gates, n_ops, goff, soff, result_loc, lds_need, glb_need = w[:7]
+    ops_words = w[8]
+    lds = np.full(max(lds_need, P.GATE_BASE), np.nan + 0j, dtype=dtype)
     glb = np.full(max(glb_need, 1), np.nan + 0j, dtype=dtype)
+    lds[P.LOC_ONE] = 1
+    lds[P.LOC_ZERO] = 0
     for g in range(n_gates):
         e = w[goff + g * P.GATE_WORDS: goff + (g + 1) * P.GATE_WORDS]
         kind, dst, n, smap = e[0], e[1], e[2], e[3] & 0xFFFFFFFF
@@ -78,35 +94,60 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128):
         for k in range(n):
             lds[dst + k] = nat[(((smap >> (4 * k)) & 15) + shift) & 7]
     pos = soff
-    for _ in range(n_steps):
-        n_in, rank_out, flags, out_loc = w[pos:pos + 4]
-        out_loc &= 0xFFFFFFFF
-        descs = []
-        for j in range(n_in):
-            d = w[pos + 4 + j * P.IN_WORDS: pos + 4 + (j + 1) * P.IN_WORDS]
-            descs.append(((d[0] & 0xFFFFFFFF), d[1], d[2], d[3] & 0xFFFFFFFF, d[4:8]))
-        n_sum = 2 if flags & 1 else 1
-        n_out = 1 << rank_out
-        o = np.arange(n_out, dtype=np.int64)
-        acc = np.zeros(n_out, dtype=dtype)
-        for sv in range(n_sum):
-            prod = np.ones(n_out, dtype=dtype)
-            for loc, top, nr, mask, runs in descs:
-                if nr > P.MAX_RUNS:
-                    off = np.array([pext(int(x), mask) for x in o], dtype=np.int64)
-                else:
-                    off = np.zeros(n_out, dtype=np.int64)
-                    for r in range(nr):
-                        word = runs[r]
-                        src, wd, dst = word & 31, (word >> 5) & 63, (word >> 11) & 31
-                        off |= ((o >> src) & ((1 << wd) - 1)) << dst
-                if sv:
-                    off = off + top
-                buf = glb if loc & P.GLOBAL else lds
-                prod = prod * buf[(loc & ~P.GLOBAL) + off]
-            acc = acc + prod
-        buf = glb if out_loc & P.GLOBAL else lds
-        buf[(out_loc & ~P.GLOBAL) + o] = acc
-        pos += 4 + n_in * P.IN_WORDS
+    for _ in range(n_ops):
+        w0 = w[pos] & 0xFFFFFFFF
+        length = w0 >> 16
+        assert pos + length <= soff + ops_words
+        if w0 & P.OP_GATHER:
+            n_steps, n_act, desc_off = w0 & 0x1F, w[pos + 1], w[pos + 2]
+            assert n_act <= lanes and 1 <= n_steps <= P.GATHER_MAX_STEPS and desc_off % 4 == 0
+            table = []
+            for t in range(-(-lanes // 4)):
+                word = w[pos + P.GATHER_HDR + t] & 0xFFFFFFFF
+                table += [(word >> (8 * b)) & 255 for b in range(4)]
+            results = []
+            for lane in range(n_act):
+                sidx = table[lane]
+                assert sidx < n_steps
+                d = [x & 0xFFFFFFFF for x in w[pos + desc_off + P.GATHER_DESC * sidx:
+                                              pos + desc_off + P.GATHER_DESC * (sidx + 1)]]
+                out_loc, meta = d[0], d[1]
+                o = lane - (meta & 255)
+                n_in, sm = (meta >> 8) & 15, (meta >> 12) & 1
+                assert 1 <= n_in <= P.MAX_IN and o >= 0
+                x, y = 1 + 0j, 1 + 0j
+                for j in range(P.MAX_IN):
+                    locw, rw = d[2 + 2 * j], d[3 + 2 * j]
+                    a = (locw & 0xFFFF) + int(_runs_offset(o, rw, 0)[0])
+                    if j >= n_in:
+                        assert (locw & 0xFFFF) == P.LOC_ONE and rw == 0
+                    x *= lds[a]
+                    y *= lds[a + ((locw >> 16) if sm else 0)]
+                results.append((out_loc + o, x + y if sm else x))
+            for addr, v in results:        # all reads of an op precede its writes
+                lds[addr] = v
+        else:
+            out_loc = w[pos + 2] & 0xFFFFFFFF
+            n_in, rank_out, flags = w0 & 15, (w0 >> 4) & 31, (w0 >> 9) & 1
+            assert length == P.STEP_HDR + P.IN_WORDS * n_in
+            assert 1 <= n_in <= P.MAX_IN, "the device executor takes at most MAX_IN inputs per step"
+            n_sum = 2 if flags else 1
+            n_out = 1 << rank_out
+            o = np.arange(n_out, dtype=np.int64)
+            acc = np.zeros(n_out, dtype=dtype)
+            for sv in range(n_sum):
+                prod = np.ones(n_out, dtype=dtype)
+                for j in range(n_in):
+                    d = w[pos + P.STEP_HDR + j * P.IN_WORDS: pos + P.STEP_HDR + (j + 1) * P.IN_WORDS]
+                    loc, top, mask, rw = d[0] & 0xFFFFFFFF, d[1], d[2] & 0xFFFFFFFF, d[3] & 0xFFFFFFFF
+                    off = _runs_offset(o, rw, mask)
+                    if sv:
+                        off = off + top
+                    buf = glb if loc & P.GLOBAL else lds
+                    prod = prod * buf[(loc & ~P.GLOBAL) + off]
+                acc = acc + prod
+            buf = glb if out_loc & P.GLOBAL else lds
+            buf[(out_loc & ~P.GLOBAL) + o] = acc
+        pos += length
     buf = glb if result_loc & P.GLOBAL else lds
     return complex(buf[(result_loc & 0xFFFFFFFF) & ~P.GLOBAL])
