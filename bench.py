@@ -85,6 +85,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--plain-edges", type=int, default=24)
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="fixed launch shape item_threads,lds_capacity,lds_tensor_limit (skips the autotuner)")
+    ap.add_argument("--slice-tw", type=int, default=None,
+                    help="with --shape: slice lightcones whose treewidth reaches this value")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -99,13 +103,29 @@ def main():
     n_terms = G.number_of_edges()
     print(f"[bench] workload {desc}: {n_terms} edge terms", file=sys.stderr, flush=True)
 
-    eng = LightconeEngine()
+    # launch shape and slicing from the autotuner (timed outside the measured region)
+    from qtensor_amd.autotune import Autotuner
+    from qtensor_amd.batched import LaunchShape
+    t0 = time.perf_counter()
+    if args.shape:
+        it, cap, lim = (int(x) for x in args.shape.split(","))
+        shape, max_tw, table = LaunchShape(it, cap, lim), args.slice_tw, []
+    else:
+        tuner = Autotuner(verbose=True)
+        shape, max_tw, table = tuner.tune(G, p, gamma, beta)
+    tune_s = time.perf_counter() - t0
+    print(f"[bench] tuned: {shape.key()} max_tw={max_tw} ({tune_s:.1f} s)", file=sys.stderr, flush=True)
+
+    eng = LightconeEngine(shape=shape, max_tw=max_tw)
     t0 = time.perf_counter()
     wl = eng.workload(G, p)
     plan_cold = time.perf_counter() - t0
     t0 = time.perf_counter()
     eng.workload(G, p)
     plan_warm = time.perf_counter() - t0
+    # algorithmic bytes: the unsliced greedy plans of the same networks
+    wl_alg = LightconeEngine(shape=shape).workload(G, p)
+    alg_bytes = wl_alg.step_model_bytes()
     batch = eng.batch([(wl, gamma, beta)])
     batch.capture()
     for _ in range(args.warmup):
@@ -132,11 +152,11 @@ def main():
         evs.append((a, b))
     torch.cuda.synchronize()
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
-    bytes_per_launch = batch.bytes_step_model
+    bytes_per_launch = alg_bytes
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
 
     # error against FP64 (complex128 run of the same plans)
-    eng128 = LightconeEngine(dtype="complex128")
+    eng128 = LightconeEngine(dtype="complex128", shape=shape)
     b128 = eng128.batch([(eng128.workload(G, p), gamma, beta)])
     b128.run()
     ref128 = np.array(b128.terms()[0])
@@ -151,15 +171,19 @@ def main():
         "data": "synthetic (seeded networkx graph and angles)",
         "config": {"workload": desc, "p": p, "edges": n_terms, "composer": "QtreeQAOAComposer",
                    "ordering": "greedy", "form": "batched persistent executor, HIP graph",
-                   "max_treewidth": wl.max_treewidth(), "distinct_plans": len(set(map(id, wl.plans.values()))),
-                   "item_threads": batch.shape.item_threads},
+                   "max_treewidth": wl_alg.max_treewidth(), "distinct_plans": len(set(map(id, wl.plans.values()))),
+                   "item_threads": shape.item_threads, "lds_capacity": shape.lds_capacity,
+                   "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw,
+                   "work_items": batch.n_items},
+        "autotune": {"seconds": round(tune_s, 1), "candidates": len(table)},
         "max_rel_err_vs_fp64": err_fp64,
         "energy": energy,
         "plan_seconds": {"cold": round(plan_cold, 3), "warm": round(plan_warm, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
-                     "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch},
+                     "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,
+                     "executed_step_bytes_per_launch": batch.bytes_step_model},
     }
     if not args.no_baselines:
         n_plain = min(args.plain_edges, n_terms)

@@ -79,6 +79,12 @@ class GraphWorkload:
         return max(self.plans[w.key].treewidth for w in self.works)
 
 
+# compiled plans, shared by every engine in the process (plans do not depend on
+# the angles or the precision, only on network structure and launch shape)
+_PLAN_CACHE = {}
+_PLAN_LOCK = threading.Lock()
+
+
 class LightconeEngine:
     """Compile lightcones and run them in batches on one GPU.
 
@@ -95,8 +101,8 @@ class LightconeEngine:
         self.shape = shape
         self.max_tw = max_tw
         self.tw_bias = tw_bias
-        self._plans = {}
-        self._lock = threading.Lock()
+        self._plans = _PLAN_CACHE
+        self._lock = _PLAN_LOCK
 
     # -- host planning -----------------------------------------------------
     def _optimizer(self):

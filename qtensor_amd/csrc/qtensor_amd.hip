@@ -181,7 +181,8 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
 
 // Plan layout (int32 words; offsets relative to the plan start):
 //   header[kPlanHdr]: n_gates, n_ops, gates_off, ops_off, result_loc,
-//                     lds_need, glb_need, n_sliced, ops_words, 0, 0, 0
+//                     lds_need, glb_need (low word), n_sliced, ops_words,
+//                     glb_need (high word), 0, 0
 //   gates: n_gates * kGateWords (distinct gate tensors only; equal leaves share one)
 //   ops:   one record per op; the item stages them in LDS after its data
 // LDS element 0 holds 1+0i, element 1 holds 0+0i, element 2 is a write sink;
@@ -547,46 +548,67 @@ __global__ __launch_bounds__(ITEM_THREADS) void exec_kernel(
 
 __global__ void zero_u32(uint32_t* p) { *p = 0u; }
 
-// Neumaier-compensated FP64 complex sums: one thread per segment, then one
-// block for the total (segment count is small: edges of one batch).
+// Neumaier-compensated FP64 complex sums in a fixed order (deterministic):
+// one wave per segment, lanes take strided items, then a shuffle tree combines
+// (sum, compensation) pairs; the total is one block over the segment sums.
 __device__ __forceinline__ void neumaier(double& s, double& c, double x) {
     const double t = s + x;
     c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
     s = t;
 }
 
-__global__ void reduce_segments(const double* item_out, const int32_t* seg, int32_t n_seg,
-                                double* term_out) {
-    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void wave_combine(double& s, double& c) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double os = __shfl_down(s, off, 64);
+        const double oc = __shfl_down(c, off, 64);
+        neumaier(s, c, os);
+        c += oc;
+    }
+}
+
+__global__ __launch_bounds__(64) void reduce_segments(const double* __restrict__ item_out,
+                                                      const int32_t* __restrict__ seg, int32_t n_seg,
+                                                      double* __restrict__ term_out) {
+    const int32_t s = blockIdx.x;
     if (s >= n_seg) return;
     double sr = 0, cr = 0, si = 0, ci = 0;
-    for (int32_t i = seg[s]; i < seg[s + 1]; ++i) {
+    for (int32_t i = seg[s] + (int32_t)threadIdx.x; i < seg[s + 1]; i += 64) {
         neumaier(sr, cr, item_out[2 * (size_t)i]);
         neumaier(si, ci, item_out[2 * (size_t)i + 1]);
     }
-    term_out[2 * (size_t)s] = sr + cr;
-    term_out[2 * (size_t)s + 1] = si + ci;
+    wave_combine(sr, cr);
+    wave_combine(si, ci);
+    if (threadIdx.x == 0) {
+        term_out[2 * (size_t)s] = sr + cr;
+        term_out[2 * (size_t)s + 1] = si + ci;
+    }
 }
 
-__global__ __launch_bounds__(256) void reduce_total(const double* term_out, int32_t n_seg, double* total) {
-    __shared__ double sh[4][256];
+__global__ __launch_bounds__(256) void reduce_total(const double* __restrict__ term_out, int32_t n_seg,
+                                                    double* __restrict__ total) {
+    __shared__ double sh[4][4];
     double sr = 0, cr = 0, si = 0, ci = 0;
     for (int32_t s = threadIdx.x; s < n_seg; s += blockDim.x) {
         neumaier(sr, cr, term_out[2 * (size_t)s]);
         neumaier(si, ci, term_out[2 * (size_t)s + 1]);
     }
-    sh[0][threadIdx.x] = sr;
-    sh[1][threadIdx.x] = cr;
-    sh[2][threadIdx.x] = si;
-    sh[3][threadIdx.x] = ci;
+    wave_combine(sr, cr);
+    wave_combine(si, ci);
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        sh[w][0] = sr;
+        sh[w][1] = cr;
+        sh[w][2] = si;
+        sh[w][3] = ci;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         double ar = 0, acr = 0, ai = 0, aci = 0;
-        for (int t = 0; t < 256; ++t) {
-            neumaier(ar, acr, sh[0][t]);
-            acr += sh[1][t];
-            neumaier(ai, aci, sh[2][t]);
-            aci += sh[3][t];
+        for (int k = 0; k < 4; ++k) {
+            neumaier(ar, acr, sh[k][0]);
+            acr += sh[k][1];
+            neumaier(ai, aci, sh[k][2]);
+            aci += sh[k][3];
         }
         total[0] = ar + acr;
         total[1] = ai + aci;
@@ -769,8 +791,8 @@ int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segmen
     if (n_segments < 1 || n_segments > (1 << 20))
         return set_error(QB_ERR_INVALID, "n_segments=%d outside [1, 2^20]", n_segments);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(reduce_segments, dim3((n_segments + 255) / 256), dim3(256), 0, st, item_out, seg,
-                       n_segments, term_out);
+    hipLaunchKernelGGL(reduce_segments, dim3(n_segments), dim3(64), 0, st, item_out, seg, n_segments,
+                       term_out);
     QB_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_total, dim3(1), dim3(256), 0, st, term_out, n_segments, total);
     QB_HIP(hipGetLastError());

@@ -441,7 +441,8 @@ class PlanCompiler:
         steps_off = gates_off + GATE_WORDS * len(gates)
         steps_words = sum(len(w) for w in op_words)
         words = [len(gates), n_ops, gates_off, steps_off, _i32(result.buf.loc),
-                 lds.peak, glb.peak, len(par_index), steps_words, 0, 0, 0]
+                 lds.peak, _i32(glb.peak & 0xFFFFFFFF), len(par_index), steps_words,
+                 _i32(glb.peak >> 32), 0, 0]
         for g in gates:
             words += g
         for w in op_words:
