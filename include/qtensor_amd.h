@@ -9,13 +9,14 @@
  * Reference interface each entry point replaces (paths in DaniloZZZ/QTensor):
  *   qb_step           -> BucketBackend.process_bucket   qtensor/ProcessingFrameworks.py:14-15
  *                        (NumpyBackend :29-31, CMKLExtendedBackend :45-147)
- *   qb_build_gates    -> the gate tensors that get_sliced_buckets materialises from
- *                        data_dict (ProcessingFrameworks.py:17-18,33-34), built on the
- *                        device in FP64 from (gamma, beta)
  *   qb_execute        -> the per-edge loop QAOASimulator.energy_expectation drives
  *                        (qtensor/QAOASimulator.py:44-66) around
  *                        bucket_elimination(process_bucket) (qtensor/Simulate.py:90-93),
- *                        one work item per (edge, slice) (FeynmanSimulator.py:49-58)
+ *                        one work item per (edge, slice) (FeynmanSimulator.py:49-58);
+ *                        each item first builds, in FP64 from (gamma, beta), the gate
+ *                        tensors that get_sliced_buckets materialises from data_dict
+ *                        (ProcessingFrameworks.py:17-18,33-34)
+ *   qb_exec_slots     -> (no reference counterpart: sizes the executor's scratch)
  *   qb_reduce_terms   -> sum of slice partials (FeynmanSimulator.py:83) and of edge
  *                        terms (QAOASimulator.py:60), in FP64
  *   qb_init / qb_shutdown / qb_last_error -> backend construction and the Python

@@ -12,6 +12,9 @@
   removed variables are sliced and go last in the order.
 * ``TamakiOptimizer``    -- ``Optimizer.py:161-176``: PACE-2017 heuristic via an
   external ``tw-heuristic`` executable (Java), used when it is on ``PATH``.
+* ``RGreedyOptimizer``   -- ``qtensor/optimisation/RGreedy.py:15-77``: pick the next
+  node with probability proportional to exp(-(degree - min degree)/temp), keep the
+  narrowest of ``repeats`` samples (seeded here).
 """
 import os
 import shutil
@@ -233,3 +236,39 @@ def _peo_from_td(text, nodes, graph):
                 done.add(v)
     peo += [n for n in nodes if n not in done]
     return peo, elimination_widths(graph, peo)
+
+
+def rgreedy_peo(graph, temp, rng):
+    """One randomized-greedy elimination: Boltzmann choice over neighbour counts."""
+    adj = _adjacency(graph)
+    peo, degs = [], []
+    while adj:
+        nodes = list(adj)
+        d = np.array([len(adj[n]) for n in nodes], dtype=float)
+        w = np.exp(-(d - d.min()) / temp)
+        k = int(rng.choice(len(nodes), p=w / w.sum()))
+        best = nodes[k]
+        nb = adj.pop(best)
+        peo.append(best)
+        degs.append(len(nb))
+        for u in nb:
+            s = adj[u]
+            s.discard(best)
+            s.update(nb)
+            s.discard(u)
+    return peo, degs
+
+
+class RGreedyOptimizer(OrderingOptimizer):
+    def __init__(self, temp=0.002, repeats=10, seed=None):
+        self.temp = temp
+        self.repeats = repeats
+        self.rng = np.random.default_rng(seed)
+
+    def _get_ordering_ints(self, graph):
+        best = None
+        for _ in range(self.repeats):
+            peo, degs = rgreedy_peo(graph, self.temp, self.rng)
+            if best is None or max(degs) < max(best[1]):
+                best = (peo, degs)
+        return best

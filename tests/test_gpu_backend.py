@@ -101,3 +101,13 @@ def test_perf_backend_reports(gpu):
     ref = QtreeSimulator(bucket_backend=npf.NumpyBackend()).simulate(comp.circuit)
     assert abs(amp[0] - ref[0]) < 1e-5
     assert "Total bucket contractions" in be.gen_report()
+
+
+def test_batch_vars_through_plain_backend(gpu):
+    """simulate_batch(batch_vars=2): the no_sum tail bucket runs on the GPU."""
+    G = nx.random_regular_graph(4, 10, seed=3)
+    comp = QtreeQAOAComposer(G, gamma=[np.pi / 3] * 2, beta=[np.pi / 2] * 2)
+    comp.ansatz_state()
+    amps = QtreeSimulator(bucket_backend=HipBackend()).simulate_batch(comp.circuit, batch_vars=2)
+    ref = QtreeSimulator(bucket_backend=npf.NumpyBackend()).simulate_batch(comp.circuit, batch_vars=2)
+    assert np.allclose(amps, ref, atol=1e-5)

@@ -171,3 +171,28 @@ def test_backend_pickles_without_device_state():
     be = HipBackend()
     be2 = pickle.loads(pickle.dumps(be))
     assert be2.dtype == "complex64" and not be2._ready
+
+
+def test_rgreedy_orders_are_valid_and_seeded():
+    from qtensor_amd.ordering import RGreedyOptimizer, rgreedy_peo
+    g = nx.random_regular_graph(3, 30, seed=2)
+    peo, degs = rgreedy_peo(g, 1.0, np.random.default_rng(3))
+    assert sorted(peo) == sorted(g.nodes()) and elimination_widths(g, peo) == degs
+    p1, d1 = RGreedyOptimizer(temp=2.0, repeats=3, seed=5)._get_ordering_ints(g)
+    p2, d2 = RGreedyOptimizer(temp=2.0, repeats=3, seed=5)._get_ordering_ints(g)
+    assert p1 == p2 and max(d1) >= max(greedy_peo(g)[1]) - 3
+
+
+def test_batch_vars_amplitudes_match_single_amplitude():
+    """simulate_batch(batch_vars=2)[0] == simulate() (reference test_simulators.py:44-65)."""
+    G = nx.random_regular_graph(4, 10, seed=3)
+    comp = QtreeQAOAComposer(G, gamma=[np.pi / 3] * 2, beta=[np.pi / 2] * 2)
+    comp.ansatz_state()
+    sim = QtreeSimulator(bucket_backend=npf.NumpyBackend())
+    amp = sim.simulate(comp.circuit)
+    amps = sim.simulate_batch(comp.circuit, batch_vars=2)
+    assert len(amps) == 4 and abs(amp[0] - amps[0]) < 1e-10
+    nq = G.number_of_nodes()
+    sv = statevector.qaoa_state(nq, [tuple(e) for e in G.edges()], [np.pi / 3] * 2, [np.pi / 2] * 2)
+    # free qubits 0,1 are the two most significant bits of the statevector index
+    assert np.allclose(amps, [sv[k << (nq - 2)] for k in range(4)], atol=1e-10)
