@@ -1,0 +1,54 @@
+"""Continuous-batching server: results alone vs in a mix, admission, statevector."""
+import networkx as nx
+import numpy as np
+import pytest
+
+from oracle import statevector
+from qtensor_amd.batched import LaunchShape, LightconeEngine
+from qtensor_amd.serving import EnergyServer, poisson_trace
+
+pytestmark = pytest.mark.gpu
+
+
+def _pool():
+    return [(f"g{k}", nx.random_regular_graph(3, n, seed=k)) for k, n in enumerate([12, 16, 20, 40, 64])]
+
+
+def test_results_same_alone_and_in_mix(gpu):
+    eng = LightconeEngine(shape=LaunchShape(64, 4096, 2048), max_tw=11)
+    server = EnergyServer(eng, n_streams=2).start()
+    rng = np.random.default_rng(1)
+    try:
+        subs = poisson_trace(server, _pool(), rate=2000.0, n_requests=40, rng=rng)
+    finally:
+        server.stop()
+    st = server.stats()
+    assert st["requests"] == 40 and st["p99_ms"] >= st["p50_ms"] > 0
+    for fut, key, G, gamma, beta in subs:
+        C, zz = fut.result()
+        C1, zz1 = eng.energy(G, gamma, beta)            # the same request alone
+        assert max(abs(a - b) for a, b in zip(zz, zz1)) <= 1e-12
+        if G.number_of_nodes() <= 16:
+            sv = statevector.zz_expectations(G, gamma, beta)
+            assert max(abs(z - sv[e]) for z, e in zip(zz, G.edges())) <= 1e-5
+
+
+def test_admission_rejects_and_queues(gpu):
+    eng = LightconeEngine(shape=LaunchShape(64, 4096, 2048), max_tw=11)
+    big = nx.random_regular_graph(3, 64, seed=5)
+    small = nx.random_regular_graph(3, 12, seed=6)
+    wl_small = eng.workload(small, 1)
+    from qtensor_amd.serving import predicted_bytes
+    budget = 3 * predicted_bytes(wl_small)
+    server = EnergyServer(eng, n_streams=2, hbm_budget=budget).start()
+    try:
+        f_big = server.submit(big, [0.1, 0.2, 0.3], [0.3, 0.2, 0.1], key="big")
+        fs = [server.submit(small, [0.1 * k], [0.2], key="small") for k in range(1, 8)]
+        with pytest.raises(MemoryError):
+            f_big.result(timeout=60)
+        for k, f in enumerate(fs, start=1):
+            C, zz = f.result(timeout=60)
+            sv = statevector.zz_expectations(small, [0.1 * k], [0.2])
+            assert max(abs(z - sv[e]) for z, e in zip(zz, small.edges())) <= 1e-5
+    finally:
+        server.stop()
