@@ -17,6 +17,9 @@
  *                        tensors that get_sliced_buckets materialises from data_dict
  *                        (ProcessingFrameworks.py:17-18,33-34)
  *   qb_exec_slots     -> (no reference counterpart: sizes the executor's scratch)
+ *   qb_wide_execute   -> bucket_elimination(process_bucket) over one large network
+ *                        (qtensor/Simulate.py:90-93) or one slice of it
+ *                        (qtensor/FeynmanSimulator.py:49-58), all steps on the device
  *   qb_reduce_terms   -> sum of slice partials (FeynmanSimulator.py:83) and of edge
  *                        terms (QAOASimulator.py:60), in FP64
  *   qb_init / qb_shutdown / qb_last_error -> backend construction and the Python
@@ -114,6 +117,29 @@ typedef struct qb_exec_desc {
 int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n_slots);
 
 int qb_execute(const qb_exec_desc* desc, void* stream);
+
+/*
+ * Wide plan: one compiled lightcone plan whose tensors (gates included) all
+ * live in `arena` (global memory), for networks too large for the executor's
+ * LDS tiers.  Builds the gate tensors from `params` for the slice assignment
+ * `slice_bits`, launches one grid-wide step kernel per step op in order on
+ * `stream`, and writes the scalar result as complex128 to out[0..1].  The host
+ * copy of the plan is parsed and every tensor range is checked against
+ * `arena_elems` before any launch.
+ */
+typedef struct qb_wide_desc {
+    const int32_t* plan_host;    /* host: plan words */
+    int64_t plan_words;
+    const int32_t* plan_dev;     /* device: the same words */
+    const double* params;        /* device: angle vector (FP64) */
+    void* arena;                 /* device: arena_elems elements of dtype */
+    int64_t arena_elems;
+    uint32_t slice_bits;
+    int32_t dtype;
+    double* out;                 /* device: 2 doubles */
+} qb_wide_desc;
+
+int qb_wide_execute(const qb_wide_desc* desc, void* stream);
 
 /*
  * FP64 reductions: segment s covers items [seg[s], seg[s+1]); term_out[s] is

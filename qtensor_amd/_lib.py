@@ -22,7 +22,7 @@ QB_MAX_RANK = 31
 QB_ITEM_WORDS = 4
 
 EXPORTED = ("qb_version", "qb_init", "qb_shutdown", "qb_last_error", "qb_step",
-            "qb_exec_slots", "qb_execute", "qb_reduce_terms")
+            "qb_exec_slots", "qb_execute", "qb_wide_execute", "qb_reduce_terms")
 
 
 class StepDesc(ctypes.Structure):
@@ -39,6 +39,14 @@ class ExecDesc(ctypes.Structure):
                 ("scratch", ctypes.c_void_p), ("scratch_elems", ctypes.c_int64),
                 ("n_slots", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("item_out", ctypes.c_void_p), ("queue", ctypes.c_void_p)]
+
+
+class WideDesc(ctypes.Structure):
+    _fields_ = [("plan_host", ctypes.c_void_p), ("plan_words", ctypes.c_int64),
+                ("plan_dev", ctypes.c_void_p), ("params", ctypes.c_void_p),
+                ("arena", ctypes.c_void_p), ("arena_elems", ctypes.c_int64),
+                ("slice_bits", ctypes.c_uint32), ("dtype", ctypes.c_int32),
+                ("out", ctypes.c_void_p)]
 
 
 class QBError(RuntimeError):
@@ -70,6 +78,7 @@ def load():
         lib.qb_exec_slots.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_int32)]
         lib.qb_execute.argtypes = [ctypes.POINTER(ExecDesc), ctypes.c_void_p]
+        lib.qb_wide_execute.argtypes = [ctypes.POINTER(WideDesc), ctypes.c_void_p]
         lib.qb_reduce_terms.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib = lib

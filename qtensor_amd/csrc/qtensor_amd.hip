@@ -175,6 +175,59 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs a) {
     }
 }
 
+// Specialised plain step for K <= 4 inputs.  Thread t of every block handles
+// elements base + t; its low offsets pext(t, mask) are computed once, and the
+// high part pext(base, mask) is block-uniform (scalar).  All input reads of an
+// element are issued before the first product.
+template <int K>
+struct StepArgsK {
+    const void* ptr[K];
+    uint32_t mask[K];   // output bits the input carries (summed bit removed)
+    uint32_t top[K];    // offset of the summed variable = 1
+    uint32_t runs[K];   // pext runs of mask (n_runs << 30 | run0 | run1 << 15)
+    void* out;
+    uint32_t rank_out;
+    uint32_t sum;
+};
+
+__device__ __forceinline__ uint32_t in_offset(uint32_t o, uint32_t mask, uint32_t runs);
+
+template <typename R, int K>
+__global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
+    using V = typename Cx<R>::V;
+    const uint32_t tid = threadIdx.x;
+    uint32_t off0[K], lo_bits[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        off0[j] = in_offset(tid, a.mask[j], a.runs[j]);
+        lo_bits[j] = (uint32_t)__builtin_popcount(a.mask[j] & 255u);
+    }
+    const uint64_t n_out = 1ull << a.rank_out;
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t base = (uint64_t)blockIdx.x * 256u; base < n_out; base += stride) {
+        const uint32_t b8 = (uint32_t)(base >> 8);
+        V x[K], y[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint32_t off = off0[j] + (pext_loop(b8, a.mask[j] >> 8) << lo_bits[j]);
+            const V* p = reinterpret_cast<const V*>(a.ptr[j]);
+            x[j] = p[off];
+            if (a.sum) y[j] = p[off + a.top[j]];
+        }
+        V r = x[0];
+#pragma unroll
+        for (int j = 1; j < K; ++j) r = cmul(r, x[j]);
+        if (a.sum) {
+            V q = y[0];
+#pragma unroll
+            for (int j = 1; j < K; ++j) q = cmul(q, y[j]);
+            r = cadd(r, q);
+        }
+        const uint64_t o = base + tid;
+        if (o < n_out) reinterpret_cast<V*>(a.out)[o] = r;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched executor
 // ---------------------------------------------------------------------------
@@ -272,9 +325,9 @@ __device__ void natural_gate(int32_t kind, double alpha, double* re, double* im)
     }
 }
 
-template <typename R>
-__device__ void build_gate(const int32_t* g, const double* params, uint32_t slice_bits,
-                           QB_LDS typename Cx<R>::V* lds, const typename Cx<R>::V* pool) {
+template <typename R, typename Dst>
+__device__ void build_gate(const int32_t* g, const double* params, uint32_t slice_bits, Dst lds,
+                           const typename Cx<R>::V* pool) {
     using V = typename Cx<R>::V;
     const int32_t kind = g[0];
     const int32_t dst = g[1];
@@ -672,6 +725,57 @@ int slots_for(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n
     return QB_OK;
 }
 
+// Wide plans (every tensor in global memory): gate build and result copy.
+template <typename R>
+__global__ void build_gates_global(const int32_t* __restrict__ gates, int32_t n_gates,
+                                   const double* __restrict__ params, uint32_t slice_bits,
+                                   void* __restrict__ arena) {
+    using V = typename Cx<R>::V;
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n_gates)
+        build_gate<R>(gates + (size_t)g * kGateWords, params, slice_bits, reinterpret_cast<V*>(arena),
+                      (const V*)nullptr);
+}
+
+template <typename R>
+__global__ void store_result(const void* __restrict__ arena, uint32_t loc, double* __restrict__ out) {
+    using V = typename Cx<R>::V;
+    const V r = reinterpret_cast<const V*>(arena)[loc];
+    out[0] = (double)r.x;
+    out[1] = (double)r.y;
+}
+
+// Host: pext runs word of a mask (n_runs << 30 | run0 | run1 << 15); 3 << 30 when
+// the mask has more than two runs.
+uint32_t runs_word(uint32_t mask) {
+    uint8_t src[2], w[2], dst[2];
+    const int n = mask_runs(mask, src, w, dst, 2);
+    if (n > 2) return 3u << 30;
+    uint32_t r = (uint32_t)n << 30;
+    for (int k = 0; k < n; ++k) r |= ((uint32_t)src[k] | ((uint32_t)w[k] << 5) | ((uint32_t)dst[k] << 10)) << (15 * k);
+    return r;
+}
+
+template <int K>
+int launch_step_k(const StepArgs& a, const uint32_t* runs, int32_t dtype, int grid, hipStream_t st) {
+    StepArgsK<K> k;
+    for (int j = 0; j < K; ++j) {
+        k.ptr[j] = a.in[j].ptr;
+        k.mask[j] = a.in[j].mask;
+        k.top[j] = a.in[j].top;
+        k.runs[j] = runs[j];
+    }
+    k.out = a.out;
+    k.rank_out = a.rank_out;
+    k.sum = a.n_sum == 2 ? 1u : 0u;
+    if (dtype == QB_C128)
+        hipLaunchKernelGGL((step_kernel_k<double, K>), dim3(grid), dim3(256), 0, st, k);
+    else
+        hipLaunchKernelGGL((step_kernel_k<float, K>), dim3(grid), dim3(256), 0, st, k);
+    QB_HIP(hipGetLastError());
+    return QB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -732,6 +836,16 @@ int qb_step(const qb_step_desc* d, const void* const* inputs, void* out, void* s
     const uint64_t cap = (uint64_t)g_num_cus * 8;
     const int grid = (int)(want < cap ? (want > 0 ? want : 1) : cap);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (d->n_inputs <= 4) {
+        uint32_t runs[QB_MAX_STEP_INPUTS];
+        for (int j = 0; j < d->n_inputs; ++j) runs[j] = runs_word(a.in[j].mask);
+        switch (d->n_inputs) {
+        case 1: return launch_step_k<1>(a, runs, d->dtype, grid, st);
+        case 2: return launch_step_k<2>(a, runs, d->dtype, grid, st);
+        case 3: return launch_step_k<3>(a, runs, d->dtype, grid, st);
+        default: return launch_step_k<4>(a, runs, d->dtype, grid, st);
+        }
+    }
     if (d->dtype == QB_C128)
         hipLaunchKernelGGL(step_kernel<double>, dim3(grid), dim3(256), 0, st, a);
     else
@@ -782,6 +896,90 @@ int qb_execute(const qb_exec_desc* d, void* stream) {
     const int lds_bytes = d->lds_elems * elem * groups;
     return d->dtype == QB_C128 ? exec_launch_dispatch<double>(d->item_threads, a, grid, lds_bytes, st)
                                : exec_launch_dispatch<float>(d->item_threads, a, grid, lds_bytes, st);
+}
+
+int qb_wide_execute(const qb_wide_desc* d, void* stream) {
+    if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
+    if (!d || !d->plan_host || !d->plan_dev || !d->params || !d->arena || !d->out)
+        return set_error(QB_ERR_INVALID, "null pointer in wide descriptor");
+    if (d->dtype != QB_C64 && d->dtype != QB_C128) return set_error(QB_ERR_INVALID, "dtype %d", d->dtype);
+    const int32_t* w = d->plan_host;
+    if (d->plan_words < kPlanHdr) return set_error(QB_ERR_INVALID, "plan shorter than its header");
+    const int32_t n_gates = w[0], n_ops = w[1], gates_off = w[2], ops_off = w[3];
+    const uint32_t result_loc = (uint32_t)w[4];
+    const int64_t elem = d->dtype == QB_C128 ? 16 : 8;
+    const int64_t cap = d->arena_elems;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if ((int64_t)gates_off + (int64_t)n_gates * kGateWords > d->plan_words)
+        return set_error(QB_ERR_INVALID, "gate table outside the plan");
+    for (int32_t g = 0; g < n_gates; ++g) {
+        const int32_t* e = w + gates_off + (size_t)g * kGateWords;
+        if (e[1] < 0 || (int64_t)e[1] + e[2] > cap) return set_error(QB_ERR_INVALID, "gate %d outside the arena", g);
+    }
+    if (n_gates > 0) {
+        const int blocks = (n_gates + 255) / 256;
+        if (d->dtype == QB_C128)
+            hipLaunchKernelGGL(build_gates_global<double>, dim3(blocks), dim3(256), 0, st, d->plan_dev + gates_off,
+                               n_gates, d->params, d->slice_bits, d->arena);
+        else
+            hipLaunchKernelGGL(build_gates_global<float>, dim3(blocks), dim3(256), 0, st, d->plan_dev + gates_off,
+                               n_gates, d->params, d->slice_bits, d->arena);
+        QB_HIP(hipGetLastError());
+    }
+    char* arena = reinterpret_cast<char*>(d->arena);
+    int64_t pos = ops_off;
+    for (int32_t k = 0; k < n_ops; ++k) {
+        if (pos + kStepHdr > d->plan_words) return set_error(QB_ERR_INVALID, "op %d outside the plan", k);
+        const uint32_t w0 = (uint32_t)w[pos];
+        if (w0 & kOpGather) return set_error(QB_ERR_INVALID, "wide plans take step ops only");
+        const uint32_t n_in = w0 & 15u, rank_out = (w0 >> 4) & 31u, sum = (w0 >> 9) & 1u;
+        const uint32_t out_loc = (uint32_t)w[pos + 2];
+        if (n_in < 1 || n_in > (uint32_t)kExecMaxIn || pos + kStepHdr + kInWords * (int64_t)n_in > d->plan_words)
+            return set_error(QB_ERR_INVALID, "op %d has %u inputs", k, n_in);
+        if (!(out_loc & kGlobal) || (int64_t)(out_loc & ~kGlobal) + (1ll << rank_out) > cap)
+            return set_error(QB_ERR_INVALID, "op %d output outside the arena", k);
+        StepArgs a;
+        memset(&a, 0, sizeof(a));
+        a.n_in = n_in;
+        a.n_sum = sum ? 2u : 1u;
+        a.rank_out = rank_out;
+        a.out = arena + (int64_t)(out_loc & ~kGlobal) * elem;
+        uint32_t runs[kExecMaxIn];
+        for (uint32_t j = 0; j < n_in; ++j) {
+            const int32_t* in = w + pos + kStepHdr + kInWords * j;
+            const uint32_t loc = (uint32_t)in[0];
+            const uint32_t top = (uint32_t)in[1], mask = (uint32_t)in[2];
+            const int64_t n_elems = (1ll << __builtin_popcount(mask)) * (top ? 2 : 1);
+            if (!(loc & kGlobal) || (int64_t)(loc & ~kGlobal) + n_elems > cap)
+                return set_error(QB_ERR_INVALID, "op %d input %u outside the arena", k, j);
+            if (mask >> rank_out) return set_error(QB_ERR_INVALID, "op %d input %u mask beyond rank", k, j);
+            a.in[j].ptr = arena + (int64_t)(loc & ~kGlobal) * elem;
+            a.in[j].mask = mask;
+            a.in[j].top = top;
+            runs[j] = runs_word(mask);
+        }
+        const uint64_t n_out = 1ull << rank_out;
+        const uint64_t want = (n_out + 255) / 256;
+        const uint64_t gcap = (uint64_t)g_num_cus * 8;
+        const int grid = (int)(want < gcap ? (want > 0 ? want : 1) : gcap);
+        int rc;
+        switch (n_in) {
+        case 1: rc = launch_step_k<1>(a, runs, d->dtype, grid, st); break;
+        case 2: rc = launch_step_k<2>(a, runs, d->dtype, grid, st); break;
+        case 3: rc = launch_step_k<3>(a, runs, d->dtype, grid, st); break;
+        default: rc = launch_step_k<4>(a, runs, d->dtype, grid, st); break;
+        }
+        if (rc) return rc;
+        pos += w0 >> 16;
+    }
+    if (!(result_loc & kGlobal) || (int64_t)(result_loc & ~kGlobal) >= cap)
+        return set_error(QB_ERR_INVALID, "result outside the arena");
+    if (d->dtype == QB_C128)
+        hipLaunchKernelGGL(store_result<double>, dim3(1), dim3(1), 0, st, d->arena, result_loc & ~kGlobal, d->out);
+    else
+        hipLaunchKernelGGL(store_result<float>, dim3(1), dim3(1), 0, st, d->arena, result_loc & ~kGlobal, d->out);
+    QB_HIP(hipGetLastError());
+    return QB_OK;
 }
 
 int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segments,

@@ -222,7 +222,7 @@ class PlanCompiler:
     """
 
     def __init__(self, lds_tensor_limit=1 << 12, lds_capacity=None, elem_bytes=8, lanes=64,
-                 gather=True, window=None):
+                 gather=True, window=None, gate_space="lds"):
         self.lds_tensor_limit = lds_tensor_limit
         self.lds_capacity = lds_capacity      # elements available per item, or None
         self.elem_bytes = elem_bytes
@@ -230,6 +230,10 @@ class PlanCompiler:
         self.gather = gather
         # a step may join one of the last `window` ops (None: any op after its inputs)
         self.window = window
+        # "global": gates and every intermediate live in global memory (wide plans)
+        if gate_space not in ("lds", "global"):
+            raise ValueError("gate_space is 'lds' or 'global'")
+        self.gate_space = gate_space
 
     # -- front end: circuit -> reordered symbolic buckets
     @staticmethod
@@ -263,7 +267,8 @@ class PlanCompiler:
 
         # leaves: distinct gate tensors in the LDS gate area
         gates, gate_buf = [], {}
-        gate_area = GATE_BASE
+        wide = self.gate_space == "global"
+        gate_area = 0 if wide else GATE_BASE
         sym = []
         for bucket in buckets:
             nb = []
@@ -271,7 +276,7 @@ class PlanCompiler:
                 node, entry, size = self._leaf(t, data_dict, fixed, par_index, gate_area)
                 sig = tuple(entry[:1] + entry[2:])
                 if sig not in gate_buf:
-                    gate_buf[sig] = Buf(size, loc=gate_area, leaf=True)
+                    gate_buf[sig] = Buf(size, loc=(GLOBAL | gate_area) if wide else gate_area, leaf=True)
                     gates.append(entry)
                     gate_area += size
                 node.buf = gate_buf[sig]
@@ -379,8 +384,10 @@ class PlanCompiler:
         for i, st in enumerate(steps):
             for nd in st.inputs:
                 nd.buf.last_op = max(nd.buf.last_op, op_of[i])
-        lds = _Arena(base=gate_area, align=2)
-        glb = _Arena(base=0, align=64)
+        lds = _Arena(base=0 if wide else gate_area, align=2)
+        glb = _Arena(base=(gate_area + 63) // 64 * 64 if wide else 0, align=64)
+        if wide:
+            lim = 0
         by_last = {}
         for i, st in enumerate(steps):
             for nd in st.inputs:

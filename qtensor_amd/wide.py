@@ -1,0 +1,149 @@
+"""Wide plans: one large lightcone network, every tensor in global memory.
+
+For networks whose intermediates do not fit the executor's LDS tiers (large
+treewidth), the plan compiler emits step ops only, with gates and
+intermediates in one global arena, and ``qb_wide_execute`` runs them as one
+grid-wide kernel per step (HBM-bound for large ranks).  Slices of a sliced
+network (``qtensor/FeynmanSimulator.py:49-58``) run one after the other into
+per-slice FP64 results, summed with Neumaier compensation by
+``qb_reduce_terms`` (``FeynmanSimulator.py:83``).
+
+``forced_width_order`` reproduces a deliberately wide elimination order with
+the reference's randomized greedy rule (``qtensor/optimisation/RGreedy.py``):
+seeded samples at a given temperature until one reaches the target width.
+``slice_vars_for_order`` picks slice variables for a fixed order by how much
+of the wide elimination neighbourhoods they cover.
+"""
+import ctypes
+from collections import Counter
+
+import numpy as np
+
+from . import _lib
+from .circuit import reorder_buckets
+from .ordering import _adjacency, _vars, elimination_widths, greedy_peo, rgreedy_peo
+from .plan import PlanCompiler
+from .tensornet import QtreeTensorNet
+
+
+def forced_width_order(graph, target_tw, temp=8.0, seed=0, max_tries=500):
+    """First seeded randomized-greedy order whose width equals ``target_tw``."""
+    rng = np.random.default_rng(seed)
+    for _ in range(max_tries):
+        peo, degs = rgreedy_peo(graph, temp, rng)
+        if max(degs) == target_tw:
+            return peo
+    raise RuntimeError(f"no order of width {target_tw} in {max_tries} samples at temp {temp}")
+
+
+def _neighbourhoods(graph, order):
+    adj = _adjacency(graph)
+    out = []
+    for n in order:
+        nb = adj.pop(n)
+        out.append((n, set(nb)))
+        for u in nb:
+            s = adj[u]
+            s.discard(n)
+            s.update(nb)
+            s.discard(u)
+    return out
+
+
+def slice_vars_for_order(graph, order, n_vars):
+    """Slice variables for a fixed order: repeatedly take the variable whose
+    presence weighs most in the elimination neighbourhoods within one of the
+    current width (weight 2^width each)."""
+    par = []
+    for _ in range(n_vars):
+        g = graph.copy()
+        g.remove_nodes_from(par)
+        rest = [v for v in order if v not in par]
+        nh = _neighbourhoods(g, rest)
+        tw = max(len(b) for _, b in nh)
+        cnt = Counter()
+        for n, b in nh:
+            if len(b) >= tw - 1:
+                for v in b | {n}:
+                    cnt[v] += 1 << len(b)
+        par.append(cnt.most_common(1)[0][0])
+    return par
+
+
+class WidePlan:
+    def __init__(self, plan, treewidth, n_slice_vars):
+        self.plan = plan
+        self.treewidth = treewidth
+        self.n_slices = 1 << n_slice_vars
+
+
+def compile_wide(circuit, order=None, par_ids=(), lanes=256):
+    """Compile ``circuit`` with elimination ``order`` (variable ids of the line
+    graph; greedy when None) and slice variables ``par_ids`` into a wide plan."""
+    tn = QtreeTensorNet.from_qtree_gates(circuit)
+    graph = tn.get_line_graph()
+    if order is None:
+        order, _ = greedy_peo(graph)
+    par_ids = list(par_ids)
+    rest = [v for v in order if v not in set(par_ids)]
+    g = graph.copy()
+    g.remove_nodes_from(par_ids)
+    tw = max(elimination_widths(g, rest)) if rest else 0
+    peo = list(tn.ket_vars) + list(tn.bra_vars) + _vars(graph, rest) + _vars(graph, par_ids)
+    buckets, perm = reorder_buckets(tn.buckets, peo)
+    pc = PlanCompiler(lds_tensor_limit=0, gather=False, gate_space="global", lanes=lanes)
+    plan = pc.compile(buckets, tn.data_dict, [perm[v] for v in tn.ket_vars],
+                      [perm[v] for v in tn.bra_vars], [perm[v] for v in _vars(graph, par_ids)], tw)
+    return WidePlan(plan, tw, len(par_ids))
+
+
+class WideEngine:
+    """Run wide plans on one GPU (complex64 or complex128)."""
+
+    def __init__(self, device=0, dtype="complex64"):
+        self.device_index = device
+        self.dtype = dtype
+
+    def run(self, wplan, params, stream=None, timing=False):
+        """Sum of the plan over all its slices (complex, FP64 accumulation).
+        With ``timing`` also returns the device time in seconds."""
+        import torch
+        lib = _lib.init(self.device_index)
+        dev = torch.device("cuda", self.device_index)
+        qb_dtype = _lib.QB_C64 if self.dtype == "complex64" else _lib.QB_C128
+        tdtype = torch.complex64 if self.dtype == "complex64" else torch.complex128
+        plan = wplan.plan
+        words = np.ascontiguousarray(plan.words, dtype=np.int32)
+        plan_dev = torch.from_numpy(words).to(dev)
+        arena = torch.empty(max(plan.glb_need, 1), dtype=tdtype, device=dev)
+        prm = torch.tensor([float(x) for x in params], dtype=torch.float64, device=dev)
+        n = wplan.n_slices
+        out = torch.zeros(2 * n, dtype=torch.float64, device=dev)
+        seg = torch.tensor([0, n], dtype=torch.int32, device=dev)
+        term = torch.zeros(2, dtype=torch.float64, device=dev)
+        total = torch.zeros(2, dtype=torch.float64, device=dev)
+        st = stream if stream is not None else torch.cuda.current_stream(dev)
+        h = ctypes.c_void_p(st.cuda_stream)
+        d = _lib.WideDesc()
+        d.plan_host = words.ctypes.data
+        d.plan_words = words.size
+        d.plan_dev = plan_dev.data_ptr()
+        d.params = prm.data_ptr()
+        d.arena = arena.data_ptr()
+        d.arena_elems = arena.numel()
+        d.dtype = qb_dtype
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(st)
+        for s in range(n):
+            d.slice_bits = s
+            d.out = out.data_ptr() + 16 * s
+            _lib.check(lib.qb_wide_execute(ctypes.byref(d), h))
+        _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(seg.data_ptr()), 1,
+                                       ctypes.c_void_p(term.data_ptr()), ctypes.c_void_p(total.data_ptr()), h))
+        ev1.record(st)
+        ev1.synchronize()
+        r = total.cpu().numpy()
+        val = complex(r[0], r[1])
+        if timing:
+            return val, ev0.elapsed_time(ev1) * 1e-3
+        return val

@@ -55,10 +55,21 @@ def pext(x, m):
     return r
 
 
+def pext_vec(o, mask):
+    o = np.atleast_1d(np.asarray(o, dtype=np.int64))
+    r = np.zeros_like(o)
+    bit = 0
+    for b in range(32):
+        if (mask >> b) & 1:
+            r |= ((o >> b) & 1) << bit
+            bit += 1
+    return r
+
+
 def _runs_offset(o, rw, mask):
     nr = rw >> 30
     if nr == 3:
-        return np.array([pext(int(x), mask) for x in np.atleast_1d(o)], dtype=np.int64)
+        return pext_vec(o, mask)
     off = np.zeros_like(np.atleast_1d(o))
     for r in range(nr):
         word = (rw >> (15 * r)) & 0x7FFF
@@ -70,12 +81,15 @@ def _runs_offset(o, rw, mask):
 def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
     """Execute a plan; ``lanes`` must be the item size the plan was compiled for."""
     w = [int(x) for x in np.asarray(words)]
-    n_gates, n_ops, goff, soff, result_loc, lds_need, glb_need = w[:7]
+    n_gates, n_ops, goff, soff, result_loc, lds_need = w[:6]
+    glb_need = (w[6] & 0xFFFFFFFF) | ((w[9] & 0xFFFFFFFF) << 32)
     ops_words = w[8]
     lds = np.full(max(lds_need, P.GATE_BASE), np.nan + 0j, dtype=dtype)
     glb = np.full(max(glb_need, 1), np.nan + 0j, dtype=dtype)
     lds[P.LOC_ONE] = 1
     lds[P.LOC_ZERO] = 0
+    # wide plans (no LDS) keep their gate tensors in the global arena
+    gate_buf = glb if lds_need == 0 else lds
     for g in range(n_gates):
         e = w[goff + g * P.GATE_WORDS: goff + (g + 1) * P.GATE_WORDS]
         kind, dst, n, smap = e[0], e[1], e[2], e[3] & 0xFFFFFFFF
@@ -92,7 +106,7 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
             shift += ((slice_bits >> ((e[11] >> 16) & 0xFF)) & 1) * e[13]
         nat = natural_gate(kind, alpha)
         for k in range(n):
-            lds[dst + k] = nat[(((smap >> (4 * k)) & 15) + shift) & 7]
+            gate_buf[dst + k] = nat[(((smap >> (4 * k)) & 15) + shift) & 7]
     pos = soff
     for _ in range(n_ops):
         w0 = w[pos] & 0xFFFFFFFF

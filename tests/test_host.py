@@ -196,3 +196,26 @@ def test_batch_vars_amplitudes_match_single_amplitude():
     sv = statevector.qaoa_state(nq, [tuple(e) for e in G.edges()], [np.pi / 3] * 2, [np.pi / 2] * 2)
     # free qubits 0,1 are the two most significant bits of the statevector index
     assert np.allclose(amps, [sv[k << (nq - 2)] for k in range(4)], atol=1e-10)
+
+
+def test_wide_plan_program_vs_oracle():
+    """Wide plans (all tensors global, step ops only) interpret to the oracle value,
+    sliced or not, with greedy and with a forced wide order."""
+    from qtensor_amd.wide import compile_wide, forced_width_order, slice_vars_for_order
+    G = nx.random_regular_graph(3, 14, seed=2)
+    g, b = [0.3, 0.7], [0.5, 0.1]
+    e = list(G.edges())[1]
+    comp = QtreeQAOAComposer(G, gamma=g, beta=b)
+    comp.energy_expectation_lightcone(e)
+    ref = statevector.zz_expectations(G, g, b)[e]
+    wp = compile_wide(comp.circuit)
+    assert wp.plan.lds_need == 0 and wp.plan.n_gather == 0
+    assert abs(run_plan(wp.plan.words, g + b, lanes=256) - ref) < 1e-10
+    from qtensor_amd.tensornet import QtreeTensorNet
+    graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+    order = forced_width_order(graph, target_tw=wp.treewidth + 3, temp=3.0, seed=1)
+    par = slice_vars_for_order(graph, order, 2)
+    ws = compile_wide(comp.circuit, order=order, par_ids=par)
+    assert ws.n_slices == 4
+    total = sum(run_plan(ws.plan.words, g + b, slice_bits=s, lanes=256) for s in range(4))
+    assert abs(total - ref) < 1e-10
