@@ -185,16 +185,50 @@ struct StepArgsK {
     uint32_t mask[K];   // output bits the input carries (summed bit removed)
     uint32_t top[K];    // offset of the summed variable = 1
     uint32_t runs[K];   // pext runs of mask (n_runs << 30 | run0 | run1 << 15)
+    uint32_t roff[K][8];// offset of register-bit pattern r in input j
+    uint32_t rdep[8];   // output offset of register-bit pattern r
+    uint32_t hi_runs[K];// pext runs of mask >> 8
+    uint32_t out_hi;    // output bits >= 8 that are not register bits, shifted down by 8
+    uint32_t dep_runs[2];// out_hi as deposit runs: count << 28 | (dst | width << 5) << 10 k
+    uint32_t n_reg;     // register bits (0..3)
     void* out;
-    uint32_t rank_out;
+    uint32_t rank_out;  // output rank without the register bits (loop extent)
     uint32_t sum;
 };
+
+// Deposit of the low bits of x into the bit runs listed in dep (at most 5 runs,
+// lowest first; dep[0] bits 28..30 hold the count).
+__device__ __forceinline__ uint32_t pdep_runs(uint32_t x, const uint32_t* dep) {
+    const uint32_t n = dep[0] >> 28;
+    uint32_t r = 0, src = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 5; ++k) {
+        if (k >= n) break;
+        const uint32_t f = k < 2 ? (dep[0] >> (10 * k)) : (dep[1] >> (10 * (k - 2)));
+        const uint32_t dst = f & 31u, w = (f >> 5) & 31u;
+        r |= __builtin_amdgcn_ubfe(x, src, w) << dst;
+        src += w;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t pdep_loop(uint32_t x, uint32_t m) {
+    uint32_t r = 0;
+    while (m) {
+        const uint32_t low = m & (0u - m);
+        if (x & 1u) r |= low;
+        x >>= 1;
+        m &= m - 1;
+    }
+    return r;
+}
 
 __device__ __forceinline__ uint32_t in_offset(uint32_t o, uint32_t mask, uint32_t runs);
 
 template <typename R, int K>
 __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
     using V = typename Cx<R>::V;
+    constexpr int kU = 4;   // grid-stride iterations in flight per thread
     const uint32_t tid = threadIdx.x;
     uint32_t off0[K], lo_bits[K];
 #pragma unroll
@@ -202,29 +236,59 @@ __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
         off0[j] = in_offset(tid, a.mask[j], a.runs[j]);
         lo_bits[j] = (uint32_t)__builtin_popcount(a.mask[j] & 255u);
     }
+    const uint32_t n_reg = 1u << a.n_reg;
     const uint64_t n_out = 1ull << a.rank_out;
     const uint64_t stride = (uint64_t)gridDim.x * 256u;
-    for (uint64_t base = (uint64_t)blockIdx.x * 256u; base < n_out; base += stride) {
-        const uint32_t b8 = (uint32_t)(base >> 8);
-        V x[K], y[K];
+    V* out = reinterpret_cast<V*>(a.out);
+    const V* p0 = reinterpret_cast<const V*>(a.ptr[0]);
+    for (uint64_t base0 = (uint64_t)blockIdx.x * 256u; base0 < n_out; base0 += kU * stride) {
+        uint32_t off[kU][K];
+        uint64_t o0[kU];
+        bool live[kU];
+        V x0[kU], y0[kU];
+        // offsets and the largest input's reads for kU block iterations
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const uint32_t off = off0[j] + (pext_loop(b8, a.mask[j] >> 8) << lo_bits[j]);
-            const V* p = reinterpret_cast<const V*>(a.ptr[j]);
-            x[j] = p[off];
-            if (a.sum) y[j] = p[off + a.top[j]];
+        for (int u = 0; u < kU; ++u) {
+            const uint64_t base = base0 + (uint64_t)u * stride;
+            live[u] = base + tid < n_out;
+            const uint32_t hb = base < n_out ? pdep_runs((uint32_t)(base >> 8), a.dep_runs) : 0u;
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                off[u][j] = off0[j] + (in_offset(hb, a.mask[j] >> 8, a.hi_runs[j]) << lo_bits[j]);
+            o0[u] = ((uint64_t)hb << 8) + tid;
+            if (live[u]) {
+                x0[u] = p0[off[u][0]];
+                if (a.sum) y0[u] = p0[off[u][0] + a.top[0]];
+            }
         }
-        V r = x[0];
 #pragma unroll
-        for (int j = 1; j < K; ++j) r = cmul(r, x[j]);
-        if (a.sum) {
-            V q = y[0];
+        for (int u = 0; u < kU; ++u) {
+            if (!live[u]) continue;
+            // input 0 is the largest: it carries no register bit, so it is read once
+            // for all register patterns
 #pragma unroll
-            for (int j = 1; j < K; ++j) q = cmul(q, y[j]);
-            r = cadd(r, q);
+            for (uint32_t r = 0; r < 8; ++r) {
+                if (r >= n_reg) break;
+                V x[K], y[K];
+#pragma unroll
+                for (int j = 1; j < K; ++j) {
+                    const V* p = reinterpret_cast<const V*>(a.ptr[j]);
+                    const uint32_t oj = off[u][j] + a.roff[j][r];
+                    x[j] = p[oj];
+                    if (a.sum) y[j] = p[oj + a.top[j]];
+                }
+                V v = x0[u];
+#pragma unroll
+                for (int j = 1; j < K; ++j) v = cmul(v, x[j]);
+                if (a.sum) {
+                    V q = y0[u];
+#pragma unroll
+                    for (int j = 1; j < K; ++j) q = cmul(q, y[j]);
+                    v = cadd(v, q);
+                }
+                out[o0[u] + a.rdep[r]] = v;
+            }
         }
-        const uint64_t o = base + tid;
-        if (o < n_out) reinterpret_cast<V*>(a.out)[o] = r;
     }
 }
 
@@ -756,18 +820,89 @@ uint32_t runs_word(uint32_t mask) {
     return r;
 }
 
+uint32_t pext_host(uint32_t x, uint32_t m) {
+    uint32_t r = 0, bit = 1;
+    for (; m; m &= m - 1, bit <<= 1)
+        if (x & m & (0u - m)) r |= bit;
+    return r;
+}
+
+uint32_t pdep_host(uint32_t x, uint32_t m) {
+    uint32_t r = 0;
+    for (; m; m &= m - 1, x >>= 1)
+        if (x & 1u) r |= m & (0u - m);
+    return r;
+}
+
+// Launch the plain step for K <= 4 inputs.  Inputs are reordered so the largest
+// comes first; up to three output bits >= 8 that the largest input does not
+// carry become register bits (each thread reads the large input once and
+// writes the 2^n outputs that share it).  The loop runs over the remaining
+// ("kept") output bits; an element's offset in input j is
+// pext(tid, m_j & 255) + pext(pdep(base >> 8, kept >> 8), m_j >> 8) << popc(m_j & 255)
+// plus the register pattern's pext(pdep(r, reg), m_j).
 template <int K>
-int launch_step_k(const StepArgs& a, const uint32_t* runs, int32_t dtype, int grid, hipStream_t st) {
-    StepArgsK<K> k;
-    for (int j = 0; j < K; ++j) {
-        k.ptr[j] = a.in[j].ptr;
-        k.mask[j] = a.in[j].mask;
-        k.top[j] = a.in[j].top;
-        k.runs[j] = runs[j];
+int launch_step_k(const StepArgs& a, const uint32_t* runs_unused, int32_t dtype, int grid_unused, hipStream_t st) {
+    (void)runs_unused;
+    (void)grid_unused;
+    int order[K];
+    for (int j = 0; j < K; ++j) order[j] = j;
+    int big = 0;
+    for (int j = 1; j < K; ++j) {
+        const int sj = __builtin_popcount(a.in[j].mask) + (a.in[j].top ? 1 : 0);
+        const int sb = __builtin_popcount(a.in[big].mask) + (a.in[big].top ? 1 : 0);
+        if (sj > sb) big = j;
     }
+    order[0] = big;
+    order[big] = 0;
+    const uint32_t full = a.rank_out >= 32 ? 0xffffffffu : ((1u << a.rank_out) - 1u);
+    uint32_t missing = full & ~a.in[big].mask & ~255u;
+    uint32_t reg = 0;
+    int n_reg = 0;
+    if (a.rank_out >= 11) {
+        for (int b = 31; b >= 8 && n_reg < 3; --b)
+            if ((missing >> b) & 1u) {
+                reg |= 1u << b;
+                ++n_reg;
+            }
+    }
+    const uint32_t keep = full & ~reg;
+    StepArgsK<K> k;
+    memset(&k, 0, sizeof(k));
+    for (int j = 0; j < K; ++j) {
+        const StepInput& in = a.in[order[j]];
+        k.ptr[j] = in.ptr;
+        k.top[j] = in.top;
+        k.mask[j] = in.mask;
+        k.runs[j] = runs_word(in.mask & 255u);
+        for (uint32_t r = 0; r < (1u << n_reg); ++r) k.roff[j][r] = pext_host(pdep_host(r, reg), in.mask);
+    }
+    for (uint32_t r = 0; r < (1u << n_reg); ++r) k.rdep[r] = pdep_host(r, reg);
+    for (int j = 0; j < K; ++j) k.hi_runs[j] = runs_word(k.mask[j] >> 8);
+    k.out_hi = keep >> 8;
+    {
+        // keep >> 8 has at most n_reg + 1 runs
+        uint32_t m = k.out_hi, n = 0, w0 = 0, w1 = 0;
+        for (uint32_t b = 0; b < 32 && m;) {
+            if (!((m >> b) & 1u)) { ++b; continue; }
+            uint32_t s0 = b;
+            while (b < 32 && ((m >> b) & 1u)) ++b;
+            const uint32_t f = s0 | ((b - s0) << 5);
+            if (n < 2) w0 |= f << (10 * n); else w1 |= f << (10 * (n - 2));
+            ++n;
+            m &= ~(((b < 32) ? (1u << b) : 0u) - (1u << s0));
+        }
+        k.dep_runs[0] = w0 | (n << 28);
+        k.dep_runs[1] = w1;
+    }
+    k.n_reg = (uint32_t)n_reg;
     k.out = a.out;
-    k.rank_out = a.rank_out;
+    k.rank_out = a.rank_out - (uint32_t)n_reg;
     k.sum = a.n_sum == 2 ? 1u : 0u;
+    const uint64_t n_out = 1ull << k.rank_out;
+    const uint64_t want = (n_out + 255) / 256;
+    const uint64_t cap = (uint64_t)g_num_cus * 8;
+    const int grid = (int)(want < cap ? (want > 0 ? want : 1) : cap);
     if (dtype == QB_C128)
         hipLaunchKernelGGL((step_kernel_k<double, K>), dim3(grid), dim3(256), 0, st, k);
     else

@@ -104,43 +104,60 @@ class WideEngine:
         self.device_index = device
         self.dtype = dtype
 
-    def run(self, wplan, params, stream=None, timing=False):
+    def run(self, wplan, params, stream=None, timing=False, n_streams=4):
         """Sum of the plan over all its slices (complex, FP64 accumulation).
-        With ``timing`` also returns the device time in seconds."""
+
+        Slices are independent: they are spread over ``n_streams`` streams, each
+        with a private arena, so the latency-bound small steps of one slice
+        overlap the bandwidth-bound large steps of another.  With ``timing``
+        also returns the device time in seconds."""
         import torch
         lib = _lib.init(self.device_index)
         dev = torch.device("cuda", self.device_index)
         qb_dtype = _lib.QB_C64 if self.dtype == "complex64" else _lib.QB_C128
         tdtype = torch.complex64 if self.dtype == "complex64" else torch.complex128
         plan = wplan.plan
+        n = wplan.n_slices
+        n_streams = max(1, min(n_streams, n))
         words = np.ascontiguousarray(plan.words, dtype=np.int32)
         plan_dev = torch.from_numpy(words).to(dev)
-        arena = torch.empty(max(plan.glb_need, 1), dtype=tdtype, device=dev)
+        arenas = [torch.empty(max(plan.glb_need, 1), dtype=tdtype, device=dev) for _ in range(n_streams)]
         prm = torch.tensor([float(x) for x in params], dtype=torch.float64, device=dev)
-        n = wplan.n_slices
         out = torch.zeros(2 * n, dtype=torch.float64, device=dev)
         seg = torch.tensor([0, n], dtype=torch.int32, device=dev)
         term = torch.zeros(2, dtype=torch.float64, device=dev)
         total = torch.zeros(2, dtype=torch.float64, device=dev)
-        st = stream if stream is not None else torch.cuda.current_stream(dev)
-        h = ctypes.c_void_p(st.cuda_stream)
-        d = _lib.WideDesc()
-        d.plan_host = words.ctypes.data
-        d.plan_words = words.size
-        d.plan_dev = plan_dev.data_ptr()
-        d.params = prm.data_ptr()
-        d.arena = arena.data_ptr()
-        d.arena_elems = arena.numel()
-        d.dtype = qb_dtype
+        main = stream if stream is not None else torch.cuda.current_stream(dev)
+        streams = [main] + [torch.cuda.Stream(device=dev) for _ in range(n_streams - 1)]
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        ev0.record(st)
+        ev0.record(main)
+        for st in streams[1:]:
+            st.wait_event(ev0)
+        descs = []
+        for k in range(n_streams):
+            d = _lib.WideDesc()
+            d.plan_host = words.ctypes.data
+            d.plan_words = words.size
+            d.plan_dev = plan_dev.data_ptr()
+            d.params = prm.data_ptr()
+            d.arena = arenas[k].data_ptr()
+            d.arena_elems = arenas[k].numel()
+            d.dtype = qb_dtype
+            descs.append(d)
         for s in range(n):
+            k = s % n_streams
+            d = descs[k]
             d.slice_bits = s
             d.out = out.data_ptr() + 16 * s
-            _lib.check(lib.qb_wide_execute(ctypes.byref(d), h))
+            _lib.check(lib.qb_wide_execute(ctypes.byref(d), ctypes.c_void_p(streams[k].cuda_stream)))
+        for st in streams[1:]:
+            ev = torch.cuda.Event()
+            ev.record(st)
+            main.wait_event(ev)
+        h = ctypes.c_void_p(main.cuda_stream)
         _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(seg.data_ptr()), 1,
                                        ctypes.c_void_p(term.data_ptr()), ctypes.c_void_p(total.data_ptr()), h))
-        ev1.record(st)
+        ev1.record(main)
         ev1.synchronize()
         r = total.cpu().numpy()
         val = complex(r[0], r[1])

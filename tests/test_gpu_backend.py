@@ -111,3 +111,29 @@ def test_batch_vars_through_plain_backend(gpu):
     amps = QtreeSimulator(bucket_backend=HipBackend()).simulate_batch(comp.circuit, batch_vars=2)
     ref = QtreeSimulator(bucket_backend=npf.NumpyBackend()).simulate_batch(comp.circuit, batch_vars=2)
     assert np.allclose(amps, ref, atol=1e-5)
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex64", 2e-5), ("complex128", 1e-12)])
+@pytest.mark.parametrize("seed", range(6))
+def test_step_large_rank_register_tiling(gpu, dtype, tol, seed):
+    """Rank ~20 steps with 1-4 inputs where the largest input lacks high output
+    bits (register-reuse path) or carries all of them."""
+    rng = np.random.default_rng(100 + seed)
+    n_vars = 21
+    vs = [Var(i) for i in range(n_vars)]
+    k = 1 + seed % 4
+    big_ids = sorted({0} | set(int(x) for x in rng.choice(np.arange(1, n_vars), size=14, replace=False)))
+    bucket = [Tensor("A", [vs[i] for i in big_ids],
+                     data=rng.normal(size=(2,) * len(big_ids)) + 1j * rng.normal(size=(2,) * len(big_ids)))]
+    rest = [i for i in range(1, n_vars) if i not in big_ids]
+    for j in range(1, k):
+        ids = sorted({0} | set(rest[(j - 1)::max(1, k - 1)]) | {int(rng.integers(1, n_vars))})
+        bucket.append(Tensor(f"T{j}", [vs[i] for i in ids],
+                             data=rng.normal(size=(2,) * len(ids)) + 1j * rng.normal(size=(2,) * len(ids))))
+    be = HipBackend(dtype=dtype)
+    be._ensure()
+    for no_sum in (False, True):
+        got = be.get_result_data(be.process_bucket(_to_dev(be, bucket), no_sum=no_sum))
+        ref = np.asarray(npf.process_bucket(bucket, no_sum=no_sum).data)
+        assert got.shape == ref.shape
+        assert np.max(np.abs(got - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
