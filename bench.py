@@ -102,6 +102,77 @@ def forced_edge(G, gamma, beta, executor_value, n_slice_vars=3, target_tw=30):
             "abs_err_vs_greedy_plan": abs(val - executor_value)}
 
 
+def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
+    """C4: Erdos-Renyi N=64, average degree 4 (seed 4), p=3, n_sets angle sets from
+    default_rng(4); every edge term in complex64 (FP64 gate construction, FP64
+    compensated slice sums) against the same plans in complex128; the complex128
+    path is itself checked against the numpy oracle on the edges with treewidth
+    <= oracle_tw."""
+    import networkx as nx
+    import numpy as np
+    from qtensor_amd.composer import DefaultQAOAComposer
+    from qtensor_amd.ordering import greedy_peo
+    from qtensor_amd.tensornet import QtreeTensorNet
+    from qtensor_amd.wide import WideEngine, compile_wide, slice_vars_for_order
+    G = nx.erdos_renyi_graph(64, 4 / 63, seed=4)
+    rng = np.random.default_rng(4)
+    sets = [(list(rng.uniform(0, np.pi, 3)), list(rng.uniform(0, np.pi / 2, 3))) for _ in range(n_sets)]
+    t0 = time.perf_counter()
+    plans, tws = [], []
+    for e in G.edges():
+        comp = DefaultQAOAComposer(G, gamma=[0.0] * 3, beta=[0.0] * 3)
+        comp.energy_expectation_lightcone(e)
+        graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+        order, degs = greedy_peo(graph)
+        s = max(0, max(degs) - max_tw_slice)
+        plans.append(compile_wide(comp.circuit, order=order,
+                                  par_ids=slice_vars_for_order(graph, order, s) if s else []))
+        tws.append(max(degs))
+    plan_s = time.perf_counter() - t0
+    e64, e128 = WideEngine(dtype="complex64"), WideEngine(dtype="complex128")
+    per_set, t64, t128 = [], 0.0, 0.0
+    oracle_err = 0.0
+    oracle_edges = [k for k, tw in enumerate(tws) if tw <= oracle_tw]
+    for si, (gamma, beta) in enumerate(sets):
+        z64, z128 = [], []
+        for wp in plans:
+            v, dt = e64.run(wp, gamma + beta, timing=True)
+            z64.append(v)
+            t64 += dt
+            v, dt = e128.run(wp, gamma + beta, timing=True)
+            z128.append(v)
+            t128 += dt
+        z64, z128 = np.array(z64), np.array(z128)
+        abs_err = np.abs(z64 - z128)
+        rel_err = abs_err / np.maximum(np.abs(z128), 1e-300)
+        C64 = (G.number_of_edges() - np.sum(z64.real)) / 2
+        C128 = (G.number_of_edges() - np.sum(z128.real)) / 2
+        per_set.append({"edge_abs_err_max": float(abs_err.max()), "edge_abs_err_mean": float(abs_err.mean()),
+                        "edge_rel_err_max": float(rel_err.max()), "energy64": float(C64), "energy128": float(C128),
+                        "energy_abs_err": float(abs(C64 - C128)), "energy_rel_err": float(abs(C64 - C128) / abs(C128))})
+        if si == 0:
+            from oracle import np_framework as npf
+            from qtensor_amd.simulator import QAOAQtreeSimulator
+            sim = QAOAQtreeSimulator(DefaultQAOAComposer, bucket_backend=npf.NumpyBackend())
+            edges = list(G.edges())
+            for k in oracle_edges:
+                ref = complex(sim._get_edge_energy(G, gamma, beta, edges[k])[0])
+                oracle_err = max(oracle_err, abs(z128[k] - ref))
+        print(f"[precision] set {si}: max |dZZ| {abs_err.max():.2e}, |dC| {abs(C64 - C128):.2e}", file=sys.stderr, flush=True)
+    worst = max(per_set, key=lambda r: r["edge_abs_err_max"])
+    return {"workload": "erdos_renyi_n64_d4_seed4_p3", "edges": G.number_of_edges(), "angle_sets": n_sets,
+            "composer": "DefaultQAOAComposer (ZZ + simplify)", "treewidth_max": max(tws),
+            "slice_to_treewidth": max_tw_slice, "plan_seconds": round(plan_s, 1),
+            "gpu_seconds_c64": round(t64, 2), "gpu_seconds_c128": round(t128, 2),
+            "step_model_GB_per_set": round(sum(wp.plan.bytes_step_model * wp.n_slices for wp in plans) / 1e9, 1),
+            "edge_abs_err_max": worst["edge_abs_err_max"],
+            "edge_rel_err_max": max(r["edge_rel_err_max"] for r in per_set),
+            "energy_abs_err_max": max(r["energy_abs_err"] for r in per_set),
+            "energy_rel_err_max": max(r["energy_rel_err"] for r in per_set),
+            "c128_vs_numpy_oracle_max_abs": oracle_err, "oracle_edges": len(oracle_edges),
+            "per_set": per_set}
+
+
 def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11):
     """Poisson stream of energy requests (C5) through the continuous-batching server."""
     import networkx as nx
@@ -145,6 +216,9 @@ def main():
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="fixed launch shape item_threads,lds_capacity,lds_tensor_limit (skips the autotuner)")
+    ap.add_argument("--precision", action="store_true",
+                    help="run only the precision study (Erdos-Renyi N=64, p=3) and print its JSON line")
+    ap.add_argument("--precision-sets", type=int, default=16)
     ap.add_argument("--serve", action="store_true",
                     help="run only the serving trace (Poisson arrivals) and print its JSON line")
     ap.add_argument("--serve-rates", default="20,80,320,1280")
@@ -162,6 +236,12 @@ def main():
     import torch
     from qtensor_amd.batched import LightconeEngine
 
+    if args.precision:
+        res = precision_study(n_sets=args.precision_sets)
+        print(json.dumps({"metric": "complex64 edge-term error vs complex128", "value": res["edge_abs_err_max"],
+                          "unit": "abs error", "higher_is_better": False, "n_gpus": 1, "dtype": "complex64",
+                          "data": "synthetic", "config": {"workload": res["workload"]}, "precision": res}), flush=True)
+        return
     if args.serve:
         rates = [float(x) for x in args.serve_rates.split(",")]
         res = serve_bench(rates, args.serve_requests, args.serve_pool)
