@@ -76,11 +76,13 @@ def plain_form(G, gamma, beta, n_edges):
     return vals, time.perf_counter() - t0
 
 
-def forced_edge(G, gamma, beta, executor_value, n_slice_vars=3, target_tw=30):
+def forced_edge(G, gamma, beta, executor_value=None, n_slice_vars=3, target_tw=30):
     """C2's extra edge: the first edge under a deliberately wide order (seeded
     randomized greedy, temperature 8, first sample of width 30), sliced over
     ``n_slice_vars`` variables, contracted with one grid-wide step kernel per
-    step (the plain per-bucket regime at large rank)."""
+    step (the plain per-bucket regime at large rank).  Checked against
+    ``executor_value`` (the batched executor's term for the same edge) or, if
+    None, against the greedy-ordered wide plan of the same edge."""
     from qtensor_amd.composer import QtreeQAOAComposer
     from qtensor_amd.tensornet import QtreeTensorNet
     from qtensor_amd.wide import WideEngine, compile_wide, forced_width_order, slice_vars_for_order
@@ -91,6 +93,8 @@ def forced_edge(G, gamma, beta, executor_value, n_slice_vars=3, target_tw=30):
     order = forced_width_order(graph, target_tw, temp=8.0, seed=0)
     wp = compile_wide(comp.circuit, order=order, par_ids=slice_vars_for_order(graph, order, n_slice_vars))
     eng = WideEngine()
+    if executor_value is None:
+        executor_value = eng.run(compile_wide(comp.circuit), list(gamma) + list(beta))
     eng.run(wp, list(gamma) + list(beta))                       # warm
     val, secs = eng.run(wp, list(gamma) + list(beta), timing=True)
     nbytes = wp.plan.bytes_step_model * wp.n_slices
@@ -224,6 +228,11 @@ def main():
     ap.add_argument("--serve-rates", default="20,80,320,1280")
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--serve-pool", type=int, default=8)
+    ap.add_argument("--forced-edge", action="store_true",
+                    help="run only the forced treewidth-30 edge (wide path) and print its JSON line")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="batched timing and executor kernel time only (for counter runs), then one "
+                         "1 GiB device copy that calibrates the memory counters")
     ap.add_argument("--slice-tw", type=int, default=None,
                     help="with --shape: slice lightcones whose treewidth reaches this value")
     args = ap.parse_args()
@@ -254,6 +263,13 @@ def main():
         return
 
     G, p, gamma, beta, desc = workload(args.workload)
+    if args.forced_edge:
+        fe = forced_edge(G, gamma, beta)
+        print(json.dumps({"metric": "forced treewidth-30 edge: step-model GB/s", "value": fe["achieved_GBs"],
+                          "unit": "GB/s", "n_gpus": 1, "higher_is_better": True, "dtype": "complex64",
+                          "data": "synthetic", "config": {"workload": desc + "_forced_tw30_edge"},
+                          "forced_edge": fe}), flush=True)
+        return
     n_terms = G.number_of_edges()
     print(f"[bench] workload {desc}: {n_terms} edge terms", file=sys.stderr, flush=True)
 
@@ -281,7 +297,8 @@ def main():
     wl_alg = LightconeEngine(shape=shape).workload(G, p)
     alg_bytes = wl_alg.step_model_bytes()
     batch = eng.batch([(wl, gamma, beta)])
-    batch.capture()
+    if not args.kernel_only:          # counter runs see plain launches, not graph nodes
+        batch.capture()
     for _ in range(args.warmup):
         batch.run()
     torch.cuda.synchronize()
@@ -308,6 +325,23 @@ def main():
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
     bytes_per_launch = alg_bytes
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    if args.kernel_only:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings give
+        # the scale of those counters (1 GiB read, 1 GiB written)
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"metric": "executor kernel time", "value": round(kern_ms, 5), "unit": "ms",
+                          "higher_is_better": False, "n_gpus": 1, "dtype": "complex64", "data": "synthetic",
+                          "config": {"workload": desc, "item_threads": shape.item_threads,
+                                     "lds_capacity": shape.lds_capacity,
+                                     "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw,
+                                     "work_items": batch.n_items, "resident_items": batch.n_slots},
+                          "step_model_bytes_per_launch": bytes_per_launch,
+                          "terms_per_s": round(value, 1), "calibration_copy_bytes": 2 << 30}), flush=True)
+        return
 
     # error against FP64 (complex128 run of the same plans)
     eng128 = LightconeEngine(dtype="complex128", shape=shape)

@@ -51,7 +51,9 @@ extern "C" {
 /* item[0] = offset of the item's plan in the program buffer (int32 words)
  * item[1] = offset of the item's angle vector in params (doubles)
  * item[2] = slice assignment: bit k = value of the plan's k-th sliced variable
- * item[3] = reserved (0) */
+ * item[3] = output slot: the item's value goes to item_out[2*slot .. 2*slot+1]
+ *           (slots are a permutation of 0..n_items-1; items may be listed in any
+ *           order, e.g. longest first) */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int qb_version(void);
@@ -102,7 +104,8 @@ typedef struct qb_exec_desc {
     int32_t item_threads;        /* 64, 128, 256, 512 or 1024 */
     int32_t lds_elems;           /* LDS elements reserved per item (>= every plan's need) */
     int32_t dtype;               /* QB_C64 or QB_C128 */
-    const double* params;        /* device: angle vectors (FP64) */
+    const double* params;        /* device: angle vectors (FP64); readable for 8 doubles
+                                    past every item's offset (pad the buffer) */
     const void* const_pool;      /* device: constant gate data, dtype elements */
     void* scratch;               /* device: n_slots * scratch_elems elements */
     int64_t scratch_elems;       /* elements per slot (>= every plan's global need) */

@@ -3,9 +3,9 @@
 For one (graph, p) workload it compiles the lightcone plans under each
 candidate (item threads, LDS capacity, LDS tensor limit, slicing treewidth)
 and times the executor kernel on the device with HIP events, keeping the
-fastest.  Candidates come from the workload's treewidth profile: slicing is
-only tried below the largest greedy treewidth, and the LDS budget per item
-is chosen so several items share a CU.
+fastest.  The search runs in two stages: every slicing threshold below the
+workload's largest greedy treewidth under a base shape, then the other
+shapes (item width, LDS budget per item) at the two best thresholds.
 
 Results are memoised per (graph signature, p, composer, dtype) in-process and
 can be saved to / loaded from a JSON file.
@@ -24,15 +24,24 @@ def _signature(G, p, composer, dtype):
     return (G.number_of_nodes(), G.number_of_edges(), hash(degs), p, composer.__name__, dtype)
 
 
-def candidates(max_tw):
-    """Launch shapes x slicing thresholds worth timing for a workload whose
-    greedy lightcone treewidth reaches ``max_tw``."""
-    shapes = [LaunchShape(64, 4096, 2048), LaunchShape(64, 2048, 1024), LaunchShape(128, 4096, 2048)]
+BASE_SHAPE = LaunchShape(128, 4096, 2048)
+OTHER_SHAPES = (LaunchShape(128, 2048, 1024), LaunchShape(256, 4096, 2048), LaunchShape(256, 2048, 1024),
+                LaunchShape(64, 4096, 2048))
+
+
+def slicing_candidates(max_tw):
+    """Slicing thresholds worth timing for a workload whose greedy lightcone
+    treewidth reaches ``max_tw`` (None = no slicing)."""
     slicing = [None]
     for t in (13, 12, 11, 10):
         if t < max_tw:
             slicing.append(t)
-    return [(s, m) for s in shapes for m in slicing]
+    return slicing
+
+
+def candidates(max_tw):
+    """Every launch shape x slicing threshold (the exhaustive search space)."""
+    return [(s, m) for s in (BASE_SHAPE,) + OTHER_SHAPES for m in slicing_candidates(max_tw)]
 
 
 class Autotuner:
@@ -72,23 +81,41 @@ class Autotuner:
         key = _signature(G, p, self.composer, self.dtype)
         if key in self.memo:
             return self.memo[key]
-        if cands is None:
+        table = []
+        staged = cands is None
+        if staged:
+            # stage 1: slicing thresholds under the base shape; stage 2: the
+            # other shapes at the two best thresholds
             probe = LightconeEngine(composer=self.composer, dtype=self.dtype)
             max_tw = probe.workload(G, p).max_treewidth()
-            cands = candidates(max_tw)
-        table = []
+            cands = [(BASE_SHAPE, m) for m in slicing_candidates(max_tw)]
         best = None
-        for shape, mtw in cands:
+        timed = []
+
+        def run(shape, mtw):
+            nonlocal best
             try:
                 ms, plan_s, _, _ = self.time_config(G, p, gamma, beta, shape, mtw)
             except ValueError as exc:      # shape cannot hold these plans
                 table.append({"shape": asdict(shape), "max_tw": mtw, "error": str(exc)})
-                continue
+                return
             table.append({"shape": asdict(shape), "max_tw": mtw, "exec_ms": ms, "plan_s": plan_s})
+            timed.append((ms, mtw))
             if self.verbose:
                 print(f"[autotune] {shape.key()} max_tw={mtw}: {ms:.4f} ms", flush=True)
             if best is None or ms < best[0]:
                 best = (ms, shape, mtw)
+
+        for shape, mtw in cands:
+            run(shape, mtw)
+        if staged:
+            top = []
+            for _, m in sorted(timed, key=lambda x: x[0]):
+                if m not in top:
+                    top.append(m)
+            for shape in OTHER_SHAPES:
+                for m in top[:2]:
+                    run(shape, m)
         if best is None:
             raise RuntimeError("no executor configuration fits this workload")
         res = (best[1], best[2], table)

@@ -24,7 +24,7 @@ from . import _lib
 from .composer import QtreeQAOAComposer
 from .lightcone import canonical_graph, canonical_lightcone
 from .ordering import OrderingOptimizer, SlicesOptimizer
-from .plan import PROGRAM_PAD, PlanCompiler
+from .plan import PARAM_PAD, PROGRAM_PAD, PlanCompiler
 
 
 def _torch():
@@ -141,7 +141,7 @@ class LightconeEngine:
                              "its gate tensors alone do not fit the launch shape")
         if plan.lds_elems(16 if self.dtype == "complex128" else 8) * (16 if self.dtype == "complex128" else 8) \
                 > 160 * 1024:
-            raise ValueError("plan data plus staged step records exceed 160 KiB of LDS")
+            raise ValueError("plan data exceed 160 KiB of LDS")
         with self._lock:
             self._plans[ck] = plan
         return plan
@@ -175,6 +175,15 @@ class LightconeEngine:
         return C, zz
 
 
+def order_items(items, costs):
+    """Work items in launch order: longest first (by ``costs``, stable), each
+    carrying its output slot (its index in the given order) in word 3."""
+    items = np.array(items, dtype=np.int32).reshape(-1, 4)
+    items[:, 3] = np.arange(items.shape[0], dtype=np.int32)
+    order = np.argsort(-np.asarray(costs, dtype=np.int64), kind="stable")
+    return np.ascontiguousarray(items[order])
+
+
 class Batch:
     """Device buffers and launch of one set of (workload, angles) requests."""
 
@@ -202,7 +211,7 @@ class Batch:
                     plan_off[ident] = pos
                     words.append(plan.words)
                     pos += plan.words.size
-        words.append(np.zeros(PROGRAM_PAD, dtype=np.int32))   # record prefetch reads ahead
+        words.append(np.zeros(PROGRAM_PAD, dtype=np.int32))   # record fetches read past the last op
         program = np.concatenate(words).astype(np.int32)
         # params, items, segments (one segment per edge term)
         params, items, seg = [], [], [0]
@@ -225,7 +234,7 @@ class Batch:
                 self.flops += plan.flops * w.n_slices
                 for s in range(w.n_slices):
                     items.append((plan_off[id(plan)], poff, s, 0))
-                    order.append(plan.bytes_step_model)
+                    order.append(plan.n_ops)
                 seg.append(len(items))
             self.term_slices.append((start, len(seg) - 1))
         self.n_items = len(items)
@@ -236,10 +245,12 @@ class Batch:
         _lib.check(self.lib.qb_exec_slots(self.shape.item_threads, self.lds_elems, self.qb_dtype,
                                           ctypes.byref(slots)))
         self.n_slots = slots.value
-        items_np = np.asarray(items, dtype=np.int32).reshape(-1, _lib.QB_ITEM_WORDS)
+        items_np = order_items(items, order)
         self.program = torch.from_numpy(program).to(dev)
         self.items = torch.from_numpy(np.ascontiguousarray(items_np)).to(dev)
-        self.params = torch.tensor(params, dtype=torch.float64, device=dev)
+        # the executor reads 8 angles from every item's offset in one load
+        self.params = torch.tensor(params + [0.0] * PARAM_PAD, dtype=torch.float64, device=dev)
+        self.n_params = len(params)
         self.seg = torch.tensor(seg, dtype=torch.int32, device=dev)
         self.item_out = torch.zeros(2 * self.n_items, dtype=torch.float64, device=dev)
         self.term_out = torch.zeros(2 * self.n_terms, dtype=torch.float64, device=dev)
@@ -275,9 +286,9 @@ class Batch:
         flat = []
         for gamma, beta in requests_params:
             flat += [float(x) for x in gamma] + [float(x) for x in beta]
-        if len(flat) != self.params.numel():
+        if len(flat) != self.n_params:
             raise ValueError("parameter count does not match the batch")
-        self.params.copy_(_torch().tensor(flat, dtype=_torch().float64))
+        self.params[:self.n_params].copy_(_torch().tensor(flat, dtype=_torch().float64))
 
     def launch_exec(self, stream=None):
         """Launch only the executor (queue reset + persistent kernel)."""

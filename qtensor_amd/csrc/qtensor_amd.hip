@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
 //                     lds_need, glb_need (low word), n_sliced, ops_words,
 //                     glb_need (high word), 0, 0
 //   gates: n_gates * kGateWords (distinct gate tensors only; equal leaves share one)
-//   ops:   one record per op; the item stages them in LDS after its data
+//   ops:   one record per op, read from global memory one op ahead of use
 // LDS element 0 holds 1+0i, element 1 holds 0+0i, element 2 is a write sink;
 // gate tensors start at element 4.
 //
@@ -312,9 +312,10 @@ __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
 //     runs = n_runs << 30 | run0 | run1 << 15, run = src | width << 5 | dst << 10;
 //     n_runs == 3 means "more than two runs": the offset is pext(o, mask).
 // Gather op (several small steps, one output element per lane, all in LDS):
-//   w0 = kOpGather | n_steps | len << 16, w1 = active lanes, w2 = descriptor offset
+//   w0 = kOpGather | n_steps | len << 16, w1 = active lanes, w2 = descriptor offset,
+//   w3 = largest input count of its steps
 //   w4...: lane -> step table, one byte per lane
-//   per step, 12 words at w2 + 12 s: out_loc, first_lane | n_in << 8 | sum << 12,
+//   per step, 12 words at w2 + 12 s: out_loc, first_lane | sum << 12 | n_in << 16,
 //     then per input (loc | top << 16, runs); unused inputs read element 0 (1+0i)
 constexpr int kPlanHdr = 12;
 constexpr int kGateWords = 16;
@@ -389,9 +390,22 @@ __device__ void natural_gate(int32_t kind, double alpha, double* re, double* im)
     }
 }
 
+// Angles of a work item: the executor loads an item's first kParamRegs angles
+// with one uniform (scalar) load, so the lanes of the grid do not all read one
+// cache line, and each wave keeps a copy in LDS (the params buffer is padded so
+// the load stays inside it).  Higher indices are read from global memory.
+constexpr int kParamRegs = 8;
+
+__device__ __forceinline__ double angle_at(const QB_LDS double* lds_angles, int32_t n_lds, const double* params,
+                                           int32_t j) {
+    if (j < n_lds) return lds_angles[j];
+    return params[j];
+}
+
 template <typename R, typename Dst>
-__device__ void build_gate(const int32_t* g, const double* params, uint32_t slice_bits, Dst lds,
-                           const typename Cx<R>::V* pool) {
+__device__ __forceinline__ void build_gate(const int32_t* g, const double* params,
+                                           const QB_LDS double* lds_angles, int32_t n_lds, uint32_t slice_bits,
+                                           Dst lds, const typename Cx<R>::V* pool) {
     using V = typename Cx<R>::V;
     const int32_t kind = g[0];
     const int32_t dst = g[1];
@@ -407,8 +421,8 @@ __device__ void build_gate(const int32_t* g, const double* params, uint32_t slic
         return;
     }
     double alpha = __hiloint2double(g[9], g[8]);
-    if (g[4] >= 0) alpha += (double)__int_as_float(g[5]) * params[g[4]];
-    if (g[6] >= 0) alpha += (double)__int_as_float(g[7]) * params[g[6]];
+    if (g[4] >= 0) alpha += (double)__int_as_float(g[5]) * angle_at(lds_angles, n_lds, params, g[4]);
+    if (g[6] >= 0) alpha += (double)__int_as_float(g[7]) * angle_at(lds_angles, n_lds, params, g[6]);
     double re[8], im[8];
     natural_gate(kind, alpha, re, im);
     for (int e = 0; e < n; ++e) {
@@ -462,13 +476,17 @@ __device__ __forceinline__ V load_loc(uint32_t loc, uint32_t off, const QB_LDS V
     return lds[loc + off];
 }
 
+// Offset of union element o in an input whose mask has at most two runs: one
+// bit-field extract and one shift per run (an absent run has width 0 and adds 0).
+__device__ __forceinline__ uint32_t run_offset(uint32_t o, uint32_t runs) {
+    const uint32_t a = bfe(o, runs & 31u, (runs >> 5) & 31u) << ((runs >> 10) & 31u);
+    const uint32_t b = bfe(o, (runs >> 15) & 31u, (runs >> 20) & 31u) << ((runs >> 25) & 31u);
+    return a | b;
+}
+
 __device__ __forceinline__ uint32_t in_offset(uint32_t o, uint32_t mask, uint32_t runs) {
-    const uint32_t n = runs >> 30;
-    if (n == 3u) return pext_loop(o, mask);
-    uint32_t off = 0;
-    if (n > 0) off = bfe(o, runs & 31u, (runs >> 5) & 31u) << ((runs >> 10) & 31u);
-    if (n > 1) off |= bfe(o, (runs >> 15) & 31u, (runs >> 20) & 31u) << ((runs >> 25) & 31u);
-    return off;
+    if ((runs >> 30) == 3u) return pext_loop(o, mask);
+    return run_offset(o, runs);
 }
 
 #define QB_RL(r, k) ((uint32_t)__builtin_amdgcn_readlane((r), (k)))
@@ -476,91 +494,155 @@ __device__ __forceinline__ uint32_t in_offset(uint32_t o, uint32_t mask, uint32_
 template <typename V>
 __device__ __forceinline__ V gather_in(const QB_LDS V* lds, uint32_t locw, uint32_t runs, uint32_t o,
                                        uint32_t sum, V* y) {
-    const uint32_t a = (locw & 0xffffu) + in_offset(o, 0u, runs);
+    const uint32_t a = (locw & 0xffffu) + run_offset(o, runs);
     *y = lds[a + (sum ? (locw >> 16) : 0u)];
     return lds[a];
 }
 
-// One gather op: lane l computes element o = l - first of step table[l].
-template <typename R>
-__device__ __forceinline__ void gather_op(const QB_LDS int32_t* op, QB_LDS typename Cx<R>::V* lds,
-                                          int lane) {
+// Op records are read from global memory (they stay cached) ahead of use.
+// Lane l fetches word (l & 63) of the record (the program buffer is padded so
+// the read stays inside it for any op).
+__device__ __forceinline__ int32_t fetch_rec(const QB_GLB int32_t* st, int lane) {
+    return st[lane & 63];
+}
+
+// The lane's step descriptor of a gather op (lanes past the active ones read
+// step 0).  Byte l of the lane -> step table is record word 4 + l/4: for items of
+// up to 236 lanes that word is already in lane 4 + l/4 of the wave (its record
+// word), fetched by a cross-lane permute; larger items load it.
+template <int ITEM_THREADS>
+__device__ __forceinline__ void fetch_desc(const QB_GLB int32_t* st, int32_t r, int lane,
+                                           i32x4& d0, i32x4& d1, i32x4& d2) {
     static_assert(sizeof(i32x4) == 16, "descriptor rows are 16 bytes");
+    uint32_t tw;
+    if constexpr (ITEM_THREADS <= 4 * (64 - kGatherHdr))
+        tw = (uint32_t)__builtin_amdgcn_ds_bpermute((kGatherHdr + (lane >> 2)) << 2, r);
+    else
+        tw = (uint32_t)st[kGatherHdr + (lane >> 2)];
+    const uint32_t n_act = QB_RL(r, 1);
+    const uint32_t sidx = (uint32_t)lane < n_act ? (tw >> (8 * (lane & 3))) & 255u : 0u;
+    const QB_GLB i32x4* d = (const QB_GLB i32x4*)(st + QB_RL(r, 2) + kGatherDesc * sidx);
+    const uint32_t ni = QB_RL(r, 3);
+    d0 = d[0];
+    if (ni > 1) d1 = d[1];
+    if (ni > 3) d2 = d[2];
+}
+
+// One gather op: lane l computes element o = l - first of its step.  NI is
+// the op's largest input count (record word 3); steps with fewer inputs name
+// element 0 (1+0i) in the unused slots.
+template <typename R, int NI>
+__device__ __forceinline__ void gather_op_n(const i32x4& d0, const i32x4& d1, const i32x4& d2,
+                                            QB_LDS typename Cx<R>::V* lds, int lane) {
     using V = typename Cx<R>::V;
-    const uint32_t n_act = (uint32_t)op[1];
-    if ((uint32_t)lane >= n_act) return;
-    const uint32_t sidx = ((const QB_LDS uint8_t*)(op + kGatherHdr))[lane];
-    const QB_LDS i32x4* d = (const QB_LDS i32x4*)(op + (uint32_t)op[2] + kGatherDesc * sidx);
-    const i32x4 d0 = d[0], d1 = d[1], d2 = d[2];
     const uint32_t meta = (uint32_t)d0.y;
-    const uint32_t o = (uint32_t)lane - (meta & 255u);
+    const uint32_t o = (uint32_t)lane - (meta & 0xfffu);
     const uint32_t sum = (meta >> 12) & 1u;
-    V y0, y1, y2, y3;
-    const V x0 = gather_in(lds, (uint32_t)d0.z, (uint32_t)d0.w, o, sum, &y0);
-    const V x1 = gather_in(lds, (uint32_t)d1.x, (uint32_t)d1.y, o, sum, &y1);
-    const V x2 = gather_in(lds, (uint32_t)d1.z, (uint32_t)d1.w, o, sum, &y2);
-    const V x3 = gather_in(lds, (uint32_t)d2.x, (uint32_t)d2.y, o, sum, &y3);
-    V p = cmul(cmul(x0, x1), cmul(x2, x3));
-    if (sum) p = cadd(p, cmul(cmul(y0, y1), cmul(y2, y3)));
+    V x[4], y[4];
+    x[0] = gather_in(lds, (uint32_t)d0.z, (uint32_t)d0.w, o, sum, &y[0]);
+    if (NI > 1) x[1] = gather_in(lds, (uint32_t)d1.x, (uint32_t)d1.y, o, sum, &y[1]);
+    if (NI > 2) x[2] = gather_in(lds, (uint32_t)d1.z, (uint32_t)d1.w, o, sum, &y[2]);
+    if (NI > 3) x[3] = gather_in(lds, (uint32_t)d2.x, (uint32_t)d2.y, o, sum, &y[3]);
+    V p, q;
+    if (NI == 1) {
+        p = x[0];
+        q = y[0];
+    } else if (NI == 2) {
+        p = cmul(x[0], x[1]);
+        q = cmul(y[0], y[1]);
+    } else if (NI == 3) {
+        p = cmul(cmul(x[0], x[1]), x[2]);
+        q = cmul(cmul(y[0], y[1]), y[2]);
+    } else {
+        p = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
+        q = cmul(cmul(y[0], y[1]), cmul(y[2], y[3]));
+    }
+    if (sum) p = cadd(p, q);
     lds[(uint32_t)d0.x + o] = p;
+}
+
+template <typename R>
+__device__ __forceinline__ void gather_op(int32_t r, const i32x4& d0, const i32x4& d1, const i32x4& d2,
+                                          QB_LDS typename Cx<R>::V* lds, int lane) {
+    if ((uint32_t)lane >= QB_RL(r, 1)) return;
+    switch (QB_RL(r, 3)) {
+    case 1: gather_op_n<R, 1>(d0, d1, d2, lds, lane); break;
+    case 2: gather_op_n<R, 2>(d0, d1, d2, lds, lane); break;
+    case 3: gather_op_n<R, 3>(d0, d1, d2, lds, lane); break;
+    default: gather_op_n<R, 4>(d0, d1, d2, lds, lane); break;
+    }
+}
+
+// Output blocks [base, base + U*ITEM_THREADS) of an all-LDS step, for every
+// base below n_out in steps of U*ITEM_THREADS (n_out is a multiple of that
+// when U > 1).
+template <typename R, int ITEM_THREADS, int U>
+__device__ __forceinline__ void step_lds_pass(uint32_t n_out, uint32_t n_in, uint32_t n_sum,
+                                              const uint32_t* loc, const uint32_t* top, const uint32_t* mask,
+                                              const uint32_t* runs, uint32_t out_loc,
+                                              QB_LDS typename Cx<R>::V* lds, int lane) {
+    using V = typename Cx<R>::V;
+    for (uint32_t base = 0; base < n_out; base += U * ITEM_THREADS) {
+        V p[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t o = base + u * ITEM_THREADS + (uint32_t)lane;
+            uint32_t ad[kExecMaxIn];
+#pragma unroll
+            for (int j = 0; j < kExecMaxIn; ++j)
+                ad[j] = (uint32_t)j < n_in ? loc[j] + in_offset(o, mask[j], runs[j]) : 0u;
+            V x[kExecMaxIn], y[kExecMaxIn];
+#pragma unroll
+            for (int j = 0; j < kExecMaxIn; ++j) x[j] = lds[ad[j]];
+            if (n_sum == 2) {
+#pragma unroll
+                for (int j = 0; j < kExecMaxIn; ++j) y[j] = lds[ad[j] + top[j]];
+            }
+            p[u] = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
+            if (n_sum == 2) p[u] = cadd(p[u], cmul(cmul(y[0], y[1]), cmul(y[2], y[3])));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t o = base + u * ITEM_THREADS + (uint32_t)lane;
+            if (U > 1 || o < n_out) lds[out_loc + o] = p[u];
+        }
+    }
 }
 
 // One step op: all lanes of the item cover the step's output elements.  Lane l
 // of each wave holds word l of the record; fields reach scalar registers by
 // readlane.
 template <typename R, int ITEM_THREADS>
-__device__ __forceinline__ void step_op(const QB_LDS int32_t* st, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
+__device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
                                         QB_GLB typename Cx<R>::V* glb, int lane) {
     using V = typename Cx<R>::V;
-    const int32_t r_cur = st[lane & 63];
-    const uint32_t out_loc = QB_RL(r_cur, 2);
+    const uint32_t out_loc = QB_RL(r, 2);
     const uint32_t n_in = w0 & 15u;
     const uint32_t rank_out = (w0 >> 4) & 31u;
     const uint32_t n_sum = ((w0 >> 9) & 1u) + 1u;
-    // per input: location, summed-variable offset, mask, and this lane's offset
-    // for element o = lane; element o = base + lane adds pext(base, mask), which
-    // is uniform across the item (base is a multiple of the item size)
-    uint32_t loc[kExecMaxIn], top[kExecMaxIn], mask[kExecMaxIn], off0[kExecMaxIn];
+    uint32_t loc[kExecMaxIn], top[kExecMaxIn], mask[kExecMaxIn], runs[kExecMaxIn];
 #pragma unroll
     for (int j = 0; j < kExecMaxIn; ++j) {
         if ((uint32_t)j < n_in) {
-            loc[j] = QB_RL(r_cur, 3 + 4 * j);
-            top[j] = QB_RL(r_cur, 4 + 4 * j);
-            mask[j] = QB_RL(r_cur, 5 + 4 * j);
-            off0[j] = in_offset((uint32_t)lane, mask[j], QB_RL(r_cur, 6 + 4 * j));
+            loc[j] = QB_RL(r, 3 + 4 * j);
+            top[j] = QB_RL(r, 4 + 4 * j);
+            mask[j] = QB_RL(r, 5 + 4 * j);
+            runs[j] = QB_RL(r, 6 + 4 * j);
+        } else {
+            loc[j] = 0u; top[j] = 0u; mask[j] = 0u; runs[j] = 0u;
         }
     }
     const uint32_t n_out = 1u << rank_out;
     if (w0 & (1u << 12)) {
-        // all-LDS: every read of the step is issued before the first use;
-        // unused input slots read element 0 and are dropped by selects
-        uint32_t a0[kExecMaxIn];
-#pragma unroll
-        for (int j = 0; j < kExecMaxIn; ++j) a0[j] = (uint32_t)j < n_in ? loc[j] + off0[j] : 0u;
-        for (uint32_t base = 0; base < n_out; base += ITEM_THREADS) {
-            uint32_t ad[kExecMaxIn];
-#pragma unroll
-            for (int j = 0; j < kExecMaxIn; ++j)
-                ad[j] = a0[j] + ((base && (uint32_t)j < n_in) ? pext_loop(base, mask[j]) : 0u);
-            V x[kExecMaxIn], y[kExecMaxIn];
-#pragma unroll
-            for (int j = 0; j < kExecMaxIn; ++j) x[j] = lds[ad[j]];
-            if (n_sum == 2) {
-#pragma unroll
-                for (int j = 0; j < kExecMaxIn; ++j) y[j] = lds[ad[j] + ((uint32_t)j < n_in ? top[j] : 0u)];
-            }
-            V p = x[0];
-#pragma unroll
-            for (int j = 1; j < kExecMaxIn; ++j) p = (uint32_t)j < n_in ? cmul(p, x[j]) : p;
-            if (n_sum == 2) {
-                V q = y[0];
-#pragma unroll
-                for (int j = 1; j < kExecMaxIn; ++j) q = (uint32_t)j < n_in ? cmul(q, y[j]) : q;
-                p = cadd(p, q);
-            }
-            const uint32_t o = base + (uint32_t)lane;
-            if (o < n_out) lds[out_loc + o] = p;
-        }
+        // all-LDS: up to four blocks of ITEM_THREADS outputs per pass, every read
+        // of a pass issued before its first product and its stores (outputs
+        // never alias inputs); unused input slots read element 0 (1+0i)
+        if (n_out >= 4u * ITEM_THREADS)
+            step_lds_pass<R, ITEM_THREADS, 4>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, lane);
+        else if (n_out >= 2u * ITEM_THREADS)
+            step_lds_pass<R, ITEM_THREADS, 2>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, lane);
+        else
+            step_lds_pass<R, ITEM_THREADS, 1>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, lane);
         return;
     }
     for (uint32_t base = 0; base < n_out; base += ITEM_THREADS) {
@@ -572,8 +654,7 @@ __device__ __forceinline__ void step_op(const QB_LDS int32_t* st, uint32_t w0, Q
 #pragma unroll
                 for (int j = 0; j < kExecMaxIn; ++j) {
                     if ((uint32_t)j < n_in) {
-                        uint32_t off = off0[j] + (sv ? top[j] : 0u);
-                        if (base) off += pext_loop(base, mask[j]);
+                        const uint32_t off = in_offset(o, mask[j], runs[j]) + (sv ? top[j] : 0u);
                         const V v = load_loc(loc[j], off, lds, glb);
                         prod = j == 0 ? v : cmul(prod, v);
                     }
@@ -588,28 +669,67 @@ __device__ __forceinline__ void step_op(const QB_LDS int32_t* st, uint32_t w0, Q
     }
 }
 
-// All ops of one plan, from the copy staged in LDS.
-template <typename R, int ITEM_THREADS>
-__device__ void run_ops(const QB_LDS int32_t* st, int32_t n_ops, QB_LDS typename Cx<R>::V* lds,
-                        QB_GLB typename Cx<R>::V* glb, int lane) {
-    for (int32_t k = 0; k < n_ops; ++k) {
-        const uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(st[0]);
-        if constexpr (ITEM_THREADS == 64) {
-            // one wave: LDS operations complete in issue order, so the previous op's
-            // LDS results are visible once the compiler keeps memory operations on
-            // their side of this point; global data needs a real fence
-            if (w0 & (1u << 11)) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            if (w0 & (1u << 11)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        } else {
+// Synchronisation before an op.  Only LDS data flows between ops unless the op
+// is marked (bit 11) as touching global scratch written since the last marked op.
+template <int ITEM_THREADS>
+__device__ __forceinline__ void op_sync(uint32_t w0) {
+    const bool glb = (w0 & (1u << 11)) != 0;
+    if constexpr (ITEM_THREADS == 64) {
+        // one wave: LDS operations complete in issue order, so the previous op's
+        // LDS results are visible once the compiler keeps memory operations on
+        // their side of this point; global data needs a real fence
+        if (glb) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (glb) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        if (glb) {
             __syncthreads();
+        } else {
+            // LDS-only fence: record prefetches in flight are not waited for
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         }
+    }
+}
+
+// All ops of one plan.  Software pipeline over the records: while op k runs,
+// op k+1's gather descriptors and op k+2's record word are in flight.  r0 is op
+// 0's record word, fetched by the caller.
+template <typename R, int ITEM_THREADS>
+__device__ void run_ops(const QB_GLB int32_t* st0, int32_t n_ops, int32_t r0,
+                        QB_LDS typename Cx<R>::V* lds, QB_GLB typename Cx<R>::V* glb, int lane) {
+    if (n_ops <= 0) return;
+    uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(r0);
+    i32x4 d0, d1, d2;       // loaded only for gather ops, read only by them
+    if (w0 & kOpGather) fetch_desc<ITEM_THREADS>(st0, r0, lane, d0, d1, d2);
+    const QB_GLB int32_t* st1 = st0 + (w0 >> 16);
+    int32_t r1 = 0;
+    if (n_ops > 1) r1 = fetch_rec(st1, lane);
+    for (int32_t k = 0; k < n_ops; ++k) {
+        uint32_t w1 = 0;
+        i32x4 e0, e1, e2;       // loaded only for gather ops, read only by them
+        const QB_GLB int32_t* st2 = st1;
+        int32_t r2 = 0;
+        if (k + 1 < n_ops) {
+            w1 = (uint32_t)__builtin_amdgcn_readfirstlane(r1);
+            if (w1 & kOpGather) fetch_desc<ITEM_THREADS>(st1, r1, lane, e0, e1, e2);
+            st2 = st1 + (w1 >> 16);
+            if (k + 2 < n_ops) r2 = fetch_rec(st2, lane);
+        }
+        op_sync<ITEM_THREADS>(w0);
         if (w0 & kOpGather)
-            gather_op<R>(st, lds, lane);
+            gather_op<R>(r0, d0, d1, d2, lds, lane);
         else
-            step_op<R, ITEM_THREADS>(st, w0, lds, glb, lane);
-        st += w0 >> 16;
+            step_op<R, ITEM_THREADS>(r0, w0, lds, glb, lane);
+        w0 = w1;
+        r0 = r1;
+        d0 = e0;
+        d1 = e1;
+        d2 = e2;
+        st1 = st2;
+        r1 = r2;
     }
 }
 
@@ -622,44 +742,61 @@ __global__ __launch_bounds__(ITEM_THREADS) void exec_kernel(
     using V = typename Cx<R>::V;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int32_t item_sh;
+    __shared__ double angles_sh[(ITEM_THREADS + 63) / 64][kParamRegs];
     const int lane = threadIdx.x;
     QB_LDS V* lds = (QB_LDS V*)(smem_raw);
     QB_GLB V* glb = (QB_GLB V*)(scratch) + (int64_t)blockIdx.x * scratch_elems;
     const V* pool = reinterpret_cast<const V*>(pool_v);
     (void)lds_elems;
-    for (;;) {
-        if (lane == 0) item_sh = (int32_t)atomicAdd(queue, 1u);
-        group_sync<ITEM_THREADS>();
-        const int32_t item = __builtin_amdgcn_readfirstlane(item_sh);
-        if (item >= n_items) break;
+    // the first item of every block is static (item = block index), so the
+    // launch does not start with a burst of atomics on one address; later
+    // items come from the queue.  A ticket is taken only when an item is done
+    // (items finish at different times, so the atomics do not pile up), and
+    // not at all when every item had a block of its own.  (Vector memory
+    // operations complete in order, so an atomic issued early would hold up
+    // every later load of the item until it returns.)
+    int32_t item = (int32_t)blockIdx.x;
+    while (item < n_items) {
         const int32_t* __restrict__ it = items + (size_t)item * QB_ITEM_WORDS;
         const int32_t* __restrict__ plan = program + it[0];
         const double* __restrict__ prm = params + it[1];
         const uint32_t slice_bits = (uint32_t)it[2];
+        const int32_t slot = it[3];
         const int32_t n_gates = plan[0];
         const int32_t* __restrict__ gates = plan + plan[2];
-        // stage the op records in LDS, after the plan's data (plus 64 words of
-        // padding read by the lane-distributed record loads); constants 1, 0
-        const int32_t rec_words = plan[8];
-        QB_LDS int32_t* recs = (QB_LDS int32_t*)(lds + plan[5]);
-        const int32_t* __restrict__ src = plan + plan[3];
-        for (int32_t w = lane; w < rec_words + 64; w += ITEM_THREADS) recs[w] = w < rec_words ? src[w] : 0;
+        // op 0's record is in flight while the gates are built
+        const QB_GLB int32_t* st0 = (const QB_GLB int32_t*)(plan + plan[3]);
+        const int32_t r0 = fetch_rec(st0, lane);
         if (lane == 0) {
             lds[0] = V{1, 0};
             lds[1] = V{0, 0};
         }
+        double th[kParamRegs];
+#pragma unroll
+        for (int q = 0; q < kParamRegs; ++q) th[q] = prm[q];
+        QB_LDS double* my_angles = (QB_LDS double*)&angles_sh[lane >> 6][0];
+        if ((lane & 63) == 0) {
+#pragma unroll
+            for (int q = 0; q < kParamRegs; ++q) my_angles[q] = th[q];
+        }
+        // the wave's own LDS writes are visible to its later LDS reads
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
-            build_gate<R>(gates + (size_t)g * kGateWords, prm, slice_bits, lds, pool);
+            build_gate<R>(gates + (size_t)g * kGateWords, prm, my_angles, kParamRegs, slice_bits, lds, pool);
         group_sync<ITEM_THREADS>();
-        run_ops<R, ITEM_THREADS>(recs, plan[1], lds, glb, lane);
+        run_ops<R, ITEM_THREADS>(st0, plan[1], r0, lds, glb, lane);
         group_sync<ITEM_THREADS>();
         if (lane == 0) {
             const V r = load_loc((uint32_t)plan[4], 0u, lds, glb);
-            item_out[2 * (size_t)item] = (double)r.x;
-            item_out[2 * (size_t)item + 1] = (double)r.y;
+            item_out[2 * (size_t)slot] = (double)r.x;
+            item_out[2 * (size_t)slot + 1] = (double)r.y;
         }
-        // item_sh is rewritten only after every lane has passed this point
+        if ((int64_t)gridDim.x >= (int64_t)n_items) break;
+        if (lane == 0) item_sh = (int32_t)(gridDim.x + atomicAdd(queue, 1u));
         group_sync<ITEM_THREADS>();
+        item = __builtin_amdgcn_readfirstlane(item_sh);
+        // item_sh is rewritten only after the next item's barriers
     }
 }
 
@@ -797,8 +934,8 @@ __global__ void build_gates_global(const int32_t* __restrict__ gates, int32_t n_
     using V = typename Cx<R>::V;
     const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n_gates)
-        build_gate<R>(gates + (size_t)g * kGateWords, params, slice_bits, reinterpret_cast<V*>(arena),
-                      (const V*)nullptr);
+        build_gate<R>(gates + (size_t)g * kGateWords, params, (const QB_LDS double*)nullptr, 0, slice_bits,
+                      reinterpret_cast<V*>(arena), (const V*)nullptr);
 }
 
 template <typename R>

@@ -48,7 +48,11 @@ GATE_WORDS = 16
 IN_WORDS = 4
 STEP_HDR = 3
 MAX_IN = 4
-PROGRAM_PAD = 64
+# zero words after the last plan of a program: the executor fetches each op's
+# record word (lane & 63) and lane -> step byte (up to 4 + 1024/4 words) speculatively
+PROGRAM_PAD = 320
+# zero angles after the last item's: the executor loads 8 angles per item at once
+PARAM_PAD = 8
 MAX_RUNS = 2
 GLOBAL = 0x80000000
 OP_GATHER = 1 << 15
@@ -166,10 +170,8 @@ class Plan:
     dag: list = field(default_factory=list)   # per step: (producer steps of inputs, n_out, n_in, sum)
 
     def lds_elems(self, elem_bytes=8):
-        """LDS elements one item needs: data, then the staged op records
-        (plus 64 words of padding)."""
-        return self.lds_need + -(-(self.steps_words + PROGRAM_PAD) * 4 // elem_bytes)
-
+        """LDS elements one item needs (its data; op records stay in global memory)."""
+        return self.lds_need
 
 class _Arena:
     """First-fit allocator over element offsets."""
@@ -214,6 +216,58 @@ class _Arena:
         self.free = merged
 
 
+def _place_order(sizes, starts, ends, order, base):
+    """Offsets for buffers placed in ``order``, each at the lowest offset free
+    over its whole op interval; returns (offsets, peak)."""
+    m = len(sizes)
+    p_off = np.zeros(m, dtype=np.int64)
+    p_end = np.zeros(m, dtype=np.int64)
+    p_s = np.zeros(m, dtype=np.int64)
+    p_e = np.zeros(m, dtype=np.int64)
+    offsets = np.zeros(m, dtype=np.int64)
+    peak = base
+    for k, idx in enumerate(order):
+        n, s, e = sizes[idx], starts[idx], ends[idx]
+        off = base
+        if k:
+            live = (p_s[:k] <= e) & (s <= p_e[:k])
+            offs, tops = p_off[:k][live], p_end[:k][live]
+            srt = np.argsort(offs, kind="stable")
+            for a, z in zip(offs[srt].tolist(), tops[srt].tolist()):
+                if off + n <= a:
+                    break
+                if z > off:
+                    off = z
+        offsets[idx] = off
+        p_off[k], p_end[k], p_s[k], p_e[k] = off, off + n, s, e
+        peak = max(peak, int(off + n))
+    return offsets, peak
+
+
+def _place_intervals(items, base, align):
+    """Offline interval allocation: ``items`` are (buf, first_op, last_op);
+    buffers whose op intervals overlap get disjoint element ranges.  Three
+    placement orders are tried (largest first; largest size x lifetime first;
+    earliest first), the one with the lowest peak is kept.  Sets ``buf.loc``;
+    returns the peak offset."""
+    if not items:
+        return base
+    sizes = np.array([(b.n + align - 1) // align * align for b, _, _ in items], dtype=np.int64)
+    starts = np.array([s for _, s, _ in items], dtype=np.int64)
+    ends = np.array([e for _, _, e in items], dtype=np.int64)
+    orders = (np.lexsort((starts, -sizes)),
+              np.lexsort((starts, -sizes * (ends - starts + 1))),
+              np.lexsort((-sizes, starts)))
+    best = None
+    for order in orders:
+        offsets, peak = _place_order(sizes, starts, ends, order, base)
+        if best is None or peak < best[1]:
+            best = (offsets, peak)
+    for (b, _, _), off in zip(items, best[0].tolist()):
+        b.loc = int(off)
+    return best[1]
+
+
 class PlanCompiler:
     """Compile a (reordered) bucket network into a device plan.
 
@@ -222,14 +276,18 @@ class PlanCompiler:
     """
 
     def __init__(self, lds_tensor_limit=1 << 12, lds_capacity=None, elem_bytes=8, lanes=64,
-                 gather=True, window=None, gate_space="lds"):
+                 gather=True, window=None, gate_space="lds", schedule="list"):
         self.lds_tensor_limit = lds_tensor_limit
         self.lds_capacity = lds_capacity      # elements available per item, or None
         self.elem_bytes = elem_bytes
         self.lanes = lanes
         self.gather = gather
-        # a step may join one of the last `window` ops (None: any op after its inputs)
+        # first-fit scheduling: a step may join one of the last `window` ops
+        # (None: any op after its inputs)
         self.window = window
+        if schedule not in ("levels", "alap", "first_fit", "list"):
+            raise ValueError(f"unknown schedule {schedule!r}")
+        self.schedule = schedule
         # "global": gates and every intermediate live in global memory (wide plans)
         if gate_space not in ("lds", "global"):
             raise ValueError("gate_space is 'lds' or 'global'")
@@ -354,61 +412,49 @@ class PlanCompiler:
         # phase 2: schedule steps into ops
         lanes = self.lanes
         lim = self.lds_tensor_limit
-        ops = []          # list of lists of step indices
-        op_kind = []      # True: gather op
-        op_fill = []
+        if self.schedule == "first_fit":
+            ops, op_kind = self._schedule_first_fit(steps, lanes, lim)
+        elif self.schedule == "list":
+            ops, op_kind = self._schedule_list(steps, lanes, lim)
+        else:
+            ops, op_kind = self._schedule_levels(steps, lanes, lim, alap=self.schedule == "alap")
         op_of = [0] * len(steps)
-        for i, st in enumerate(steps):
-            ready = max([op_of[nd.step] + 1 for nd in st.inputs if nd.step >= 0] + [0])
-            if self.window is not None:
-                ready = max(ready, len(ops) - self.window)
-            if self.gather and st.out.n <= lim and st.gatherable(lanes, lim):
-                k = ready
-                while k < len(ops) and not (op_kind[k] and op_fill[k] + st.out.n <= lanes
-                                            and len(ops[k]) < GATHER_MAX_STEPS):
-                    k += 1
-                if k == len(ops):
-                    ops.append([])
-                    op_kind.append(True)
-                    op_fill.append(0)
-                ops[k].append(i)
-                op_fill[k] += st.out.n
-            else:
-                k = len(ops) if ready >= len(ops) else max(ready, len(ops))
-                ops.append([i])
-                op_kind.append(False)
-                op_fill.append(st.out.n)
-            op_of[i] = k
+        for k, members in enumerate(ops):
+            for i in members:
+                op_of[i] = k
 
-        # phase 3: allocation in op order
+        # phase 3: allocation.  An intermediate lives from its op to its last
+        # reading op, inclusive (the result to the end), so an op's outputs never
+        # share storage with its inputs.  Offsets come from an offline interval
+        # allocator; the first-fit arena only provides the gate area's base.
         for i, st in enumerate(steps):
             for nd in st.inputs:
                 nd.buf.last_op = max(nd.buf.last_op, op_of[i])
-        lds = _Arena(base=0 if wide else gate_area, align=2)
-        glb = _Arena(base=(gate_area + 63) // 64 * 64 if wide else 0, align=64)
         if wide:
             lim = 0
-        by_last = {}
-        for i, st in enumerate(steps):
-            for nd in st.inputs:
-                if not nd.buf.leaf:
-                    by_last.setdefault(nd.buf.last_op, set()).add(nd.buf)
+        lds_items, glb_items = [], []
+        end_all = len(ops)
+        seen = set()
         for k, members in enumerate(ops):
             for i in members:
                 b = steps[i].out.buf
-                if b.n <= lim:
-                    s = lds.alloc(b.n, self.lds_capacity)
-                    if s is None:
-                        raise ValueError("LDS capacity exhausted; use a larger capacity, a smaller "
-                                         "LDS tensor limit or more slicing")
-                    b.loc = s
-                else:
-                    b.loc = GLOBAL | glb.alloc(b.n)
-            for b in by_last.get(k, ()):
-                if b.loc & GLOBAL:
-                    glb.release(b.loc & ~GLOBAL, b.n)
-                else:
-                    lds.release(b.loc, b.n)
+                assert id(b) not in seen, "a buffer is written by two steps"
+                seen.add(id(b))
+                end = b.last_op if b.last_op >= 0 else end_all
+                (lds_items if b.n <= lim else glb_items).append((b, k, end))
+        lds_base = 0 if wide else gate_area
+        glb_base = (gate_area + 63) // 64 * 64 if wide else 0
+        lds_peak = _place_intervals(lds_items, lds_base, 2)
+        if self.lds_capacity is not None and lds_peak > self.lds_capacity:
+            raise ValueError("LDS capacity exhausted; use a larger capacity, a smaller "
+                             "LDS tensor limit or more slicing")
+        glb_peak = _place_intervals(glb_items, glb_base, 64)
+        for b, _, _ in glb_items:
+            b.loc = GLOBAL | b.loc
+        lds = _Arena(base=lds_base)
+        lds.peak = lds_peak
+        glb = _Arena(base=glb_base)
+        glb.peak = glb_peak
 
         # phase 4: encode ops; barriers only matter for global data
         op_words = []
@@ -460,6 +506,126 @@ class PlanCompiler:
                     bytes_step_model=total_bytes, flops=total_flops, n_barriers=barriers,
                     n_gather=n_gather, step_sizes=step_sizes, steps_words=steps_words, dag=dag)
 
+    def _gather_ok(self, st, lanes, lim):
+        return self.gather and st.out.n <= lim and st.gatherable(lanes, lim)
+
+    def _schedule_first_fit(self, steps, lanes, lim):
+        """Steps in order; a gatherable step joins the first gather op after its
+        inputs with room, any other step opens a new op at the end."""
+        ops, op_kind, op_fill = [], [], []
+        op_of = [0] * len(steps)
+        for i, st in enumerate(steps):
+            ready = max([op_of[nd.step] + 1 for nd in st.inputs if nd.step >= 0] + [0])
+            if self.window is not None:
+                ready = max(ready, len(ops) - self.window)
+            if self._gather_ok(st, lanes, lim):
+                k = ready
+                while k < len(ops) and not (op_kind[k] and op_fill[k] + st.out.n <= lanes
+                                            and len(ops[k]) < GATHER_MAX_STEPS):
+                    k += 1
+                if k == len(ops):
+                    ops.append([])
+                    op_kind.append(True)
+                    op_fill.append(0)
+                ops[k].append(i)
+                op_fill[k] += st.out.n
+            else:
+                k = len(ops)
+                ops.append([i])
+                op_kind.append(False)
+                op_fill.append(st.out.n)
+            op_of[i] = k
+        return ops, op_kind
+
+    def _schedule_levels(self, steps, lanes, lim, alap=False):
+        """Level scheduling: every step gets its dependency level (as soon as
+        possible, or as late as possible with ``alap``); the gatherable steps of
+        a level are packed into gather ops (first fit by decreasing size, at most
+        ``lanes`` outputs and GATHER_MAX_STEPS steps each), the others get one
+        step op each.  The op count approaches the dependency depth."""
+        n = len(steps)
+        prods = [[nd.step for nd in st.inputs if nd.step >= 0] for st in steps]
+        level = [0] * n
+        for i in range(n):
+            level[i] = 1 + max([level[j] for j in prods[i]] + [-1])
+        if alap:
+            last = max(level) if n else 0
+            cons = [[] for _ in range(n)]
+            for i in range(n):
+                for j in prods[i]:
+                    cons[j].append(i)
+            for i in reversed(range(n)):
+                level[i] = min([level[c] for c in cons[i]] + [last + 1]) - 1
+        by_level = {}
+        for i in range(n):
+            by_level.setdefault(level[i], []).append(i)
+        ops, op_kind = [], []
+        for lv in sorted(by_level):
+            members = by_level[lv]
+            bins = []
+            for i in sorted((i for i in members if self._gather_ok(steps[i], lanes, lim)),
+                            key=lambda i: -steps[i].out.n):
+                m = steps[i].out.n
+                for b in bins:
+                    if b[0] + m <= lanes and len(b[1]) < GATHER_MAX_STEPS:
+                        b[0] += m
+                        b[1].append(i)
+                        break
+                else:
+                    bins.append([m, [i]])
+            for _, b in bins:
+                ops.append(sorted(b))
+                op_kind.append(True)
+            for i in members:
+                if not self._gather_ok(steps[i], lanes, lim):
+                    ops.append([i])
+                    op_kind.append(False)
+        return ops, op_kind
+
+    def _schedule_list(self, steps, lanes, lim):
+        """List scheduling by height (longest path to the result): each op is
+        built from the ready steps, highest first; a gather op takes ready
+        gatherable steps while it has room, a step that cannot be gathered gets
+        an op of its own when it is the highest ready step."""
+        n = len(steps)
+        prods = [[nd.step for nd in st.inputs if nd.step >= 0] for st in steps]
+        cons = [[] for _ in range(n)]
+        for i in range(n):
+            for j in set(prods[i]):
+                cons[j].append(i)
+        gok = [self._gather_ok(st, lanes, lim) for st in steps]
+        height = [0] * n
+        for i in reversed(range(n)):
+            height[i] = (1 if gok[i] else 2) + max([height[c] for c in cons[i]] + [0])
+        waiting = [len(set(pr)) for pr in prods]
+        ready = [i for i in range(n) if waiting[i] == 0]
+        ops, op_kind = [], []
+        done = 0
+        while done < n:
+            ready.sort(key=lambda i: (-height[i], i))
+            top = ready[0]
+            if not gok[top]:
+                chosen = [top]
+                kind = False
+            else:
+                chosen, fill = [], 0
+                for i in ready:
+                    if gok[i] and fill + steps[i].out.n <= lanes and len(chosen) < GATHER_MAX_STEPS:
+                        chosen.append(i)
+                        fill += steps[i].out.n
+                kind = True
+            ops.append(sorted(chosen))
+            op_kind.append(kind)
+            chosen_set = set(chosen)
+            ready = [i for i in ready if i not in chosen_set]
+            done += len(chosen)
+            for i in chosen:
+                for c in cons[i]:
+                    waiting[c] -= 1
+                    if waiting[c] == 0:
+                        ready.append(c)
+        return ops, op_kind
+
     @staticmethod
     def _encode_step(st, sync_glb):
         rec_len = STEP_HDR + IN_WORDS * len(st.inputs)
@@ -487,13 +653,14 @@ class PlanCompiler:
         hdr_len = GATHER_HDR + table_words
         desc_off = -(-hdr_len // 4) * 4      # 16-byte aligned descriptors
         length = desc_off + GATHER_DESC * n
-        words = [OP_GATHER | n | (length << 16), pos, desc_off, 0]
+        words = [OP_GATHER | n | (length << 16), pos, desc_off, max(len(st.inputs) for st in members)]
         for t in range(table_words):
             b = lane_step[4 * t: 4 * t + 4]
             words.append(_i32(b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24)))
         words += [0] * (desc_off - len(words))
         for s_idx, st in enumerate(members):
-            d = [_i32(st.out.buf.loc), first[s_idx] | (len(st.inputs) << 8) | (st.sum_flag << 12)]
+            assert first[s_idx] < (1 << 12)
+            d = [_i32(st.out.buf.loc), first[s_idx] | (st.sum_flag << 12) | (len(st.inputs) << 16)]
             for j in range(MAX_IN):
                 if j < len(st.inputs):
                     loc, top, low = st.inputs[j].buf.loc, st.tops[j], st.lows[j]

@@ -31,7 +31,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .plan import PROGRAM_PAD
+from .batched import order_items
+from .plan import PARAM_PAD, PROGRAM_PAD
 
 
 def predicted_bytes(workload, elem_bytes=8):
@@ -286,7 +287,7 @@ class EnergyServer:
         cached = getattr(wl, "_serving_items", None)
         if cached is not None and cached[0] is self.store:
             return cached[1:]
-        rows, sizes = [], []
+        rows, sizes, costs = [], [], []
         lds_need, glb_need = 2, 0
         for w in wl.works:
             plan = wl.plans[w.key]
@@ -294,24 +295,28 @@ class EnergyServer:
             lds_need = max(lds_need, plan.lds_elems(self.elem))
             glb_need = max(glb_need, plan.glb_need)
             rows += [(off, 0, s, 0) for s in range(w.n_slices)]
+            costs += [plan.n_ops] * w.n_slices
             sizes.append(w.n_slices)
         arr = np.asarray(rows, dtype=np.int32).reshape(-1, 4)
-        wl._serving_items = (self.store, arr, np.asarray(sizes, dtype=np.int64), lds_need, glb_need)
+        wl._serving_items = (self.store, arr, np.asarray(sizes, dtype=np.int64), lds_need, glb_need,
+                             np.asarray(costs, dtype=np.int64))
         return wl._serving_items[1:]
 
     def _launch(self, slot, batch):
         import torch
-        parts, sizes, params = [], [], []
+        parts, sizes, params, costs = [], [], [], []
         lds_need, glb_need = 2, 0
         for req in batch:
-            rows, seg_sizes, ln, gn = self._items_of(req)
+            rows, seg_sizes, ln, gn, cost = self._items_of(req)
             rows = rows.copy()
             rows[:, 1] = len(params)
             params += req.gamma + req.beta
             parts.append(rows)
+            costs.append(cost)
             sizes.append(seg_sizes)
             lds_need, glb_need = max(lds_need, ln), max(glb_need, gn)
-        items = np.concatenate(parts)
+        items = order_items(np.concatenate(parts), np.concatenate(costs))
+        params += [0.0] * PARAM_PAD      # the executor reads 8 angles per item at once
         seg = np.concatenate([[0], np.cumsum(np.concatenate(sizes))]).astype(np.int32)
         n_items, n_terms = items.shape[0], seg.size - 1
         # newly stored plans were copied on this thread's stream

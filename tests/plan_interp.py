@@ -113,8 +113,9 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
         length = w0 >> 16
         assert pos + length <= soff + ops_words
         if w0 & P.OP_GATHER:
-            n_steps, n_act, desc_off = w0 & 0x1F, w[pos + 1], w[pos + 2]
+            n_steps, n_act, desc_off, ni = w0 & 0x1F, w[pos + 1], w[pos + 2], w[pos + 3]
             assert n_act <= lanes and 1 <= n_steps <= P.GATHER_MAX_STEPS and desc_off % 4 == 0
+            assert 1 <= ni <= P.MAX_IN
             table = []
             for t in range(-(-lanes // 4)):
                 word = w[pos + P.GATHER_HDR + t] & 0xFFFFFFFF
@@ -126,11 +127,11 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
                 d = [x & 0xFFFFFFFF for x in w[pos + desc_off + P.GATHER_DESC * sidx:
                                               pos + desc_off + P.GATHER_DESC * (sidx + 1)]]
                 out_loc, meta = d[0], d[1]
-                o = lane - (meta & 255)
-                n_in, sm = (meta >> 8) & 15, (meta >> 12) & 1
+                o = lane - (meta & 0xfff)
+                n_in, sm = (meta >> 16) & 15, (meta >> 12) & 1
                 assert 1 <= n_in <= P.MAX_IN and o >= 0
                 x, y = 1 + 0j, 1 + 0j
-                for j in range(P.MAX_IN):
+                for j in range(ni):           # the device reads the op's first ni inputs
                     locw, rw = d[2 + 2 * j], d[3 + 2 * j]
                     a = (locw & 0xFFFF) + int(_runs_offset(o, rw, 0)[0])
                     if j >= n_in:

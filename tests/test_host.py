@@ -123,6 +123,21 @@ def test_plan_program_vs_oracle(Comp, p, lim, cap):
             assert plan.lds_need <= cap
 
 
+@pytest.mark.parametrize("schedule", ["list", "first_fit", "levels", "alap"])
+@pytest.mark.parametrize("lanes", [64, 256])
+def test_plan_schedules_vs_statevector(schedule, lanes):
+    """Every op scheduler and gather width gives the same terms; ops never
+    exceed the lane count and the allocation respects the LDS capacity."""
+    G = nx.random_regular_graph(3, 14, seed=5)
+    rng = np.random.default_rng(5)
+    g, b = list(rng.uniform(0, np.pi, 2)), list(rng.uniform(0, np.pi / 2, 2))
+    sv = statevector.zz_expectations(G, g, b)
+    for e in list(G.edges())[:3]:
+        plan = _edge_plan(G, e, 2, lds_tensor_limit=512, lds_capacity=4096, lanes=lanes, schedule=schedule)
+        assert plan.lds_need <= 4096
+        assert abs(run_plan(plan.words, g + b, lanes=lanes) - sv[e]) < 1e-10
+
+
 def test_plan_p3_and_slices_vs_oracle():
     G = nx.random_regular_graph(3, 40, seed=1)
     g, b = [0.3, 0.7, 0.2], [0.5, 0.1, 0.9]
@@ -219,3 +234,31 @@ def test_wide_plan_program_vs_oracle():
     assert ws.n_slices == 4
     total = sum(run_plan(ws.plan.words, g + b, slice_bits=s, lanes=256) for s in range(4))
     assert abs(total - ref) < 1e-10
+
+
+def test_autotuner_two_stage_search(monkeypatch):
+    """Stage 1 times every slicing threshold under the base shape, stage 2 the
+    other shapes at the two best thresholds; the fastest candidate wins."""
+    import networkx as nx
+    from qtensor_amd import autotune
+    from qtensor_amd.batched import LaunchShape
+    G = nx.random_regular_graph(3, 16, seed=7)
+    calls = []
+
+    def fake_time(self, G, p, gamma, beta, shape, mtw):
+        calls.append((shape.key(), mtw))
+        if shape.key() == (64, 4096, 2048):
+            raise ValueError("LDS capacity exhausted")
+        cost = {None: 5.0, 13: 4.0, 12: 3.0, 11: 2.0, 10: 2.5}[mtw]
+        return cost - 0.1 * (shape.item_threads == 256) - 0.01 * (shape.lds_capacity == 2048), 0.0, None, None
+
+    monkeypatch.setattr(autotune.Autotuner, "time_config", fake_time)
+    monkeypatch.setattr(autotune, "slicing_candidates", lambda max_tw: [None, 13, 12, 11, 10])
+    shape, mtw, table = autotune.Autotuner().tune(G, 2, [0.1, 0.2], [0.3, 0.4])
+    base = [c for c in calls if c[0] == autotune.BASE_SHAPE.key()]
+    assert [m for _, m in base] == [None, 13, 12, 11, 10]
+    rest = [c for c in calls if c[0] != autotune.BASE_SHAPE.key()]
+    assert sorted({m for _, m in rest}, key=str) == sorted({11, 10}, key=str)
+    assert len(rest) == 2 * len(autotune.OTHER_SHAPES)
+    assert shape == LaunchShape(256, 2048, 1024) and mtw == 11
+    assert sum("error" in row for row in table) == 2
