@@ -22,6 +22,9 @@
  *                        (qtensor/FeynmanSimulator.py:49-58), all steps on the device
  *   qb_reduce_terms   -> sum of slice partials (FeynmanSimulator.py:83) and of edge
  *                        terms (QAOASimulator.py:60), in FP64
+ *   qb_greedy_order   -> min-degree greedy elimination order (qtensor/utils.py:12-44,
+ *                        used by OrderingOptimizer, qtensor/optimisation/Optimizer.py:46-88)
+ *   qb_elimination_widths -> widths of a fixed order (the same elimination, utils.py:77-86)
  *   qb_init / qb_shutdown / qb_last_error -> backend construction and the Python
  *                        exceptions of the reference (AssertionError, ValueError)
  */
@@ -151,6 +154,23 @@ int qb_wide_execute(const qb_wide_desc* desc, void* stream);
  */
 int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segments,
                     double* term_out, double* total, void* stream);
+
+/*
+ * Host-only planning (no device work; usable without qb_init).
+ * A graph of n_nodes nodes 0..n_nodes-1 in the caller's node order is given in
+ * CSR form: the neighbours of node v are adj_idx[adj_ptr[v] .. adj_ptr[v+1]).
+ * Self loops and repeated neighbours are ignored; edges are taken as undirected.
+ *
+ * qb_greedy_order: repeatedly eliminate the node with the fewest neighbours
+ * (the first such node in node order) and join its neighbours into a clique;
+ * order[k] is the k-th eliminated node and widths[k] its neighbour count then.
+ */
+int qb_greedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, int32_t* order,
+                    int32_t* widths);
+
+/* Widths of eliminating order[0..n_order) in that order (same elimination rule). */
+int qb_elimination_widths(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, int32_t n_order,
+                          const int32_t* order, int32_t* widths);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,8 @@ QB_MAX_RANK = 31
 QB_ITEM_WORDS = 4
 
 EXPORTED = ("qb_version", "qb_init", "qb_shutdown", "qb_last_error", "qb_step",
-            "qb_exec_slots", "qb_execute", "qb_wide_execute", "qb_reduce_terms")
+            "qb_exec_slots", "qb_execute", "qb_wide_execute", "qb_reduce_terms",
+            "qb_greedy_order", "qb_elimination_widths")
 
 
 class StepDesc(ctypes.Structure):
@@ -78,6 +79,9 @@ def load():
         lib.qb_exec_slots.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_int32)]
         lib.qb_execute.argtypes = [ctypes.POINTER(ExecDesc), ctypes.c_void_p]
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        lib.qb_greedy_order.argtypes = [ctypes.c_int32, i32p, i32p, i32p, i32p]
+        lib.qb_elimination_widths.argtypes = [ctypes.c_int32, i32p, i32p, ctypes.c_int32, i32p, i32p]
         lib.qb_wide_execute.argtypes = [ctypes.POINTER(WideDesc), ctypes.c_void_p]
         lib.qb_reduce_terms.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]

@@ -1,0 +1,124 @@
+// planner.cpp -- host-side elimination ordering for the bucket-elimination engine.
+//
+// Min-degree greedy ordering of an interaction (line) graph, the rule of
+// qtensor/utils.py:12-44 (get_neighbours_peo_vars) used by
+// qtensor/optimisation/Optimizer.py:46-88: repeatedly eliminate the node with
+// the fewest distinct neighbours -- the first such node in graph order -- and
+// join its neighbours into a clique.  The width of a step is the neighbour
+// count at elimination.  The same elimination with a fixed order gives the
+// widths of that order.  Plain host code: no device work.
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/qtensor_amd.h"
+#include "internal.h"
+
+namespace {
+
+struct Graph {
+    std::vector<std::vector<int32_t>> nb;   // distinct neighbours, no self loops
+    std::vector<uint32_t> mark;
+    uint32_t stamp = 0;
+
+    bool load(int32_t n, const int32_t* ptr, const int32_t* idx) {
+        nb.assign(n, {});
+        mark.assign(n, 0u);
+        for (int32_t v = 0; v < n; ++v) {
+            if (ptr[v + 1] < ptr[v]) return false;
+            ++stamp;
+            mark[v] = stamp;      // drop self loops
+            for (int32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
+                const int32_t u = idx[k];
+                if (u < 0 || u >= n) return false;
+                if (mark[u] != stamp) {
+                    mark[u] = stamp;
+                    nb[v].push_back(u);
+                }
+            }
+        }
+        // symmetric closure, as an undirected graph
+        for (int32_t v = 0; v < n; ++v)
+            for (int32_t u : nb[v]) {
+                bool found = false;
+                for (int32_t w : nb[u])
+                    if (w == v) { found = true; break; }
+                if (!found) nb[u].push_back(v);
+            }
+        return true;
+    }
+
+    // Remove v: its neighbours lose v and become a clique.  Returns its width.
+    int32_t eliminate(int32_t v) {
+        std::vector<int32_t> nv;
+        nv.swap(nb[v]);
+        for (int32_t u : nv) {
+            std::vector<int32_t>& s = nb[u];
+            ++stamp;
+            size_t w = 0;
+            for (size_t k = 0; k < s.size(); ++k) {
+                if (s[k] == v) continue;
+                mark[s[k]] = stamp;
+                s[w++] = s[k];
+            }
+            s.resize(w);
+            mark[u] = stamp;
+            for (int32_t x : nv)
+                if (mark[x] != stamp) {
+                    mark[x] = stamp;
+                    s.push_back(x);
+                }
+        }
+        return (int32_t)nv.size();
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int qb_greedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, int32_t* order,
+                    int32_t* widths) {
+    if (n_nodes < 0 || (n_nodes > 0 && (!adj_ptr || !adj_idx || !order || !widths)))
+        return qb_internal::fail(QB_ERR_INVALID, "qb_greedy_order: bad node count or null pointer");
+    Graph g;
+    if (!g.load(n_nodes, adj_ptr, adj_idx))
+        return qb_internal::fail(QB_ERR_INVALID, "qb_greedy_order: malformed adjacency");
+    std::vector<char> alive(n_nodes, 1);
+    for (int32_t step = 0; step < n_nodes; ++step) {
+        int32_t best = -1;
+        size_t best_deg = 0;
+        for (int32_t v = 0; v < n_nodes; ++v) {
+            if (!alive[v]) continue;
+            if (best < 0 || g.nb[v].size() < best_deg) {
+                best = v;
+                best_deg = g.nb[v].size();
+            }
+        }
+        alive[best] = 0;
+        order[step] = best;
+        widths[step] = g.eliminate(best);
+    }
+    return QB_OK;
+}
+
+int qb_elimination_widths(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, int32_t n_order,
+                          const int32_t* order, int32_t* widths) {
+    if (n_nodes < 0 || n_order < 0 || n_order > n_nodes || (n_nodes > 0 && (!adj_ptr || !adj_idx)) ||
+        (n_order > 0 && (!order || !widths)))
+        return qb_internal::fail(QB_ERR_INVALID, "qb_elimination_widths: bad size or null pointer");
+    Graph g;
+    if (!g.load(n_nodes, adj_ptr, adj_idx))
+        return qb_internal::fail(QB_ERR_INVALID, "qb_elimination_widths: malformed adjacency");
+    std::vector<char> alive(n_nodes, 1);
+    for (int32_t k = 0; k < n_order; ++k) {
+        const int32_t v = order[k];
+        if (v < 0 || v >= n_nodes || !alive[v])
+            return qb_internal::fail(QB_ERR_INVALID, "qb_elimination_widths: order repeats or leaves the graph");
+        alive[v] = 0;
+        widths[k] = g.eliminate(v);
+    }
+    return QB_OK;
+}
+
+}  // extern "C"

@@ -28,6 +28,7 @@
 #include <string>
 
 #include "../../include/qtensor_amd.h"
+#include "internal.h"
 
 namespace {
 
@@ -1049,6 +1050,10 @@ int launch_step_k(const StepArgs& a, const uint32_t* runs_unused, int32_t dtype,
 }
 
 }  // namespace
+
+namespace qb_internal {
+int fail(int code, const char* msg) { return set_error(code, "%s", msg); }
+}  // namespace qb_internal
 
 extern "C" {
 

@@ -16,6 +16,7 @@
   node with probability proportional to exp(-(degree - min degree)/temp), keep the
   narrowest of ``repeats`` samples (seeded here).
 """
+import ctypes
 import os
 import shutil
 import subprocess
@@ -23,6 +24,7 @@ import tempfile
 
 import numpy as np
 
+from . import _lib
 from .model import Var
 
 
@@ -30,36 +32,49 @@ def _adjacency(graph):
     return {n: set(graph[n]) - {n} for n in graph.nodes()}
 
 
+def _csr(graph):
+    """Nodes in graph order and the CSR adjacency (int32) of ``graph``."""
+    nodes = list(graph.nodes())
+    index = {n: i for i, n in enumerate(nodes)}
+    ptr = np.zeros(len(nodes) + 1, dtype=np.int32)
+    idx = []
+    for i, n in enumerate(nodes):
+        nb = [index[u] for u in graph[n] if u != n]
+        idx += nb
+        ptr[i + 1] = ptr[i] + len(nb)
+    return nodes, ptr, np.asarray(idx, dtype=np.int32)
+
+
+def _i32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
 def greedy_peo(graph):
-    """Return ``(peo, degrees)`` of the min-degree greedy elimination."""
-    adj = _adjacency(graph)
-    peo, degs = [], []
-    while adj:
-        best = min(adj, key=lambda n: len(adj[n]))
-        nb = adj.pop(best)
-        peo.append(best)
-        degs.append(len(nb))
-        for u in nb:
-            s = adj[u]
-            s.discard(best)
-            s.update(nb)
-            s.discard(u)
-    return peo, degs
+    """Return ``(peo, degrees)`` of the min-degree greedy elimination
+    (``qtensor/utils.py:12-44``): the node with the fewest distinct
+    neighbours, first in graph order, is eliminated and its neighbours are
+    joined into a clique.  Runs in the native planner (``qb_greedy_order``)."""
+    nodes, ptr, idx = _csr(graph)
+    n = len(nodes)
+    order = np.zeros(max(n, 1), dtype=np.int32)
+    widths = np.zeros(max(n, 1), dtype=np.int32)
+    lib = _lib.load()
+    _lib.check(lib.qb_greedy_order(n, _i32p(ptr), _i32p(idx if idx.size else np.zeros(1, np.int32)),
+                                   _i32p(order), _i32p(widths)))
+    return [nodes[k] for k in order[:n]], [int(w) for w in widths[:n]]
 
 
 def elimination_widths(graph, peo):
-    """Neighbour count of each node when eliminated in ``peo`` order."""
-    adj = _adjacency(graph)
-    degs = []
-    for n in peo:
-        nb = adj.pop(n)
-        degs.append(len(nb))
-        for u in nb:
-            s = adj[u]
-            s.discard(n)
-            s.update(nb)
-            s.discard(u)
-    return degs
+    """Neighbour count of each node when eliminated in ``peo`` order
+    (``qb_elimination_widths``)."""
+    nodes, ptr, idx = _csr(graph)
+    index = {v: i for i, v in enumerate(nodes)}
+    order = np.asarray([index[v] for v in peo] or [0], dtype=np.int32)
+    widths = np.zeros(max(len(peo), 1), dtype=np.int32)
+    lib = _lib.load()
+    _lib.check(lib.qb_elimination_widths(len(nodes), _i32p(ptr), _i32p(idx if idx.size else np.zeros(1, np.int32)),
+                                         len(peo), _i32p(order), _i32p(widths)))
+    return [int(w) for w in widths[:len(peo)]]
 
 
 def _vars(graph, ids):

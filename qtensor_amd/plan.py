@@ -246,9 +246,9 @@ def _place_order(sizes, starts, ends, order, base):
 
 def _place_intervals(items, base, align):
     """Offline interval allocation: ``items`` are (buf, first_op, last_op);
-    buffers whose op intervals overlap get disjoint element ranges.  Three
-    placement orders are tried (largest first; largest size x lifetime first;
-    earliest first), the one with the lowest peak is kept.  Sets ``buf.loc``;
+    buffers whose op intervals overlap get disjoint element ranges.  Two
+    placement orders are tried (largest first; largest size x lifetime first),
+    the one with the lowest peak is kept.  Sets ``buf.loc``;
     returns the peak offset."""
     if not items:
         return base
@@ -256,8 +256,7 @@ def _place_intervals(items, base, align):
     starts = np.array([s for _, s, _ in items], dtype=np.int64)
     ends = np.array([e for _, _, e in items], dtype=np.int64)
     orders = (np.lexsort((starts, -sizes)),
-              np.lexsort((starts, -sizes * (ends - starts + 1))),
-              np.lexsort((-sizes, starts)))
+              np.lexsort((starts, -sizes * (ends - starts + 1))))
     best = None
     for order in orders:
         offsets, peak = _place_order(sizes, starts, ends, order, base)

@@ -262,3 +262,33 @@ def test_autotuner_two_stage_search(monkeypatch):
     assert len(rest) == 2 * len(autotune.OTHER_SHAPES)
     assert shape == LaunchShape(256, 2048, 1024) and mtw == 11
     assert sum("error" in row for row in table) == 2
+
+
+def test_native_greedy_matches_oracle():
+    """qb_greedy_order / qb_elimination_widths against the plain-Python
+    restatement: random graphs (self loops, isolated nodes, shuffled labels)
+    and QAOA lightcone line graphs."""
+    from oracle import ordering as oo
+    from qtensor_amd.composer import QtreeQAOAComposer
+    from qtensor_amd.tensornet import QtreeTensorNet
+    rng = np.random.default_rng(11)
+    graphs = [nx.Graph(), nx.empty_graph(3), nx.complete_graph(6)]
+    for k in range(30):
+        g = nx.gnp_random_graph(int(rng.integers(2, 60)), float(rng.uniform(0.02, 0.4)), seed=int(rng.integers(1 << 30)))
+        labels = list(rng.permutation(g.number_of_nodes()) * 7 + 3)
+        g = nx.relabel_nodes(g, dict(zip(g.nodes(), labels)))
+        if g.number_of_nodes() and k % 3 == 0:
+            v = next(iter(g.nodes()))
+            g.add_edge(v, v)
+        graphs.append(g)
+    G = nx.random_regular_graph(3, 24, seed=2)
+    for e in list(G.edges())[:4]:
+        comp = QtreeQAOAComposer(G, gamma=[0.1, 0.2], beta=[0.3, 0.4])
+        comp.energy_expectation_lightcone(e)
+        graphs.append(QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph())
+    for g in graphs:
+        assert greedy_peo(g) == oo.greedy_peo(g)
+        order = list(rng.permutation(list(g.nodes()))) if g.number_of_nodes() else []
+        assert elimination_widths(g, order) == oo.elimination_widths(g, order)
+    with pytest.raises(ValueError):
+        elimination_widths(nx.path_graph(3), [0, 0])

@@ -173,3 +173,20 @@ def test_planted_mistake_is_caught(monkeypatch):
     E = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend()).energy_expectation(
         G, f["gamma"], f["beta"])
     assert abs(E[0] - f["energy"]) > 1e-3
+
+
+def test_oracle_greedy_known_answers():
+    """Closed forms of min-degree elimination: a path has width 1, a cycle 2,
+    K_n gives n-1, n-2, ..., 0; a star eliminates its leaves (width 1) first."""
+    import networkx as nx
+    from oracle import ordering as oo
+    assert oo.greedy_peo(nx.path_graph(6))[1] == [1, 1, 1, 1, 1, 0]
+    assert max(oo.greedy_peo(nx.cycle_graph(7))[1]) == 2
+    assert oo.greedy_peo(nx.complete_graph(5))[1] == [4, 3, 2, 1, 0]
+    peo, degs = oo.greedy_peo(nx.star_graph(4))
+    assert peo[:3] == [1, 2, 3] and degs[:3] == [1, 1, 1]
+    # ties go to the first node in graph order
+    g = nx.Graph()
+    g.add_nodes_from([3, 1, 2])
+    g.add_edges_from([(3, 1), (1, 2)])
+    assert oo.greedy_peo(g)[0][0] == 3
