@@ -25,6 +25,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per executor launch on the default workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs (profiles/exec_kernel_qaoa_p3_n256.md): 6341.2 KiB
+# fetched x 2.0 (a 1 GiB device copy reads as 524306.6 KiB) + 78.6 KiB written
+# x 1.0 (the copy's 1 GiB writes read as 1048576 KiB).
+EXEC_TRAFFIC_BYTES = int(6341.2 * 1024 * 2.0 + 78.6 * 1024 * 1.0)
+
 
 def workload(name):
     import networkx as nx
@@ -286,6 +292,8 @@ def main():
     tune_s = time.perf_counter() - t0
     print(f"[bench] tuned: {shape.key()} max_tw={max_tw} ({tune_s:.1f} s)", file=sys.stderr, flush=True)
 
+    from qtensor_amd.batched import clear_plan_cache
+    clear_plan_cache()                  # the autotuner compiled these plans already
     eng = LightconeEngine(shape=shape, max_tw=max_tw)
     t0 = time.perf_counter()
     wl = eng.workload(G, p)
@@ -368,7 +376,8 @@ def main():
         "energy": energy,
         "plan_seconds": {"cold": round(plan_cold, 3), "warm": round(plan_warm, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": EXEC_TRAFFIC_BYTES if args.workload == "c2" else None,
                      "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
                      "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,
                      "executed_step_bytes_per_launch": batch.bytes_step_model},

@@ -20,6 +20,10 @@
  *   qb_wide_execute   -> bucket_elimination(process_bucket) over one large network
  *                        (qtensor/Simulate.py:90-93) or one slice of it
  *                        (qtensor/FeynmanSimulator.py:49-58), all steps on the device
+ *   qb_plan_create / qb_plan_execute / qb_plan_destroy -> the same energy evaluation
+ *                        with library-owned device buffers: compile once, re-evaluate
+ *                        for new angles (QAOASimulator.energy_expectation,
+ *                        QAOASimulator.py:44-66, over a fixed graph)
  *   qb_reduce_terms   -> sum of slice partials (FeynmanSimulator.py:83) and of edge
  *                        terms (QAOASimulator.py:60), in FP64
  *   qb_greedy_order   -> min-degree greedy elimination order (qtensor/utils.py:12-44,
@@ -94,8 +98,8 @@ int qb_step(const qb_step_desc* desc, const void* const* inputs, void* out, void
 
 /*
  * Batched executor: every work item contracts one compiled lightcone plan
- * (one edge, or one slice of one edge) to a scalar.  A persistent grid pulls
- * items from `queue` (one uint32, zeroed by the call); each item builds its
+ * (one edge, or one slice of one edge) to a scalar.  A persistent grid runs
+ * item b in block b first and pulls later items from `queue`; each item builds its
  * gate tensors in LDS from its FP64 angles and runs the plan's steps with
  * intermediates in LDS or, above the plan's LDS threshold, in its slot of
  * `scratch`.  Writes one complex128 (re, im) per item to item_out.
@@ -115,7 +119,8 @@ typedef struct qb_exec_desc {
     int32_t n_slots;             /* slots available in scratch */
     int32_t reserved;
     double* item_out;            /* device: 2 * n_items doubles */
-    uint32_t* queue;             /* device: one uint32 */
+    uint32_t* queue;             /* device: one uint32 (reset by the call when the
+                                    items outnumber the resident blocks) */
 } qb_exec_desc;
 
 /* Number of concurrently resident work items (= scratch slots needed) for a
@@ -154,6 +159,38 @@ int qb_wide_execute(const qb_wide_desc* desc, void* stream);
  */
 int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segments,
                     double* term_out, double* total, void* stream);
+
+/*
+ * Plan handles (owned device state).  qb_plan_create uploads a batched program
+ * (concatenated compiled plans followed by at least 320 zero words), its work
+ * items (QB_ITEM_WORDS each, as for qb_execute) and the term segments
+ * (n_terms + 1 offsets into the items), and allocates item outputs, scratch
+ * slots and the work queue on the current device.  Items and segments are
+ * checked on the host.  qb_plan_execute runs the executor and the FP64
+ * reductions on `stream` for one device angle buffer `params` (readable for
+ * n_params + 8 doubles): term_out gets 2 * n_terms doubles (complex terms),
+ * total 2 doubles.  Calls on one handle must not overlap (they share its
+ * buffers).  qb_plan_destroy frees everything (null is allowed).
+ */
+typedef struct qb_plan qb_plan;
+
+typedef struct qb_plan_desc {
+    const int32_t* program;      /* host: program words */
+    int64_t program_words;
+    const int32_t* items;        /* host: n_items * QB_ITEM_WORDS */
+    int32_t n_items;
+    int32_t n_terms;
+    const int32_t* seg;          /* host: n_terms + 1 item offsets */
+    int32_t n_params;            /* doubles of one angle buffer (without padding) */
+    int32_t item_threads;        /* 64, 128, 256, 512 or 1024 */
+    int32_t lds_elems;           /* LDS elements per item (>= every plan's need) */
+    int32_t dtype;               /* QB_C64 or QB_C128 */
+    int64_t scratch_elems;       /* global scratch elements per resident item */
+} qb_plan_desc;
+
+int qb_plan_create(const qb_plan_desc* desc, qb_plan** plan);
+int qb_plan_execute(qb_plan* plan, const double* params, double* term_out, double* total, void* stream);
+int qb_plan_destroy(qb_plan* plan);
 
 /*
  * Host-only planning (no device work; usable without qb_init).

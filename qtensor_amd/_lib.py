@@ -23,7 +23,8 @@ QB_ITEM_WORDS = 4
 
 EXPORTED = ("qb_version", "qb_init", "qb_shutdown", "qb_last_error", "qb_step",
             "qb_exec_slots", "qb_execute", "qb_wide_execute", "qb_reduce_terms",
-            "qb_greedy_order", "qb_elimination_widths")
+            "qb_greedy_order", "qb_elimination_widths",
+            "qb_plan_create", "qb_plan_execute", "qb_plan_destroy")
 
 
 class StepDesc(ctypes.Structure):
@@ -48,6 +49,13 @@ class WideDesc(ctypes.Structure):
                 ("arena", ctypes.c_void_p), ("arena_elems", ctypes.c_int64),
                 ("slice_bits", ctypes.c_uint32), ("dtype", ctypes.c_int32),
                 ("out", ctypes.c_void_p)]
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [("program", ctypes.c_void_p), ("program_words", ctypes.c_int64),
+                ("items", ctypes.c_void_p), ("n_items", ctypes.c_int32), ("n_terms", ctypes.c_int32),
+                ("seg", ctypes.c_void_p), ("n_params", ctypes.c_int32), ("item_threads", ctypes.c_int32),
+                ("lds_elems", ctypes.c_int32), ("dtype", ctypes.c_int32), ("scratch_elems", ctypes.c_int64)]
 
 
 class QBError(RuntimeError):
@@ -79,6 +87,9 @@ def load():
         lib.qb_exec_slots.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_int32)]
         lib.qb_execute.argtypes = [ctypes.POINTER(ExecDesc), ctypes.c_void_p]
+        lib.qb_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(ctypes.c_void_p)]
+        lib.qb_plan_execute.argtypes = [ctypes.c_void_p] * 5
+        lib.qb_plan_destroy.argtypes = [ctypes.c_void_p]
         i32p = ctypes.POINTER(ctypes.c_int32)
         lib.qb_greedy_order.argtypes = [ctypes.c_int32, i32p, i32p, i32p, i32p]
         lib.qb_elimination_widths.argtypes = [ctypes.c_int32, i32p, i32p, ctypes.c_int32, i32p, i32p]

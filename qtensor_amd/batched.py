@@ -82,6 +82,11 @@ class GraphWorkload:
 # compiled plans, shared by every engine in the process (plans do not depend on
 # the angles or the precision, only on network structure and launch shape)
 _PLAN_CACHE = {}
+
+
+def clear_plan_cache():
+    """Forget every compiled plan (the next workload plans from scratch)."""
+    _PLAN_CACHE.clear()
 _PLAN_LOCK = threading.Lock()
 
 
@@ -246,6 +251,7 @@ class Batch:
                                           ctypes.byref(slots)))
         self.n_slots = slots.value
         items_np = order_items(items, order)
+        self._host = (program, items_np, np.asarray(seg, dtype=np.int32), len(params))
         self.program = torch.from_numpy(program).to(dev)
         self.items = torch.from_numpy(np.ascontiguousarray(items_np)).to(dev)
         # the executor reads 8 angles from every item's offset in one load
@@ -291,7 +297,7 @@ class Batch:
         self.params[:self.n_params].copy_(_torch().tensor(flat, dtype=_torch().float64))
 
     def launch_exec(self, stream=None):
-        """Launch only the executor (queue reset + persistent kernel)."""
+        """Launch only the executor (one persistent kernel)."""
         torch = _torch()
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
         _lib.check(self.lib.qb_execute(ctypes.byref(self._desc), ctypes.c_void_p(st.cuda_stream)))
@@ -335,3 +341,59 @@ class Batch:
 
     def item_values(self):
         return self.item_out.cpu().numpy().reshape(-1, 2)
+
+
+class PlanHandle:
+    """A batch's program and items behind the C-ABI plan handle
+    (``qb_plan_create`` / ``qb_plan_execute`` / ``qb_plan_destroy``): the
+    library owns every device buffer; only the angle buffer is passed in."""
+
+    def __init__(self, batch):
+        program, items, seg, n_params = batch._host
+        self.lib = batch.lib
+        self.device = batch.device
+        self.n_terms = batch.n_terms
+        self._arrays = (np.ascontiguousarray(program, dtype=np.int32), np.ascontiguousarray(items, dtype=np.int32),
+                        np.ascontiguousarray(seg, dtype=np.int32))
+        d = _lib.PlanDesc()
+        d.program = self._arrays[0].ctypes.data
+        d.program_words = self._arrays[0].size
+        d.items = self._arrays[1].ctypes.data
+        d.n_items = batch.n_items
+        d.n_terms = batch.n_terms
+        d.seg = self._arrays[2].ctypes.data
+        d.n_params = n_params
+        d.item_threads = batch.shape.item_threads
+        d.lds_elems = batch.lds_elems
+        d.dtype = batch.qb_dtype
+        d.scratch_elems = batch.scratch_elems
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.qb_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h
+
+    def execute(self, params, stream=None):
+        """Complex terms and total for ``params`` (a device float64 tensor with
+        at least n_params + 8 entries)."""
+        torch = _torch()
+        term_out = torch.empty(2 * self.n_terms, dtype=torch.float64, device=self.device)
+        total = torch.empty(2, dtype=torch.float64, device=self.device)
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        _lib.check(self.lib.qb_plan_execute(self.handle, ctypes.c_void_p(params.data_ptr()),
+                                            ctypes.c_void_p(term_out.data_ptr()),
+                                            ctypes.c_void_p(total.data_ptr()), ctypes.c_void_p(st.cuda_stream)))
+        st.synchronize()
+        t = term_out.cpu().numpy().reshape(-1, 2)
+        tot = total.cpu().numpy()
+        return [complex(a, b) for a, b in t], complex(tot[0], tot[1])
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _lib.check(self.lib.qb_plan_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

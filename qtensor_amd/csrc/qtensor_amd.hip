@@ -801,6 +801,8 @@ __global__ __launch_bounds__(ITEM_THREADS) void exec_kernel(
     }
 }
 
+// Queue reset before a launch whose items outnumber its blocks (only such
+// launches take tickets).
 __global__ void zero_u32(uint32_t* p) { *p = 0u; }
 
 // Neumaier-compensated FP64 complex sums in a fixed order (deterministic):
@@ -1167,8 +1169,10 @@ int qb_execute(const qb_exec_desc* d, void* stream) {
     a.item_out = d->item_out;
     a.queue = d->queue;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, st, d->queue);
-    QB_HIP(hipGetLastError());
+    if (grid < d->n_items) {
+        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, st, d->queue);
+        QB_HIP(hipGetLastError());
+    }
     const int elem = d->dtype == QB_C128 ? 16 : 8;
     const int lds_bytes = d->lds_elems * elem * groups;
     return d->dtype == QB_C128 ? exec_launch_dispatch<double>(d->item_threads, a, grid, lds_bytes, st)
@@ -1271,6 +1275,136 @@ int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segmen
     QB_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_total, dim3(1), dim3(256), 0, st, term_out, n_segments, total);
     QB_HIP(hipGetLastError());
+    return QB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plan handles: a batched program, its work items and every buffer one
+// energy evaluation needs, owned by the library.
+// ---------------------------------------------------------------------------
+
+struct qb_plan {
+    int device = -1;
+    int32_t dtype = QB_C64;
+    int32_t item_threads = 0;
+    int32_t lds_elems = 0;
+    int32_t n_items = 0;
+    int32_t n_terms = 0;
+    int32_t n_params = 0;
+    int32_t n_slots = 0;
+    int64_t scratch_elems = 0;
+    int32_t* program = nullptr;
+    int32_t* items = nullptr;
+    int32_t* seg = nullptr;
+    double* item_out = nullptr;
+    void* scratch = nullptr;
+    uint32_t* queue = nullptr;
+    double* pool = nullptr;
+};
+
+static void plan_free(qb_plan* p) {
+    if (!p) return;
+    if (p->device >= 0) (void)hipSetDevice(p->device);
+    (void)hipFree(p->program);
+    (void)hipFree(p->items);
+    (void)hipFree(p->seg);
+    (void)hipFree(p->item_out);
+    (void)hipFree(p->scratch);
+    (void)hipFree(p->queue);
+    (void)hipFree(p->pool);
+    delete p;
+}
+
+int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
+    if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
+    if (!d || !out) return set_error(QB_ERR_INVALID, "null descriptor or output");
+    *out = nullptr;
+    if (d->dtype != QB_C64 && d->dtype != QB_C128) return set_error(QB_ERR_INVALID, "dtype %d", d->dtype);
+    if (!d->program || d->program_words < 1 || !d->items || d->n_items < 1 || !d->seg || d->n_terms < 1 ||
+        d->n_params < 0 || d->scratch_elems < 0)
+        return set_error(QB_ERR_INVALID, "empty or negative program, items, segments or sizes");
+    if (d->n_terms > (1 << 20)) return set_error(QB_ERR_INVALID, "n_terms=%d > 2^20", d->n_terms);
+    // host-side checks of what the kernels will index
+    for (int32_t t = 0; t < d->n_terms; ++t)
+        if (d->seg[t] > d->seg[t + 1]) return set_error(QB_ERR_INVALID, "segments must not decrease");
+    if (d->seg[0] != 0 || d->seg[d->n_terms] != d->n_items)
+        return set_error(QB_ERR_INVALID, "segments must cover items 0..n_items");
+    for (int32_t i = 0; i < d->n_items; ++i) {
+        const int32_t* it = d->items + (size_t)i * QB_ITEM_WORDS;
+        if (it[0] < 0 || (int64_t)it[0] + kPlanHdr > d->program_words)
+            return set_error(QB_ERR_INVALID, "item %d: plan offset %d outside the program", i, it[0]);
+        if (it[1] < 0 || it[1] > d->n_params) return set_error(QB_ERR_INVALID, "item %d: angle offset", i);
+        if (it[3] < 0 || it[3] >= d->n_items) return set_error(QB_ERR_INVALID, "item %d: output slot", i);
+    }
+    int32_t n_slots = 0;
+    int per_cu = 0;
+    int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, &n_slots, &per_cu);
+    if (rc) return rc;
+    qb_plan* p = new qb_plan();
+    p->device = g_device;
+    p->dtype = d->dtype;
+    p->item_threads = d->item_threads;
+    p->lds_elems = d->lds_elems;
+    p->n_items = d->n_items;
+    p->n_terms = d->n_terms;
+    p->n_params = d->n_params;
+    p->n_slots = n_slots < d->n_items ? n_slots : d->n_items;
+    p->scratch_elems = d->scratch_elems;
+    const size_t elem = d->dtype == QB_C128 ? 16 : 8;
+    hipError_t e = hipSuccess;
+#define QB_PLAN_TRY(call)                                                          \
+    do {                                                                           \
+        e = (call);                                                                \
+        if (e != hipSuccess) {                                                     \
+            plan_free(p);                                                          \
+            return set_error(QB_ERR_HIP, "%s: %s", #call, hipGetErrorString(e));   \
+        }                                                                          \
+    } while (0)
+    QB_PLAN_TRY(hipMalloc(&p->program, sizeof(int32_t) * (size_t)d->program_words));
+    QB_PLAN_TRY(hipMalloc(&p->items, sizeof(int32_t) * QB_ITEM_WORDS * (size_t)d->n_items));
+    QB_PLAN_TRY(hipMalloc(&p->seg, sizeof(int32_t) * ((size_t)d->n_terms + 1)));
+    QB_PLAN_TRY(hipMalloc(&p->item_out, sizeof(double) * 2 * (size_t)d->n_items));
+    QB_PLAN_TRY(hipMalloc(&p->queue, sizeof(uint32_t)));
+    QB_PLAN_TRY(hipMalloc(&p->pool, sizeof(double) * 4));
+    if (d->scratch_elems > 0)
+        QB_PLAN_TRY(hipMalloc(&p->scratch, elem * (size_t)d->scratch_elems * (size_t)p->n_slots));
+    QB_PLAN_TRY(hipMemcpy(p->program, d->program, sizeof(int32_t) * (size_t)d->program_words,
+                          hipMemcpyHostToDevice));
+    QB_PLAN_TRY(hipMemcpy(p->items, d->items, sizeof(int32_t) * QB_ITEM_WORDS * (size_t)d->n_items,
+                          hipMemcpyHostToDevice));
+    QB_PLAN_TRY(hipMemcpy(p->seg, d->seg, sizeof(int32_t) * ((size_t)d->n_terms + 1), hipMemcpyHostToDevice));
+    QB_PLAN_TRY(hipMemset(p->queue, 0, sizeof(uint32_t)));
+    QB_PLAN_TRY(hipMemset(p->pool, 0, sizeof(double) * 4));
+#undef QB_PLAN_TRY
+    *out = p;
+    return QB_OK;
+}
+
+int qb_plan_execute(qb_plan* p, const double* params, double* term_out, double* total, void* stream) {
+    if (!p) return set_error(QB_ERR_INVALID, "null plan");
+    if (!params || !term_out || !total) return set_error(QB_ERR_INVALID, "null device pointer");
+    qb_exec_desc d;
+    d.program = p->program;
+    d.items = p->items;
+    d.n_items = p->n_items;
+    d.item_threads = p->item_threads;
+    d.lds_elems = p->lds_elems;
+    d.dtype = p->dtype;
+    d.params = params;
+    d.const_pool = p->pool;
+    d.scratch = p->scratch;
+    d.scratch_elems = p->scratch_elems;
+    d.n_slots = p->n_slots;
+    d.reserved = 0;
+    d.item_out = p->item_out;
+    d.queue = p->queue;
+    int rc = qb_execute(&d, stream);
+    if (rc) return rc;
+    return qb_reduce_terms(p->item_out, p->seg, p->n_terms, term_out, total, stream);
+}
+
+int qb_plan_destroy(qb_plan* p) {
+    plan_free(p);
     return QB_OK;
 }
 

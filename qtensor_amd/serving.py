@@ -331,7 +331,9 @@ class EnergyServer:
         d_out, _ = self._ensure(slot, "item_out", 2 * n_items, torch.float64)
         d_terms, h_terms = self._ensure(slot, "terms", 2 * n_terms, torch.float64)
         d_total, _ = self._ensure(slot, "total", 2, torch.float64)
-        d_queue, _ = self._ensure(slot, "queue", 1, torch.int32)
+        d_queue = slot.tensors.get("queue")
+        if d_queue is None:          # reset by qb_execute whenever a launch takes tickets
+            d_queue = slot.tensors["queue"] = torch.zeros(1, dtype=torch.int32, device=self.device)
         scr_elems = glb_need * used_slots if glb_need else 1
         d_scr, _ = self._ensure(slot, "scratch", scr_elems * (self.elem // 8), torch.complex64)
         h_items[:4 * n_items].copy_(torch.from_numpy(items.reshape(-1)))

@@ -109,3 +109,35 @@ def test_mixed_requests_match_alone(gpu):
         alone.run()
         for a, m in zip(alone.terms()[0], terms):
             assert abs(a - m) <= 1e-12
+
+
+@pytest.mark.parametrize("shape,max_tw", [(LaunchShape(128, 4096, 2048), 7), (LaunchShape(64, 2048, 64), None)],
+                         ids=["sliced_lds", "global_scratch"])
+def test_plan_handle_matches_batch(gpu, shape, max_tw):
+    """qb_plan_create / qb_plan_execute (library-owned buffers) give the batch's
+    terms bit for bit, for the batch's angles and for new ones, and agree with
+    the statevector; the first case has more items than one row of blocks
+    needs, the second spills tensors to global scratch."""
+    import torch
+    from qtensor_amd.batched import PlanHandle
+    G = nx.random_regular_graph(3, 14, seed=9)
+    rng = np.random.default_rng(9)
+    g, b = list(rng.uniform(0, np.pi, 2)), list(rng.uniform(0, np.pi / 2, 2))
+    eng = LightconeEngine(shape=shape, max_tw=max_tw)
+    wl = eng.workload(G, 2)
+    batch = eng.batch([(wl, g, b)])
+    batch.run()
+    ref = batch.terms()[0]
+    h = PlanHandle(batch)
+    for rep in range(2):
+        terms, total = h.execute(batch.params)
+        assert terms == ref
+        assert abs(total - sum(ref)) < 1e-12
+    g2, b2 = list(rng.uniform(0, np.pi, 2)), list(rng.uniform(0, np.pi / 2, 2))
+    prm = torch.tensor(g2 + b2 + [0.0] * 8, dtype=torch.float64, device="cuda")
+    terms2, _ = h.execute(prm)
+    sv = statevector.zz_expectations(G, g2, b2)
+    for e, z in zip(G.edges(), terms2):
+        assert abs(z - sv[e]) <= _tol(sv[e])
+    h.close()
+    h.close()
