@@ -11,7 +11,8 @@ Reported beside the value: the plain per-bucket GPU backend on a sample of
 edges, the error of every term against a complex128 run of the same
 kernels and of a sample against the numpy oracle, the executor kernel's
 achieved step-model bandwidth against the HBM peak, and the oracle timed on
-one host core (the CPU baseline).
+the host (the CPU baseline: a process pool over edges, with single-core
+complex128 / complex64 rates and the planning / contraction split beside it).
 """
 import argparse
 import json
@@ -54,18 +55,100 @@ def workload(name):
     return G, p, gamma, beta, desc
 
 
-def cpu_baseline(G, gamma, beta, budget_s):
-    """Oracle (numpy, complex128, reference-faithful) per edge on one core."""
+class _TimedBackend:
+    """Oracle backend wrapper that accumulates the time spent contracting."""
+
+    def __init__(self, inner):
+        self.inner = inner
+        self.seconds = 0.0
+
+    def process_bucket(self, bucket, no_sum=False):
+        t0 = time.perf_counter()
+        r = self.inner.process_bucket(bucket, no_sum=no_sum)
+        self.seconds += time.perf_counter() - t0
+        return r
+
+    def get_sliced_buckets(self, buckets, data_dict, slice_dict):
+        t0 = time.perf_counter()
+        r = self.inner.get_sliced_buckets(buckets, data_dict, slice_dict)
+        self.seconds += time.perf_counter() - t0
+        return r
+
+    def get_result_data(self, result):
+        return self.inner.get_result_data(result)
+
+
+def _oracle_edges(G, gamma, beta, edges, dtype, budget_s):
+    """Edge terms by the numpy oracle on this process, one after another."""
+    import numpy as np
     from oracle import np_framework as npf
     from qtensor_amd import QAOAQtreeSimulator, QtreeQAOAComposer
-    sim = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend())
+    be = _TimedBackend(npf.NumpyBackend(dtype=getattr(np, dtype)))
+    sim = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=be)
     vals, t0 = [], time.perf_counter()
-    for e in G.edges():
+    for e in edges:
         vals.append(complex(sim._get_edge_energy(G, gamma, beta, e)[0]))
-        if time.perf_counter() - t0 > budget_s:
+        if budget_s is not None and time.perf_counter() - t0 > budget_s:
             break
-    dt = time.perf_counter() - t0
-    return vals, dt
+    return vals, time.perf_counter() - t0, be.seconds
+
+
+_POOL_JOB = None
+
+
+def _pool_init(job):
+    global _POOL_JOB
+    _POOL_JOB = job
+
+
+def _pool_edge(e):
+    G, gamma, beta = _POOL_JOB
+    return _oracle_edges(G, gamma, beta, [e], "complex128", None)[0][0]
+
+
+def _cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = "unknown"
+    try:
+        import numpy as np
+        cfg = np.show_config(mode="dicts")
+        blas = cfg.get("Build Dependencies", {}).get("blas", {}).get("name", "unknown")
+    except Exception:
+        pass
+    return model, blas
+
+
+def cpu_baseline(G, gamma, beta, budget_s):
+    """The numpy oracle (reference-faithful complex128) on the host: one core
+    over edges in order until ``budget_s`` (time split into planning and
+    contraction), the same sample in complex64, and a process pool over the
+    sample's edges as the reference's energy_expectation_parallel does
+    (qtensor/QAOASimulator.py:72-88)."""
+    import multiprocessing as mp
+    edges = list(G.edges())
+    vals, dt, contract = _oracle_edges(G, gamma, beta, edges, "complex128", budget_s)
+    sample = edges[:len(vals)]
+    _, dt64, _ = _oracle_edges(G, gamma, beta, sample, "complex64", None)
+    n_proc = max(1, min(16, len(os.sched_getaffinity(0))))
+    ctx = mp.get_context("spawn")        # children start clean: no GPU state is inherited
+    with ctx.Pool(n_proc, initializer=_pool_init, initargs=((G, gamma, beta),)) as pool:
+        pool.map(_pool_edge, sample[:n_proc])               # interpreter start-up, outside the timing
+        t0 = time.perf_counter()
+        pvals = pool.map(_pool_edge, sample, chunksize=max(1, len(sample) // (4 * n_proc)))
+        dtp = time.perf_counter() - t0
+    model, blas = _cpu_info()
+    return {"vals": vals, "n": len(vals), "seconds_c128": dt, "contract_seconds_c128": contract,
+            "seconds_c64": dt64, "pool_seconds": dtp, "pool_processes": n_proc,
+            "pool_max_abs_diff": max(abs(a - b) for a, b in zip(vals, pvals)),
+            "cpu_model": model, "blas": blas}
 
 
 def plain_form(G, gamma, beta, n_edges):
@@ -183,8 +266,10 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
             "per_set": per_set}
 
 
-def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11):
-    """Poisson stream of energy requests (C5) through the continuous-batching server."""
+def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4)):
+    """Poisson stream of energy requests (C5) through the continuous-batching server.
+    Depths 1-3 use 64-thread items sliced below treewidth 11; depth 4 uses
+    256-thread items sliced below 16, with larger tensors in global scratch."""
     import networkx as nx
     import numpy as np
     from qtensor_amd.batched import LaunchShape, LightconeEngine
@@ -195,22 +280,32 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11):
         n = int(rng.integers(10, 1001)) * 2
         pool.append((f"rr3_n{n}_s{k}", nx.random_regular_graph(3, n, seed=100 + k)))
     eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw)
-    t0 = time.perf_counter()
-    server = EnergyServer(eng, n_streams=2).start()
-    for key, G in pool:                   # cold planning, outside the timed traces
-        for p in (1, 2, 3):
+    eng4 = LightconeEngine(shape=LaunchShape(256, 4096, 2048), max_tw=16)
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=workers).start()
+    cold = {}
+    for p in p_choices:                  # cold planning, outside the timed traces
+        t0 = time.perf_counter()
+        for key, G in pool:
             f = server.submit(G, [0.1] * p, [0.2] * p, key=key)
-            f.result()
-    cold = time.perf_counter() - t0
-    out = {"pool": [k for k, _ in pool], "p_choices": [1, 2, 3], "cold_planning_s": round(cold, 1), "rates": []}
+            try:
+                f.result()
+            except MemoryError:
+                pass
+        cold[p] = round(time.perf_counter() - t0, 1)
+        print(f"[serve] pool planned for p={p} in {cold[p]} s", file=sys.stderr, flush=True)
+    rejected = server.rejected
+    out = {"pool": [k for k, _ in pool], "p_choices": list(p_choices), "cold_planning_s": cold,
+           "planning_workers": workers, "rejected_at_planning": rejected, "rates": []}
     for rate in rates:
         server.finished.clear()
-        poisson_trace(server, pool, rate, n_requests, rng)
+        server.rejected = 0
+        poisson_trace(server, pool, rate, n_requests, rng, p_choices=p_choices)
         st = server.stats()
         st["rate_per_s"] = rate
         out["rates"].append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
         print(f"[serve] rate {rate}/s: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms "
-              f"{st['terms_per_s']:.0f} terms/s", file=sys.stderr, flush=True)
+              f"{st['terms_per_s']:.0f} terms/s, rejected {st['rejected']}", file=sys.stderr, flush=True)
     server.stop()
     return out
 
@@ -231,7 +326,7 @@ def main():
     ap.add_argument("--precision-sets", type=int, default=16)
     ap.add_argument("--serve", action="store_true",
                     help="run only the serving trace (Poisson arrivals) and print its JSON line")
-    ap.add_argument("--serve-rates", default="20,80,320,1280")
+    ap.add_argument("--serve-rates", default="10,40,160,640")
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--serve-pool", type=int, default=8)
     ap.add_argument("--forced-edge", action="store_true",
@@ -264,7 +359,7 @@ def main():
         print(json.dumps({"metric": "served edge-energy terms/sec and p99 request latency",
                           "value": best["terms_per_s"], "unit": "terms/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "complex64", "data": "synthetic",
-                          "config": {"workload": "poisson_energy_requests_3reg_n20_2000_p1_3"},
+                          "config": {"workload": "poisson_energy_requests_3reg_n20_2000_p1_4"},
                           "serving": res}), flush=True)
         return
 
@@ -392,11 +487,20 @@ def main():
         out["plain"] = {"value": round(n_plain / pdt, 2), "unit": "terms/s", "edges": n_plain,
                         "max_abs_err_vs_batched": float(np.max(np.abs(np.array(pv) - terms64[:n_plain])))}
         print(f"[bench] plain: {n_plain / pdt:.2f} terms/s", file=sys.stderr, flush=True)
-        cv, cdt = cpu_baseline(G, gamma, beta, args.cpu_budget)
-        cv = np.array(cv)
-        out["cpu_baseline"] = {"value": round(len(cv) / cdt, 3), "unit": "terms/s", "cores": 1, "kind": "port",
-                               "sample": f"first {len(cv)} edges of the same workload, numpy complex128 oracle "
-                                         f"incl. per-edge planning, {cdt:.1f} s"}
+        cb = cpu_baseline(G, gamma, beta, args.cpu_budget)
+        cv = np.array(cb["vals"])
+        n = cb["n"]
+        out["cpu_baseline"] = {
+            "value": round(n / cb["pool_seconds"], 3), "unit": "terms/s", "cores": cb["pool_processes"],
+            "kind": "port",
+            "sample": f"first {n} edges of the same workload; numpy complex128 oracle incl. per-edge planning, "
+                      f"{cb['pool_processes']} processes over edges ({cb['pool_seconds']:.1f} s)",
+            "single_core_terms_per_s": {"complex128": round(n / cb["seconds_c128"], 3),
+                                        "complex64": round(n / cb["seconds_c64"], 3)},
+            "single_core_split_s": {"planning": round(cb["seconds_c128"] - cb["contract_seconds_c128"], 2),
+                                    "contraction": round(cb["contract_seconds_c128"], 2)},
+            "cpu_model": cb["cpu_model"], "numpy_blas": cb["blas"],
+            "pool_vs_single_max_abs_diff": cb["pool_max_abs_diff"]}
         out["max_abs_err_vs_oracle_sample"] = float(np.max(np.abs(terms64[:len(cv)] - cv)))
     print(json.dumps(out), flush=True)
 

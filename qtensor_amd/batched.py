@@ -64,6 +64,7 @@ class GraphWorkload:
         self.edges = edges
         self.works = works
         self.plans = plans      # key -> Plan
+        self.shape_key = None   # launch shape the plans were compiled for (set by the server)
 
     @property
     def n_terms(self):
@@ -87,7 +88,45 @@ _PLAN_CACHE = {}
 def clear_plan_cache():
     """Forget every compiled plan (the next workload plans from scratch)."""
     _PLAN_CACHE.clear()
+
+
 _PLAN_LOCK = threading.Lock()
+_POOL = None
+_POOL_SIZE = 0
+
+
+def planning_pool(workers):
+    """Process pool for host planning (started once, spawn: the workers carry no
+    GPU state).  Canonical lightcones and plan compilation are independent per
+    edge / per distinct lightcone, as the reference's edge-parallel energy is
+    (``qtensor/QAOASimulator.py:72-88``)."""
+    global _POOL, _POOL_SIZE
+    import multiprocessing as mp
+    if _POOL is None or _POOL_SIZE != workers:
+        if _POOL is not None:
+            _POOL.terminate()
+        _POOL = mp.get_context("spawn").Pool(workers)
+        _POOL_SIZE = workers
+    return _POOL
+
+
+def _keys_task(args):
+    G, edges, p = args
+    return [(e,) + canonical_lightcone(G, e, p) for e in edges]
+
+
+def _plan_task(args):
+    composer_name, dtype, shape, max_tw, tw_bias, tasks = args
+    from . import composer as _composer
+    eng = LightconeEngine(composer=getattr(_composer, composer_name), dtype=dtype, shape=shape,
+                          max_tw=max_tw, tw_bias=tw_bias)
+    out = []
+    for key, n, ce, i, j, p in tasks:
+        try:
+            out.append((key, eng.plan(key, n, ce, i, j, p, shape), None))
+        except ValueError as exc:
+            out.append((key, None, str(exc)))
+    return out
 
 
 class LightconeEngine:
@@ -151,17 +190,55 @@ class LightconeEngine:
             self._plans[ck] = plan
         return plan
 
-    def workload(self, G, p, edges=None):
+    def workload(self, G, p, edges=None, workers=None):
+        """Plans of every edge's lightcone.  With ``workers`` > 1 the canonical
+        lightcones and the distinct plans not yet cached are computed in a
+        process pool of that size."""
         shape = self.shape_for(p)
         edges = list(G.edges()) if edges is None else list(edges)
+        if workers and workers > 1 and len(edges) > 1:
+            self._plan_parallel(G, p, edges, shape, workers)
+        keyed = self._keyed.pop((id(G), p), None) if hasattr(self, "_keyed") else None
         works, plans = [], {}
         for e in edges:
-            key, n, ce, i, j = canonical_lightcone(G, e, p)
+            if keyed is not None and e in keyed:
+                key, n, ce, i, j = keyed[e]
+            else:
+                key, n, ce, i, j = canonical_lightcone(G, e, p)
             plan = self.plan(key, n, ce, i, j, p, shape)
             pk = (key, p)
             plans[pk] = plan
             works.append(EdgeWork(e, pk, 1 << plan.n_sliced))
         return GraphWorkload(G, p, edges, works, plans)
+
+    def _plan_parallel(self, G, p, edges, shape, workers):
+        pool = planning_pool(workers)
+        chunk = max(1, -(-len(edges) // (4 * workers)))
+        parts = pool.map(_keys_task, [(G, edges[k:k + chunk], p) for k in range(0, len(edges), chunk)])
+        keyed = {row[0]: row[1:] for part in parts for row in part}
+        todo, seen = [], set()
+        for e in edges:
+            key, n, ce, i, j = keyed[e]
+            ck = (key, p, shape.key(), self.composer.__name__, self.max_tw)
+            if key in seen:
+                continue
+            seen.add(key)
+            with self._lock:
+                if ck in self._plans:
+                    continue
+            todo.append((key, n, ce, i, j, p))
+        if todo:
+            chunk = max(1, -(-len(todo) // (2 * workers)))
+            jobs = [(self.composer.__name__, self.dtype, shape, self.max_tw, self.tw_bias, todo[k:k + chunk])
+                    for k in range(0, len(todo), chunk)]
+            for part in pool.map(_plan_task, jobs):
+                for key, plan, err in part:
+                    if plan is not None:
+                        with self._lock:
+                            self._plans[(key, p, shape.key(), self.composer.__name__, self.max_tw)] = plan
+        if not hasattr(self, "_keyed"):
+            self._keyed = {}
+        self._keyed[(id(G), p)] = keyed
 
     # -- device batches -----------------------------------------------------
     def batch(self, requests):

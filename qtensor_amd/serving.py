@@ -126,11 +126,19 @@ class EnergyServer:
     ``engine``: a ``LightconeEngine`` (its launch shape and slicing are used for
     every batch).  ``n_streams``: batches that may be in flight at once.
     ``max_batch_items``: work items per batch.  ``hbm_budget``: bytes for the
-    admission model (None: 80% of free HBM at start).
+    admission model (None: 80% of free HBM at start).  ``engines``: optional
+    ``{p: LightconeEngine}`` overriding ``engine`` for some depths.
+    ``planning_workers``: process-pool size for planning new graphs.
     """
 
-    def __init__(self, engine, n_streams=2, max_batch_items=1 << 16, hbm_budget=None):
+    def __init__(self, engine, n_streams=2, max_batch_items=1 << 16, hbm_budget=None, engines=None,
+                 planning_workers=None):
         self.engine = engine
+        # per-p engines (launch shape and slicing), e.g. wider items and less
+        # slicing for p = 4; requests of one batch share one launch shape
+        self.engines = dict(engines or {})
+        self.planning_workers = planning_workers
+        self.rejected = 0
         self.n_streams = n_streams
         self.max_batch_items = max_batch_items
         self.hbm_budget = hbm_budget
@@ -185,17 +193,26 @@ class EnergyServer:
         lat = np.array([r.t_done - r.t_arrival for r in done]) * 1e3
         terms = sum(r.workload.n_terms for r in done)
         span = max(r.t_done for r in done) - min(r.t_arrival for r in done)
-        return {"requests": len(done), "terms": terms, "p50_ms": float(np.percentile(lat, 50)),
-                "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
-                "terms_per_s": terms / span if span > 0 else float("nan")}
+        return {"requests": len(done), "rejected": self.rejected, "terms": terms,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "terms_per_s": terms / span if span > 0 else float("nan")}
 
     # -- scheduler ---------------------------------------------------------------
+    def engine_for(self, p):
+        return self.engines.get(p, self.engine)
+
     def _workload(self, req):
         p = len(req.gamma)
         k = (req.key, p)
         wl = self._workloads.get(k)
         if wl is None:
-            wl = self.engine.workload(req.G, p)
+            try:
+                wl = self.engine_for(p).workload(req.G, p, workers=self.planning_workers)
+            except ValueError as exc:
+                # e.g. more slice variables than a work item can carry: the
+                # reference's planner stops the same way ("Estimated OOM")
+                raise MemoryError(f"Estimated OOM: {exc}") from exc
+            wl.shape_key = self.engine_for(p).shape_for(p).key()
             self._workloads[k] = wl
         return wl
 
@@ -231,9 +248,11 @@ class EnergyServer:
                         req.workload = self._workload(req)
                         req.nbytes = predicted_bytes(req.workload, self.elem)
                     except Exception as exc:      # planning error goes to the caller
+                        self.rejected += 1
                         req.future.set_exception(exc)
                         continue
                     if not self.admission.fits_alone(req.nbytes):
+                        self.rejected += 1
                         req.future.set_exception(MemoryError(
                             f"Estimated OOM: request needs {req.nbytes} B, budget {self.admission.budget} B"))
                         continue
@@ -245,7 +264,10 @@ class EnergyServer:
                 continue
             self._progress.clear()
             batch, n_items = [], 0
+            shape_key = waiting[0].workload.shape_key
             for req in list(waiting):
+                if req.workload.shape_key != shape_key:
+                    continue              # another launch shape: a later batch
                 items = sum(w.n_slices for w in req.workload.works)
                 if batch and n_items + items > self.max_batch_items:
                     break
@@ -321,7 +343,8 @@ class EnergyServer:
         n_items, n_terms = items.shape[0], seg.size - 1
         # newly stored plans were copied on this thread's stream
         torch.cuda.current_stream(self.device).synchronize()
-        shape = self.engine.shape_for(len(batch[0].gamma))
+        p0 = len(batch[0].gamma)
+        shape = self.engine_for(p0).shape_for(p0)
         slots_c = ctypes.c_int32(0)
         _lib.check(self.lib.qb_exec_slots(shape.item_threads, lds_need, self.qb_dtype, ctypes.byref(slots_c)))
         used_slots = min(slots_c.value, n_items)
@@ -417,5 +440,8 @@ def poisson_trace(server, pool, rate, n_requests, rng, p_choices=(1, 2, 3)):
         beta = list(rng.uniform(0, np.pi / 2, p))
         out.append((server.submit(G, gamma, beta, key=key), key, G, gamma, beta))
     for f, *_ in out:
-        f.result()
+        try:
+            f.result()
+        except MemoryError:      # rejected at admission ("Estimated OOM"); counted by the server
+            pass
     return out

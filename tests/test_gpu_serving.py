@@ -52,3 +52,32 @@ def test_admission_rejects_and_queues(gpu):
             assert max(abs(z - sv[e]) for z, e in zip(zz, small.edges())) <= 1e-5
     finally:
         server.stop()
+
+
+def test_depth4_served_with_its_own_shape_and_intractable_rejected(gpu):
+    """p = 4 requests run with their own engine (256-thread items, slicing
+    below 16, global scratch) beside p <= 3 requests, match the statevector,
+    and a p = 4 request whose lightcones need more slice variables than a work
+    item carries is rejected with "Estimated OOM"."""
+    eng = LightconeEngine(shape=LaunchShape(64, 4096, 2048), max_tw=11)
+    eng4 = LightconeEngine(shape=LaunchShape(256, 4096, 2048), max_tw=16)
+    server = EnergyServer(eng, n_streams=2, engines={4: eng4}).start()
+    ring = nx.cycle_graph(12)                 # p = 4 lightcones of 10 qubits, all in LDS
+    ladder = nx.ladder_graph(7)               # 14 qubits, tensors spill to global scratch
+    G12 = nx.random_regular_graph(3, 12, seed=8)
+    rng = np.random.default_rng(3)
+    reqs = []
+    try:
+        for G, p in [(ring, 4), (G12, 2), (ladder, 4), (G12, 3)]:
+            g, b = list(rng.uniform(0, np.pi, p)), list(rng.uniform(0, np.pi / 2, p))
+            reqs.append((server.submit(G, g, b), G, g, b))
+        hard = server.submit(nx.random_regular_graph(3, 64, seed=102), [0.1] * 4, [0.2] * 4)
+        with pytest.raises(MemoryError, match="Estimated OOM"):
+            hard.result()
+        for fut, G, g, b in reqs:
+            C, zz = fut.result()
+            sv = statevector.zz_expectations(G, g, b)
+            assert max(abs(z - sv[e]) for z, e in zip(zz, G.edges())) <= 1e-5
+    finally:
+        server.stop()
+    assert server.rejected == 1
