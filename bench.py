@@ -273,7 +273,7 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
     import networkx as nx
     import numpy as np
     from qtensor_amd.batched import LaunchShape, LightconeEngine
-    from qtensor_amd.serving import EnergyServer, poisson_trace
+    from qtensor_amd.serving import EnergyServer, poisson_trace, trace_failures
     rng = np.random.default_rng(seed)
     pool = []
     for k in range(pool_size):
@@ -300,12 +300,16 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
     for rate in rates:
         server.finished.clear()
         server.rejected = 0
-        poisson_trace(server, pool, rate, n_requests, rng, p_choices=p_choices)
+        subs = poisson_trace(server, pool, rate, n_requests, rng, p_choices=p_choices)
         st = server.stats()
         st["rate_per_s"] = rate
+        fails = trace_failures(subs)
+        st["failed"] = len(fails)
+        st["failure_kinds"] = sorted({m.split(":")[0] + ": " + m.split(":")[1][:40] for m in fails})
         out["rates"].append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
         print(f"[serve] rate {rate}/s: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms "
-              f"{st['terms_per_s']:.0f} terms/s, rejected {st['rejected']}", file=sys.stderr, flush=True)
+              f"{st['terms_per_s']:.0f} terms/s, rejected {st['rejected']}, failed {st['failed']}",
+              file=sys.stderr, flush=True)
     server.stop()
     return out
 

@@ -97,7 +97,9 @@ class _Slot:
 
 
 class PlanStore:
-    """Device-resident program buffer; every distinct plan is uploaded once."""
+    """Device-resident program buffer; every distinct plan is uploaded once.
+    When full, the buffer grows (doubling) and the old one is kept until the
+    store is dropped, since batches in flight may still read it."""
 
     def __init__(self, device, capacity_words=1 << 22):
         import torch
@@ -105,6 +107,7 @@ class PlanStore:
         self.buf = torch.zeros(capacity_words + PROGRAM_PAD, dtype=torch.int32, device=device)
         self.used = 0
         self.offsets = {}
+        self.retired = []
 
     def offset(self, plan):
         import torch
@@ -113,7 +116,11 @@ class PlanStore:
             return off
         n = plan.words.size
         if self.used + n + PROGRAM_PAD > self.buf.numel():
-            raise MemoryError("plan store is full")
+            cap = max(2 * (self.buf.numel() - PROGRAM_PAD), self.used + n)
+            new = torch.zeros(cap + PROGRAM_PAD, dtype=torch.int32, device=self.device)
+            new[:self.used].copy_(self.buf[:self.used])
+            self.retired.append(self.buf)
+            self.buf = new
         self.buf[self.used:self.used + n].copy_(torch.from_numpy(plan.words))
         self.offsets[id(plan)] = self.used
         self.used += n
@@ -445,3 +452,13 @@ def poisson_trace(server, pool, rate, n_requests, rng, p_choices=(1, 2, 3)):
         except MemoryError:      # rejected at admission ("Estimated OOM"); counted by the server
             pass
     return out
+
+
+def trace_failures(out):
+    """Messages of the requests of a trace that ended in an exception."""
+    msgs = []
+    for f, *_ in out:
+        exc = f.exception()
+        if exc is not None:
+            msgs.append(f"{type(exc).__name__}: {exc}")
+    return msgs
