@@ -8,13 +8,13 @@ simulators) and a batched, plan-compiled executor for whole graphs.
 from .model import Var, Tensor
 from .composer import (QtreeQAOAComposer, ZZQtreeQAOAComposer, SimpZZQtreeComposer,
                        DefaultQAOAComposer, get_edge_subgraph)
-from .ordering import OrderingOptimizer, SlicesOptimizer, TamakiOptimizer, DefaultOptimizer
+from .ordering import OrderingOptimizer, SlicesOptimizer, TamakiOptimizer, TreeTrimSplitter, DefaultOptimizer
 from .tensornet import QtreeTensorNet
 from .simulator import QtreeSimulator, QAOAQtreeSimulator, FeynmanSimulator
 
 __all__ = [
     "Var", "Tensor", "QtreeQAOAComposer", "ZZQtreeQAOAComposer", "SimpZZQtreeComposer",
-    "DefaultQAOAComposer", "get_edge_subgraph", "OrderingOptimizer", "SlicesOptimizer",
+    "DefaultQAOAComposer", "get_edge_subgraph", "OrderingOptimizer", "SlicesOptimizer", "TreeTrimSplitter",
     "TamakiOptimizer", "DefaultOptimizer", "QtreeTensorNet", "QtreeSimulator",
     "QAOAQtreeSimulator", "FeynmanSimulator",
 ]

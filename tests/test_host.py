@@ -153,6 +153,24 @@ def test_plan_p3_and_slices_vs_oracle():
     assert abs(total - ref) < 1e-10
 
 
+def test_tree_trim_slicing_vs_oracle():
+    """TreeTrimSplitter (Optimizer.py:178-220) reaches the target width and the
+    sum over its slices equals the unsliced term (test_qaoa_energy.py:106-121)."""
+    from qtensor_amd.ordering import TreeTrimSplitter
+    G = nx.random_regular_graph(3, 40, seed=1)
+    g, b = [0.3, 0.7, 0.2], [0.5, 0.1, 0.9]
+    sim = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend())
+    e = list(G.edges())[2]
+    ref = sim._get_edge_energy(G, g, b, e)[0]
+    full = _edge_plan(G, e, 3, lds_tensor_limit=256, lds_capacity=4096)
+    opt = TreeTrimSplitter(max_tw=full.treewidth - 3, tw_bias=0)
+    sl = _edge_plan(G, e, 3, lds_tensor_limit=256, lds_capacity=4096, optimizer=opt)
+    assert sl.n_sliced >= 1 and sl.treewidth <= full.treewidth - 3
+    assert opt.slice_history[0][1] == full.treewidth and opt.slice_history[-1][1] == sl.treewidth
+    total = sum(run_plan(sl.words, g + b, slice_bits=s) for s in range(1 << sl.n_sliced))
+    assert abs(total - ref) < 1e-10
+
+
 def test_plan_reparameterise_without_recompile():
     G = nx.random_regular_graph(3, 12, seed=4)
     e = list(G.edges())[2]
