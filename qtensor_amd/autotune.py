@@ -11,6 +11,7 @@ Results are memoised per (graph signature, p, composer, dtype) in-process and
 can be saved to / loaded from a JSON file.
 """
 import json
+import sys
 import time
 from dataclasses import asdict
 
@@ -102,7 +103,7 @@ class Autotuner:
             table.append({"shape": asdict(shape), "max_tw": mtw, "exec_ms": ms, "plan_s": plan_s})
             timed.append((ms, mtw))
             if self.verbose:
-                print(f"[autotune] {shape.key()} max_tw={mtw}: {ms:.4f} ms", flush=True)
+                print(f"[autotune] {shape.key()} max_tw={mtw}: {ms:.4f} ms", file=sys.stderr, flush=True)
             if best is None or ms < best[0]:
                 best = (ms, shape, mtw)
 
