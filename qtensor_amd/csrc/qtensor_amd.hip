@@ -294,6 +294,52 @@ __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Fused steps (wide plans): several bucket steps contracted in one pass
+// ---------------------------------------------------------------------------
+
+constexpr int kFusedMaxIn = 8;
+constexpr int kFusedHdr = 8;
+constexpr uint32_t kOpFused = 1u << 14;
+
+struct FusedArgs {
+    const char* in[kFusedMaxIn];  // input base addresses in the arena
+    uint32_t mask[kFusedMaxIn];   // input variables as bits of the union offset
+    const int32_t* lo_tab;        // (1 << lo_bits) x n_in offsets of the low output bits
+    const int32_t* x_tab;         // (1 << k) x n_in offsets of the summed variables
+    char* out;
+    uint32_t omask;               // output variables as bits of the union offset
+    uint32_t rank_out, k, lo_bits;
+};
+
+// out[o] = sum_x prod_j in_j[off_j(o) + x_tab[x][j]], off_j(o) = lo_tab[o_lo][j]
+// + pext(pdep(o_hi << lo_bits, omask), mask_j).  One thread per output element,
+// grid-stride; blocks of 256 threads share o_hi, so the high part is uniform.
+template <typename R, int NI>
+__global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a) {
+    using V = typename Cx<R>::V;
+    const uint64_t n_out = 1ull << a.rank_out;
+    const uint32_t lo_mask = (1u << a.lo_bits) - 1u;
+    const uint32_t n_x = 1u << a.k;
+    for (uint64_t o = (uint64_t)blockIdx.x * 256u + threadIdx.x; o < n_out; o += (uint64_t)gridDim.x * 256u) {
+        const uint32_t lo = (uint32_t)o & lo_mask;
+        const uint32_t uh = pdep_loop((uint32_t)(o & ~(uint64_t)lo_mask), a.omask);
+        uint32_t base[NI];
+#pragma unroll
+        for (int j = 0; j < NI; ++j) base[j] = (uint32_t)a.lo_tab[lo * NI + j] + pext_loop(uh, a.mask[j]);
+        V acc = {0, 0};
+        for (uint32_t x = 0; x < n_x; ++x) {
+            const int32_t* xt = a.x_tab + x * NI;
+            V p = reinterpret_cast<const V*>(a.in[0])[base[0] + (uint32_t)xt[0]];
+#pragma unroll
+            for (int j = 1; j < NI; ++j)
+                p = cmul(p, reinterpret_cast<const V*>(a.in[j])[base[j] + (uint32_t)xt[j]]);
+            acc = cadd(acc, p);
+        }
+        reinterpret_cast<V*>(a.out)[o] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batched executor
 // ---------------------------------------------------------------------------
 
@@ -1053,6 +1099,79 @@ int launch_step_k(const StepArgs& a, const uint32_t* runs_unused, int32_t dtype,
 
 }  // namespace
 
+// Parse, check and launch one fused op of a wide plan (host copy `w`, device
+// copy `plan_dev`).  Every offset the kernel forms is a pext of a union offset
+// by an input's mask: the tables are recomputed here from the masks and must
+// match, and each input and the output must lie inside the arena.
+int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, int64_t pos, int32_t k_op,
+                 char* arena, int64_t elem, int64_t cap, int32_t dtype, hipStream_t st) {
+    if (pos + kFusedHdr > plan_words) return set_error(QB_ERR_INVALID, "fused op %d outside the plan", k_op);
+    const uint32_t w0 = (uint32_t)w[pos];
+    const uint32_t n_in = w0 & 15u;
+    const uint32_t meta = (uint32_t)w[pos + 1];
+    const uint32_t rank_u = meta & 255u, rank_out = (meta >> 8) & 255u, k = (meta >> 16) & 255u,
+                   lo_bits = meta >> 24;
+    const uint32_t out_loc = (uint32_t)w[pos + 2];
+    const uint32_t omask = (uint32_t)w[pos + 4], smask = (uint32_t)w[pos + 5];
+    if (n_in < 1 || n_in > (uint32_t)kFusedMaxIn) return set_error(QB_ERR_INVALID, "fused op %d: %u inputs", k_op, n_in);
+    if (rank_u > 30 || rank_out + k != rank_u || k > 8 || lo_bits != (rank_out < 8 ? rank_out : 8u) ||
+        (omask & smask) || (uint32_t)__builtin_popcount(omask) != rank_out ||
+        (uint32_t)__builtin_popcount(smask) != k || ((omask | smask) >> rank_u))
+        return set_error(QB_ERR_INVALID, "fused op %d: inconsistent shape", k_op);
+    const int64_t tab = pos + kFusedHdr + 4 * (int64_t)n_in;
+    const int64_t len = tab + ((int64_t)n_in << lo_bits) + ((int64_t)n_in << k) - pos;
+    if ((int64_t)(w0 >> 16) != len || pos + len > plan_words)
+        return set_error(QB_ERR_INVALID, "fused op %d: record length", k_op);
+    if (!(out_loc & kGlobal) || (int64_t)(out_loc & ~kGlobal) + (1ll << rank_out) > cap)
+        return set_error(QB_ERR_INVALID, "fused op %d output outside the arena", k_op);
+    FusedArgs a;
+    memset(&a, 0, sizeof(a));
+    for (uint32_t j = 0; j < n_in; ++j) {
+        const uint32_t loc = (uint32_t)w[pos + kFusedHdr + 4 * j], mask = (uint32_t)w[pos + kFusedHdr + 4 * j + 1];
+        if (mask >> rank_u) return set_error(QB_ERR_INVALID, "fused op %d input %u mask", k_op, j);
+        if (!(loc & kGlobal) || (int64_t)(loc & ~kGlobal) + (1ll << __builtin_popcount(mask)) > cap)
+            return set_error(QB_ERR_INVALID, "fused op %d input %u outside the arena", k_op, j);
+        a.in[j] = arena + (int64_t)(loc & ~kGlobal) * elem;
+        a.mask[j] = mask;
+        for (uint32_t o = 0; o < (1u << lo_bits); ++o)
+            if ((uint32_t)w[tab + o * n_in + j] != pext_host(pdep_host(o, omask), mask))
+                return set_error(QB_ERR_INVALID, "fused op %d: output table", k_op);
+        for (uint32_t x = 0; x < (1u << k); ++x)
+            if ((uint32_t)w[tab + ((int64_t)n_in << lo_bits) + x * n_in + j] != pext_host(pdep_host(x, smask), mask))
+                return set_error(QB_ERR_INVALID, "fused op %d: summed-variable table", k_op);
+    }
+    a.lo_tab = plan_dev + tab;
+    a.x_tab = plan_dev + tab + ((int64_t)n_in << lo_bits);
+    a.out = arena + (int64_t)(out_loc & ~kGlobal) * elem;
+    a.omask = omask;
+    a.rank_out = rank_out;
+    a.k = k;
+    a.lo_bits = lo_bits;
+    const uint64_t want = ((1ull << rank_out) + 255) / 256;
+    const uint64_t gcap = (uint64_t)g_num_cus * 16;
+    const int grid = (int)(want < gcap ? want : gcap);
+#define QB_FUSED_CASE(N)                                                                      \
+    case N:                                                                                   \
+        if (dtype == QB_C128)                                                                 \
+            hipLaunchKernelGGL((fused_kernel<double, N>), dim3(grid), dim3(256), 0, st, a);   \
+        else                                                                                  \
+            hipLaunchKernelGGL((fused_kernel<float, N>), dim3(grid), dim3(256), 0, st, a);    \
+        break;
+    switch (n_in) {
+        QB_FUSED_CASE(1)
+        QB_FUSED_CASE(2)
+        QB_FUSED_CASE(3)
+        QB_FUSED_CASE(4)
+        QB_FUSED_CASE(5)
+        QB_FUSED_CASE(6)
+        QB_FUSED_CASE(7)
+        QB_FUSED_CASE(8)
+    }
+#undef QB_FUSED_CASE
+    QB_HIP(hipGetLastError());
+    return QB_OK;
+}
+
 namespace qb_internal {
 int fail(int code, const char* msg) { return set_error(code, "%s", msg); }
 }  // namespace qb_internal
@@ -1213,6 +1332,12 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
         if (pos + kStepHdr > d->plan_words) return set_error(QB_ERR_INVALID, "op %d outside the plan", k);
         const uint32_t w0 = (uint32_t)w[pos];
         if (w0 & kOpGather) return set_error(QB_ERR_INVALID, "wide plans take step ops only");
+        if (w0 & kOpFused) {
+            const int rc = launch_fused(w, d->plan_words, d->plan_dev, pos, k, arena, elem, cap, d->dtype, st);
+            if (rc) return rc;
+            pos += w0 >> 16;
+            continue;
+        }
         const uint32_t n_in = w0 & 15u, rank_out = (w0 >> 4) & 31u, sum = (w0 >> 9) & 1u;
         const uint32_t out_loc = (uint32_t)w[pos + 2];
         if (n_in < 1 || n_in > (uint32_t)kExecMaxIn || pos + kStepHdr + kInWords * (int64_t)n_in > d->plan_words)

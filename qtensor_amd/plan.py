@@ -56,6 +56,12 @@ PARAM_PAD = 8
 MAX_RUNS = 2
 GLOBAL = 0x80000000
 OP_GATHER = 1 << 15
+# fused step (wide plans): several bucket steps contracted in one pass
+OP_FUSED = 1 << 14
+FUSED_MAX_IN = 8
+FUSED_MAX_SUM = 8
+FUSED_HDR = 8
+FUSED_IN_WORDS = 4
 GATHER_HDR = 4
 GATHER_DESC = 12
 GATHER_MAX_STEPS = 16
@@ -150,6 +156,109 @@ class SymStep:
         return all(len(mask_runs(m)) <= MAX_RUNS for m in self.lows)
 
 
+class FusedStep:
+    """Several consecutive bucket steps of a wide plan contracted in one pass:
+    out[o] = sum over the summed variables x of prod_j in_j[pext(u(o, x), mask_j)],
+    u = the union offset (variable a of the union at bit rank_u - 1 - a)."""
+    fused = True
+
+    def __init__(self, inputs, out, union_dev, members):
+        self.inputs = inputs
+        self.out = out
+        self.union = sorted(union_dev)
+        self.members = members          # original step indices, last = the output step
+
+    @property
+    def rank_out(self):
+        return len(self.out.dev)
+
+    def gatherable(self, lanes, lds_limit):
+        return False
+
+
+def _pdep(x, mask):
+    r, bit = 0, 0
+    while mask:
+        low = mask & -mask
+        if (x >> bit) & 1:
+            r |= low
+        bit += 1
+        mask &= mask - 1
+    return r
+
+
+def _pext(x, mask):
+    r, bit = 0, 0
+    while mask:
+        low = mask & -mask
+        if x & low:
+            r |= 1 << bit
+        bit += 1
+        mask &= mask - 1
+    return r
+
+
+def _step_time(in_ranks, out_rank, union_rank, bw=4e12, rate=40e12, launch=4e-6):
+    """Time model of one step or fused step on the device (memory, arithmetic
+    including its loads, launch)."""
+    n = len(in_ranks)
+    nbytes = 8 * (sum(1 << r for r in in_ranks) + (1 << out_rank))
+    work = (6 * max(n - 1, 0) + 2 + 4 * n) * (1 << union_rank)
+    return max(nbytes / bw, work / rate) + launch
+
+
+def fuse_steps(steps):
+    """Greedy fusion of single-consumer step chains (every intermediate of
+    bucket elimination has one reader): a step absorbs the group producing one
+    of its inputs when the time model says one pass beats two.  Limits: at
+    most FUSED_MAX_IN external inputs and FUSED_MAX_SUM summed variables, union
+    rank <= 30, and either an output of rank >= 14 or at most 4 summed
+    variables (each output element loops over the summed ones)."""
+    def ranks(nodes):
+        return [len(x.dev) for x in nodes]
+
+    groups = {}        # output step index -> (external inputs, union set, members, time)
+    order = []
+    for i, st in enumerate(steps):
+        out_rank = len(st.out.dev)
+        ext = list(st.inputs)
+        union = set()
+        for nd in st.inputs:
+            union |= set(nd.dev)
+        members = [i]
+        absorbed = []
+        for nd in list(st.inputs):
+            g = groups.get(nd.step) if nd.step >= 0 else None
+            if g is None:
+                continue
+            g_inputs, g_union, g_members, g_time = g
+            cand_ext = [x for x in ext if x is not nd] + g_inputs
+            cand_union = union | g_union
+            k = len(cand_union) - out_rank
+            if not (len(cand_ext) <= FUSED_MAX_IN and k <= FUSED_MAX_SUM and len(cand_union) <= 30
+                    and (out_rank >= 14 or k <= 4)):
+                continue
+            t_sep = g_time + _step_time(ranks(ext), out_rank, len(union))
+            t_fused = _step_time(ranks(cand_ext), out_rank, len(cand_union))
+            if t_fused < 0.9 * t_sep:
+                ext, union = cand_ext, cand_union
+                members = g_members + members
+                absorbed.append(nd.step)
+        for a_ in absorbed:
+            groups.pop(a_)
+            order.remove(a_)
+        groups[i] = (ext, union, members, _step_time(ranks(ext), out_rank, len(union)))
+        order.append(i)
+    out = []
+    for i in order:
+        ext, union, members, _ = groups[i]
+        if len(members) == 1:
+            out.append(steps[i])
+        else:
+            out.append(FusedStep(ext, steps[i].out, union, members))
+    return out
+
+
 @dataclass
 class Plan:
     words: np.ndarray
@@ -165,6 +274,7 @@ class Plan:
     flops: int
     n_barriers: int
     n_gather: int = 0
+    n_fused: int = 0
     step_sizes: list = field(default_factory=list)
     steps_words: int = 0
     dag: list = field(default_factory=list)   # per step: (producer steps of inputs, n_out, n_in, sum)
@@ -275,7 +385,7 @@ class PlanCompiler:
     """
 
     def __init__(self, lds_tensor_limit=1 << 12, lds_capacity=None, elem_bytes=8, lanes=64,
-                 gather=True, window=None, gate_space="lds", schedule="list"):
+                 gather=True, window=None, gate_space="lds", schedule="list", fuse=False):
         self.lds_tensor_limit = lds_tensor_limit
         self.lds_capacity = lds_capacity      # elements available per item, or None
         self.elem_bytes = elem_bytes
@@ -287,6 +397,8 @@ class PlanCompiler:
         if schedule not in ("levels", "alap", "first_fit", "list"):
             raise ValueError(f"unknown schedule {schedule!r}")
         self.schedule = schedule
+        # wide plans: contract chains of steps in one pass (see fuse_steps)
+        self.fuse = fuse
         # "global": gates and every intermediate live in global memory (wide plans)
         if gate_space not in ("lds", "global"):
             raise ValueError("gate_space is 'lds' or 'global'")
@@ -408,16 +520,22 @@ class PlanCompiler:
         if result.dev:
             raise ValueError("network result is not a scalar")
 
-        # phase 2: schedule steps into ops
+        # phase 2: schedule steps into ops.  Wide plans run one grid-wide launch
+        # per op in bucket order, optionally with fused step chains.
         lanes = self.lanes
         lim = self.lds_tensor_limit
-        if self.schedule == "first_fit":
+        exec_steps = steps
+        if wide:
+            if self.fuse:
+                exec_steps = fuse_steps(steps)
+            ops, op_kind = [[i] for i in range(len(exec_steps))], [False] * len(exec_steps)
+        elif self.schedule == "first_fit":
             ops, op_kind = self._schedule_first_fit(steps, lanes, lim)
         elif self.schedule == "list":
             ops, op_kind = self._schedule_list(steps, lanes, lim)
         else:
             ops, op_kind = self._schedule_levels(steps, lanes, lim, alap=self.schedule == "alap")
-        op_of = [0] * len(steps)
+        op_of = [0] * len(exec_steps)
         for k, members in enumerate(ops):
             for i in members:
                 op_of[i] = k
@@ -426,7 +544,7 @@ class PlanCompiler:
         # reading op, inclusive (the result to the end), so an op's outputs never
         # share storage with its inputs.  Offsets come from an offline interval
         # allocator; the first-fit arena only provides the gate area's base.
-        for i, st in enumerate(steps):
+        for i, st in enumerate(exec_steps):
             for nd in st.inputs:
                 nd.buf.last_op = max(nd.buf.last_op, op_of[i])
         if wide:
@@ -436,7 +554,7 @@ class PlanCompiler:
         seen = set()
         for k, members in enumerate(ops):
             for i in members:
-                b = steps[i].out.buf
+                b = exec_steps[i].out.buf
                 assert id(b) not in seen, "a buffer is written by two steps"
                 seen.add(id(b))
                 end = b.last_op if b.last_op >= 0 else end_all
@@ -460,9 +578,10 @@ class PlanCompiler:
         glb_written_since = set()
         barriers = 0
         n_gather = 0
+        n_fused = 0
         for k, members in enumerate(ops):
-            reads_glb = any(nd.buf.is_global for i in members for nd in steps[i].inputs)
-            writes_glb = any(steps[i].out.buf.is_global for i in members)
+            reads_glb = any(nd.buf.is_global for i in members for nd in exec_steps[i].inputs)
+            writes_glb = any(exec_steps[i].out.buf.is_global for i in members)
             sync_glb = 0
             if (reads_glb or writes_glb) and glb_written_since:
                 sync_glb = 1
@@ -473,8 +592,11 @@ class PlanCompiler:
             if op_kind[k]:
                 op_words.append(self._encode_gather([steps[i] for i in members]))
                 n_gather += 1
+            elif getattr(exec_steps[members[0]], "fused", False):
+                op_words.append(self._encode_fused(exec_steps[members[0]], sync_glb))
+                n_fused += 1
             else:
-                op_words.append(self._encode_step(steps[members[0]], sync_glb))
+                op_words.append(self._encode_step(exec_steps[members[0]], sync_glb))
 
         total_bytes, total_flops, max_rank = 0, 0, 0
         step_sizes, dag = [], []
@@ -503,7 +625,8 @@ class PlanCompiler:
                     n_gates=len(gates), treewidth=treewidth, max_rank=max_rank,
                     lds_need=lds.peak, glb_need=glb.peak, n_sliced=len(par_index),
                     bytes_step_model=total_bytes, flops=total_flops, n_barriers=barriers,
-                    n_gather=n_gather, step_sizes=step_sizes, steps_words=steps_words, dag=dag)
+                    n_gather=n_gather, n_fused=n_fused, step_sizes=step_sizes, steps_words=steps_words,
+                    dag=dag)
 
     def _gather_ok(self, st, lanes, lim):
         return self.gather and st.out.n <= lim and st.gatherable(lanes, lim)
@@ -624,6 +747,45 @@ class PlanCompiler:
                     if waiting[c] == 0:
                         ready.append(c)
         return ops, op_kind
+
+    @staticmethod
+    def _encode_fused(fs, sync_glb):
+        """Fused step record: header, per input (loc, mask over union bits),
+        then the offset tables: LO[o_lo][j] for the low output bits and
+        X[x][j] for every assignment of the summed variables."""
+        U = fs.union
+        rank_u = len(U)
+        pos = {v: rank_u - 1 - a for a, v in enumerate(U)}
+        out_set = set(fs.out.dev)
+        omask = 0
+        for v in fs.out.dev:
+            omask |= 1 << pos[v]
+        smask = 0
+        for v in U:
+            if v not in out_set:
+                smask |= 1 << pos[v]
+        k = rank_u - len(out_set)
+        rank_out = len(out_set)
+        lo_bits = min(8, rank_out)
+        n_in = len(fs.inputs)
+        masks = []
+        for nd in fs.inputs:
+            m = 0
+            for v in nd.dev:
+                m |= 1 << pos[v]
+            masks.append(m)
+        words = [0, rank_u | (rank_out << 8) | (k << 16) | (lo_bits << 24), _i32(fs.out.buf.loc), 0,
+                 _i32(omask), _i32(smask), 0, 0]
+        for nd, m in zip(fs.inputs, masks):
+            words += [_i32(nd.buf.loc), _i32(m), 0, 0]
+        for o in range(1 << lo_bits):
+            u = _pdep(o, omask)
+            words += [_pext(u, m) for m in masks]
+        for x in range(1 << k):
+            u = _pdep(x, smask)
+            words += [_pext(u, m) for m in masks]
+        words[0] = OP_FUSED | n_in | (sync_glb << 11) | (len(words) << 16)
+        return words
 
     @staticmethod
     def _encode_step(st, sync_glb):

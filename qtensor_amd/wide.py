@@ -77,9 +77,11 @@ class WidePlan:
         self.n_slices = 1 << n_slice_vars
 
 
-def compile_wide(circuit, order=None, par_ids=(), lanes=256):
+def compile_wide(circuit, order=None, par_ids=(), lanes=256, fuse=True):
     """Compile ``circuit`` with elimination ``order`` (variable ids of the line
-    graph; greedy when None) and slice variables ``par_ids`` into a wide plan."""
+    graph; greedy when None) and slice variables ``par_ids`` into a wide plan.
+    With ``fuse``, chains of steps are contracted in one pass where the time
+    model favours it (``plan.fuse_steps``)."""
     tn = QtreeTensorNet.from_qtree_gates(circuit)
     graph = tn.get_line_graph()
     if order is None:
@@ -91,7 +93,7 @@ def compile_wide(circuit, order=None, par_ids=(), lanes=256):
     tw = max(elimination_widths(g, rest)) if rest else 0
     peo = list(tn.ket_vars) + list(tn.bra_vars) + _vars(graph, rest) + _vars(graph, par_ids)
     buckets, perm = reorder_buckets(tn.buckets, peo)
-    pc = PlanCompiler(lds_tensor_limit=0, gather=False, gate_space="global", lanes=lanes)
+    pc = PlanCompiler(lds_tensor_limit=0, gather=False, gate_space="global", lanes=lanes, fuse=fuse)
     plan = pc.compile(buckets, tn.data_dict, [perm[v] for v in tn.ket_vars],
                       [perm[v] for v in tn.bra_vars], [perm[v] for v in _vars(graph, par_ids)], tw)
     return WidePlan(plan, tw, len(par_ids))

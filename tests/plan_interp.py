@@ -112,7 +112,48 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
         w0 = w[pos] & 0xFFFFFFFF
         length = w0 >> 16
         assert pos + length <= soff + ops_words
-        if w0 & P.OP_GATHER:
+        if w0 & P.OP_FUSED:
+            n_in = w0 & 15
+            meta = w[pos + 1] & 0xFFFFFFFF
+            rank_u, rank_out, k, lo_bits = meta & 255, (meta >> 8) & 255, (meta >> 16) & 255, meta >> 24
+            out_loc = w[pos + 2] & 0xFFFFFFFF
+            omask, smask = w[pos + 4] & 0xFFFFFFFF, w[pos + 5] & 0xFFFFFFFF
+            assert bin(omask).count("1") == rank_out and bin(smask).count("1") == k
+            assert omask & smask == 0 and bin(omask | smask).count("1") == rank_u
+            assert 1 <= n_in <= P.FUSED_MAX_IN and lo_bits == min(8, rank_out)
+            ins = [(w[pos + P.FUSED_HDR + 4 * j] & 0xFFFFFFFF, w[pos + P.FUSED_HDR + 4 * j + 1] & 0xFFFFFFFF)
+                   for j in range(n_in)]
+            tab = pos + P.FUSED_HDR + 4 * n_in
+            lo_tab = np.array(w[tab: tab + (n_in << lo_bits)], dtype=np.int64).reshape(-1, n_in)
+            x_tab = np.array(w[tab + (n_in << lo_bits): tab + (n_in << lo_bits) + (n_in << k)],
+                             dtype=np.int64).reshape(-1, n_in)
+            assert pos + length == tab + (n_in << lo_bits) + (n_in << k)
+            o = np.arange(1 << rank_out, dtype=np.int64)
+            x = np.arange(1 << k, dtype=np.int64)
+            # the device splits o into its low lo_bits (table) and the rest
+            # (computed); both are checked against the direct pext/pdep form
+            uo = np.array([P._pdep(int(v), omask) for v in o], dtype=np.int64)
+            ux = np.array([P._pdep(int(v), smask) for v in x], dtype=np.int64)
+            acc = np.zeros(1 << rank_out, dtype=dtype)
+            offs_o, offs_x = [], []
+            for j, (loc, m) in enumerate(ins):
+                ao = pext_vec(uo, m)
+                bx = pext_vec(ux, m)
+                assert (lo_tab[:, j] == ao[:1 << lo_bits]).all() and (x_tab[:, j] == bx).all()
+                hi = pext_vec(np.array([P._pdep(int(v) << lo_bits, omask) for v in (o >> lo_bits)],
+                                       dtype=np.int64), m)
+                assert (hi + lo_tab[o & ((1 << lo_bits) - 1), j] == ao).all()
+                offs_o.append(ao)
+                offs_x.append(bx)
+            for xi in range(1 << k):
+                prod = np.ones(1 << rank_out, dtype=dtype)
+                for j, (loc, m) in enumerate(ins):
+                    buf = glb if loc & P.GLOBAL else lds
+                    prod = prod * buf[(loc & ~P.GLOBAL) + offs_o[j] + offs_x[j][xi]]
+                acc = acc + prod
+            buf = glb if out_loc & P.GLOBAL else lds
+            buf[(out_loc & ~P.GLOBAL) + o] = acc
+        elif w0 & P.OP_GATHER:
             n_steps, n_act, desc_off, ni = w0 & 0x1F, w[pos + 1], w[pos + 2], w[pos + 3]
             assert n_act <= lanes and 1 <= n_steps <= P.GATHER_MAX_STEPS and desc_off % 4 == 0
             assert 1 <= ni <= P.MAX_IN

@@ -24,17 +24,67 @@ def _edge_circuit(n=14, seed=2, p=2):
     return G, e, g, b, comp.circuit
 
 
+@pytest.mark.parametrize("fuse", [False, True], ids=["steps", "fused"])
 @pytest.mark.parametrize("dtype,tol", [("complex64", 1e-5), ("complex128", 1e-11)])
-def test_wide_greedy_and_forced_sliced(gpu, dtype, tol):
+def test_wide_greedy_and_forced_sliced(gpu, dtype, tol, fuse):
     G, e, g, b, circ = _edge_circuit()
     ref = statevector.zz_expectations(G, g, b)[e]
     eng = WideEngine(dtype=dtype)
-    assert abs(eng.run(compile_wide(circ), g + b) - ref) <= tol
+    assert abs(eng.run(compile_wide(circ, fuse=fuse), g + b) - ref) <= tol
     graph = QtreeTensorNet.from_qtree_gates(circ).get_line_graph()
     order = forced_width_order(graph, target_tw=14, temp=3.0, seed=1)
-    wp = compile_wide(circ, order=order, par_ids=slice_vars_for_order(graph, order, 3))
+    wp = compile_wide(circ, order=order, par_ids=slice_vars_for_order(graph, order, 3), fuse=fuse)
     assert wp.n_slices == 8
     assert abs(eng.run(wp, g + b) - ref) <= tol
+
+
+def test_fused_large_rank_matches_steps(gpu):
+    """A p=3 lightcone under a wide order: fused plans (outputs of rank >= 14,
+    several blocks per output, high output bits computed on the device) give
+    the step-by-step value in complex128."""
+    G = nx.random_regular_graph(3, 64, seed=3)
+    rng = np.random.default_rng(3)
+    g, b = list(rng.uniform(0, np.pi, 3)), list(rng.uniform(0, np.pi / 2, 3))
+    comp = QtreeQAOAComposer(G, gamma=g, beta=b)
+    comp.energy_expectation_lightcone(list(G.edges())[0])
+    graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+    order = forced_width_order(graph, target_tw=20, temp=2.0, seed=0)
+    fused = compile_wide(comp.circuit, order=order, fuse=True)
+    plain = compile_wide(comp.circuit, order=order, fuse=False)
+    assert fused.plan.n_fused > 0 and fused.plan.n_ops < plain.plan.n_ops
+    eng = WideEngine(dtype="complex128")
+    assert abs(eng.run(fused, g + b) - eng.run(plain, g + b)) < 1e-12
+
+
+def test_wide_rejects_inconsistent_fused_table(gpu):
+    """The host recomputes every fused offset table from the masks before any
+    launch: a plan whose table disagrees is refused."""
+    import torch
+    from qtensor_amd import plan as P
+    G, e, g, b, circ = _edge_circuit()
+    wp = compile_wide(circ, fuse=True)
+    words = np.ascontiguousarray(wp.plan.words, dtype=np.int32).copy()
+    pos, tampered = int(words[3]), False
+    for _ in range(int(words[1])):
+        w0 = int(words[pos]) & 0xFFFFFFFF
+        if w0 & P.OP_FUSED:
+            n_in = w0 & 15
+            words[pos + P.FUSED_HDR + 4 * n_in] += 1          # first output-table entry
+            tampered = True
+            break
+        pos += w0 >> 16
+    assert tampered
+    lib = _lib.init(0)
+    dev = torch.from_numpy(words).cuda()
+    arena = torch.empty(max(wp.plan.glb_need, 1), dtype=torch.complex64, device="cuda")
+    prm = torch.tensor(g + b, dtype=torch.float64, device="cuda")
+    out = torch.zeros(2, dtype=torch.float64, device="cuda")
+    d = _lib.WideDesc()
+    d.plan_host, d.plan_words, d.plan_dev = words.ctypes.data, words.size, dev.data_ptr()
+    d.params, d.arena, d.arena_elems = prm.data_ptr(), arena.data_ptr(), arena.numel()
+    d.slice_bits, d.dtype, d.out = 0, _lib.QB_C64, out.data_ptr()
+    rc = lib.qb_wide_execute(ctypes.byref(d), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == _lib.QB_ERR_INVALID and b"table" in lib.qb_last_error()
 
 
 def test_wide_rejects_arena_overflow(gpu):

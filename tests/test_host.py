@@ -231,10 +231,15 @@ def test_batch_vars_amplitudes_match_single_amplitude():
     assert np.allclose(amps, [sv[k << (nq - 2)] for k in range(4)], atol=1e-10)
 
 
-def test_wide_plan_program_vs_oracle():
-    """Wide plans (all tensors global, step ops only) interpret to the oracle value,
-    sliced or not, with greedy and with a forced wide order."""
-    from qtensor_amd.wide import compile_wide, forced_width_order, slice_vars_for_order
+@pytest.mark.parametrize("fuse", [False, True], ids=["steps", "fused"])
+def test_wide_plan_program_vs_oracle(fuse):
+    """Wide plans (all tensors global, step ops only, or with fused chains)
+    interpret to the oracle value, sliced or not, with greedy and with a forced
+    wide order; the interpreter also re-derives every fused offset table."""
+    import functools
+    from qtensor_amd import wide as W
+    forced_width_order, slice_vars_for_order = W.forced_width_order, W.slice_vars_for_order
+    compile_wide = functools.partial(W.compile_wide, fuse=fuse)
     G = nx.random_regular_graph(3, 14, seed=2)
     g, b = [0.3, 0.7], [0.5, 0.1]
     e = list(G.edges())[1]
@@ -252,6 +257,42 @@ def test_wide_plan_program_vs_oracle():
     assert ws.n_slices == 4
     total = sum(run_plan(ws.plan.words, g + b, slice_bits=s, lanes=256) for s in range(4))
     assert abs(total - ref) < 1e-10
+    if fuse:
+        assert wp.plan.n_fused > 0 and ws.plan.n_fused > 0
+        assert wp.plan.n_ops < compile_wide(comp.circuit, fuse=False).plan.n_ops
+
+
+def test_fuse_steps_limits():
+    """Fused groups respect the kernel's limits and cover every bucket step
+    exactly once."""
+    from qtensor_amd import plan as P
+    from qtensor_amd.wide import compile_wide
+    G = nx.random_regular_graph(3, 30, seed=4)
+    comp = QtreeQAOAComposer(G, gamma=[0.1, 0.2, 0.3], beta=[0.4, 0.5, 0.6])
+    comp.energy_expectation_lightcone(list(G.edges())[0])
+    captured = []
+    orig = P.fuse_steps
+
+    def spy(steps):
+        out = orig(steps)
+        captured.append((steps, out))
+        return out
+    P.fuse_steps = spy
+    try:
+        compile_wide(comp.circuit, fuse=True)
+    finally:
+        P.fuse_steps = orig
+    steps, out = captured[0]
+    covered = []
+    for st in out:
+        if getattr(st, "fused", False):
+            k = len(st.union) - len(st.out.dev)
+            assert len(st.inputs) <= P.FUSED_MAX_IN and k <= P.FUSED_MAX_SUM and len(st.union) <= 30
+            assert len(st.out.dev) >= 14 or k <= 4
+            covered += st.members
+        else:
+            covered.append(steps.index(st))
+    assert sorted(covered) == list(range(len(steps)))
 
 
 def test_autotuner_two_stage_search(monkeypatch):
