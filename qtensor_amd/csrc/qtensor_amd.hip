@@ -523,6 +523,27 @@ __device__ __forceinline__ V load_loc(uint32_t loc, uint32_t off, const QB_LDS V
     return lds[loc + off];
 }
 
+// Global scratch of an executor item through a buffer resource (base = the
+// item's slot, bounds = its size in bytes): out-of-range accesses read zero
+// and drop writes instead of faulting, and buffer instructions stay distinct
+// from LDS accesses, so both kinds of load can sit in one straight-line pass.
+constexpr uint32_t kBufferDword3 = 0x00020000u;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ cf32 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t elem, cf32*) {
+    return __builtin_bit_cast(cf32, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8u, 0, 0));
+}
+__device__ __forceinline__ cf64 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t elem, cf64*) {
+    return __builtin_bit_cast(cf64, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 16u, 0, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t elem, cf32 v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, elem * 8u, 0, 0);
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t elem, cf64 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, elem * 16u, 0, 0);
+}
+
 // Offset of union element o in an input whose mask has at most two runs: one
 // bit-field extract and one shift per run (an absent run has width 0 and adds 0).
 __device__ __forceinline__ uint32_t run_offset(uint32_t o, uint32_t runs) {
@@ -656,12 +677,58 @@ __device__ __forceinline__ void step_lds_pass(uint32_t n_out, uint32_t n_in, uin
     }
 }
 
+// Output blocks of a step whose inputs or output live in global scratch
+// (accessed through the slot's buffer resource `srd`).
+template <typename R, int ITEM_THREADS, int U>
+__device__ __forceinline__ void step_mixed_pass(uint32_t n_out, uint32_t n_in, uint32_t n_sum,
+                                                const uint32_t* loc, const uint32_t* top, const uint32_t* mask,
+                                                const uint32_t* runs, uint32_t out_loc,
+                                                QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd,
+                                                int lane) {
+    using V = typename Cx<R>::V;
+    for (uint32_t base = 0; base < n_out; base += U * ITEM_THREADS) {
+        V p[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t o = base + u * ITEM_THREADS + (uint32_t)lane;
+            V x[kExecMaxIn], y[kExecMaxIn];
+#pragma unroll
+            for (int j = 0; j < kExecMaxIn; ++j) {
+                x[j] = V{1, 0};
+                y[j] = V{1, 0};
+                if ((uint32_t)j < n_in) {
+                    const uint32_t a = (loc[j] & ~kGlobal) + in_offset(o, mask[j], runs[j]);
+                    if (loc[j] & kGlobal) {
+                        x[j] = buf_load(srd, a, (V*)nullptr);
+                        if (n_sum == 2) y[j] = buf_load(srd, a + top[j], (V*)nullptr);
+                    } else {
+                        x[j] = lds[a];
+                        if (n_sum == 2) y[j] = lds[a + top[j]];
+                    }
+                }
+            }
+            p[u] = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
+            if (n_sum == 2) p[u] = cadd(p[u], cmul(cmul(y[0], y[1]), cmul(y[2], y[3])));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t o = base + u * ITEM_THREADS + (uint32_t)lane;
+            if (U > 1 || o < n_out) {
+                if (out_loc & kGlobal)
+                    buf_store(srd, (out_loc & ~kGlobal) + o, p[u]);
+                else
+                    lds[out_loc + o] = p[u];
+            }
+        }
+    }
+}
+
 // One step op: all lanes of the item cover the step's output elements.  Lane l
 // of each wave holds word l of the record; fields reach scalar registers by
 // readlane.
 template <typename R, int ITEM_THREADS>
 __device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
-                                        QB_GLB typename Cx<R>::V* glb, int lane) {
+                                        __amdgpu_buffer_rsrc_t srd, int lane) {
     using V = typename Cx<R>::V;
     const uint32_t out_loc = QB_RL(r, 2);
     const uint32_t n_in = w0 & 15u;
@@ -692,28 +759,12 @@ __device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename 
             step_lds_pass<R, ITEM_THREADS, 1>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, lane);
         return;
     }
-    for (uint32_t base = 0; base < n_out; base += ITEM_THREADS) {
-        const uint32_t o = base + (uint32_t)lane;
-        if (o < n_out) {
-            V acc = {0, 0};
-            for (uint32_t sv = 0; sv < n_sum; ++sv) {
-                V prod = {1, 0};
-#pragma unroll
-                for (int j = 0; j < kExecMaxIn; ++j) {
-                    if ((uint32_t)j < n_in) {
-                        const uint32_t off = in_offset(o, mask[j], runs[j]) + (sv ? top[j] : 0u);
-                        const V v = load_loc(loc[j], off, lds, glb);
-                        prod = j == 0 ? v : cmul(prod, v);
-                    }
-                }
-                acc = sv ? cadd(acc, prod) : prod;
-            }
-            if (out_loc & kGlobal)
-                store_glb(glb, (out_loc & ~kGlobal) + o, acc);
-            else
-                lds[out_loc + o] = acc;
-        }
-    }
+    // global scratch involved: every input is uniformly in LDS or in the
+    // item's scratch slot; reads of a pass are issued before its products
+    if (n_out >= 4u * ITEM_THREADS)
+        step_mixed_pass<R, ITEM_THREADS, 4>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
+    else
+        step_mixed_pass<R, ITEM_THREADS, 1>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
 }
 
 // Synchronisation before an op.  Only LDS data flows between ops unless the op
@@ -746,7 +797,7 @@ __device__ __forceinline__ void op_sync(uint32_t w0) {
 // 0's record word, fetched by the caller.
 template <typename R, int ITEM_THREADS>
 __device__ void run_ops(const QB_GLB int32_t* st0, int32_t n_ops, int32_t r0,
-                        QB_LDS typename Cx<R>::V* lds, QB_GLB typename Cx<R>::V* glb, int lane) {
+                        QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
     if (n_ops <= 0) return;
     uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(r0);
     i32x4 d0, d1, d2;       // loaded only for gather ops, read only by them
@@ -769,7 +820,7 @@ __device__ void run_ops(const QB_GLB int32_t* st0, int32_t n_ops, int32_t r0,
         if (w0 & kOpGather)
             gather_op<R>(r0, d0, d1, d2, lds, lane);
         else
-            step_op<R, ITEM_THREADS>(r0, w0, lds, glb, lane);
+            step_op<R, ITEM_THREADS>(r0, w0, lds, srd, lane);
         w0 = w1;
         r0 = r1;
         d0 = e0;
@@ -793,6 +844,10 @@ __global__ __launch_bounds__(ITEM_THREADS) void exec_kernel(
     const int lane = threadIdx.x;
     QB_LDS V* lds = (QB_LDS V*)(smem_raw);
     QB_GLB V* glb = (QB_GLB V*)(scratch) + (int64_t)blockIdx.x * scratch_elems;
+    // the item's scratch slot as a bounds-checked buffer (qb_execute keeps a
+    // slot below 4 GiB)
+    const __amdgpu_buffer_rsrc_t srd = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)glb, (short)0, (int)(uint32_t)(scratch_elems * (int64_t)sizeof(V)), (int)kBufferDword3);
     const V* pool = reinterpret_cast<const V*>(pool_v);
     (void)lds_elems;
     // the first item of every block is static (item = block index), so the
@@ -832,7 +887,7 @@ __global__ __launch_bounds__(ITEM_THREADS) void exec_kernel(
         for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
             build_gate<R>(gates + (size_t)g * kGateWords, prm, my_angles, kParamRegs, slice_bits, lds, pool);
         group_sync<ITEM_THREADS>();
-        run_ops<R, ITEM_THREADS>(st0, plan[1], r0, lds, glb, lane);
+        run_ops<R, ITEM_THREADS>(st0, plan[1], r0, lds, srd, lane);
         group_sync<ITEM_THREADS>();
         if (lane == 0) {
             const V r = load_loc((uint32_t)plan[4], 0u, lds, glb);
@@ -1276,6 +1331,9 @@ int qb_execute(const qb_exec_desc* d, void* stream) {
     if ((int64_t)grid * groups > d->n_slots && d->scratch_elems > 0)
         return set_error(QB_ERR_INVALID, "scratch holds %d slots, launch needs %d", d->n_slots, grid * groups);
     if (d->scratch_elems > 0 && !d->scratch) return set_error(QB_ERR_INVALID, "scratch is null");
+    if (d->scratch_elems < 0 || d->scratch_elems * (d->dtype == QB_C128 ? 16 : 8) > (int64_t)0xFFFFFFFF)
+        return set_error(QB_ERR_UNSUPPORTED, "scratch slot of %lld elements exceeds 4 GiB",
+                         (long long)d->scratch_elems);
     ExecArgs a;
     a.program = d->program;
     a.items = d->items;
