@@ -62,6 +62,12 @@ class HipBackend:
             self.np_dtype = np.complex64 if self.dtype == "complex64" else np.complex128
             self.qb_dtype = _lib.QB_C64 if self.dtype == "complex64" else _lib.QB_C128
             self.device = torch.device("cuda", self.device_index)
+            raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+            if raw is not None:
+                idx = self.device_index
+                self._stream_handle = lambda: raw(idx)
+            else:
+                self._stream_handle = lambda: torch.cuda.current_stream(self.device).cuda_stream
             self._ready = True
 
     # -- BucketBackend API
@@ -100,12 +106,16 @@ class HipBackend:
         for t in bucket:
             for v in t.indices:
                 union.setdefault(int(v), v)
-        U = [union[k] for k in sorted(union)]
-        big = [v for v in U if v.size == 2]
-        for v in U:
-            if v.size not in (1, 2):
+        keys = sorted(union)
+        U = [union[k] for k in keys]
+        big = []
+        for k, v in zip(keys, U):
+            if v.size == 2:
+                big.append(k)
+            elif v.size != 1:
                 raise ValueError(f"variable {v} has size {v.size}; only sizes 1 and 2 are supported")
-        pos = {int(v): len(big) - 1 - a for a, v in enumerate(big)}
+        nb = len(big)
+        pos = {k: nb - 1 - a for a, k in enumerate(big)}
         rank_u = len(big)
         if rank_u > _lib.QB_MAX_RANK:
             raise ValueError(f"bucket union rank {rank_u} exceeds {_lib.QB_MAX_RANK}")
@@ -126,22 +136,23 @@ class HipBackend:
         for j, t in enumerate(bucket):
             m = 0
             for v in t.indices:
-                if v.size == 2:
-                    m |= 1 << pos[int(v)]
+                k = int(v)
+                if k in pos:
+                    m |= 1 << pos[k]
             desc.masks[j] = m
             data = t.data
-            if not (hasattr(data, "is_cuda") and data.is_cuda):
+            if not isinstance(data, torch.Tensor) or not data.is_cuda:
                 data = torch.as_tensor(np.asarray(data), dtype=self.torch_dtype, device=self.device)
             elif data.dtype != self.torch_dtype:
                 data = data.to(self.torch_dtype)
-            data = data.contiguous()
+            if not data.is_contiguous():
+                data = data.contiguous()
             keep.append(data)
             ptrs[j] = data.data_ptr()
         out_shape = tuple(v.size for v in out_indices)
         out = torch.empty(out_shape, dtype=self.torch_dtype, device=self.device)
-        stream = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(self._lib.qb_step(ctypes.byref(desc), ptrs, ctypes.c_void_p(out.data_ptr()),
-                                     ctypes.c_void_p(stream)))
+                                     ctypes.c_void_p(self._stream_handle())))
         return Tensor(f"E{tag}", out_indices, data=out)
 
     def get_result_data(self, result):
