@@ -794,41 +794,45 @@ __device__ __forceinline__ void op_sync(uint32_t w0) {
 
 // All ops of one plan.  Software pipeline over the records: while op k runs,
 // op k+1's gather descriptors and op k+2's record word are in flight.  r0 is op
-// 0's record word, fetched by the caller.
+// 0's record word, fetched by the caller.  The loop is unrolled twice so the
+// current and the next op's descriptors alternate between two register sets
+// instead of being copied every op.
 template <typename R, int ITEM_THREADS>
 __device__ void run_ops(const QB_GLB int32_t* st0, int32_t n_ops, int32_t r0,
                         QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
     if (n_ops <= 0) return;
-    uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(r0);
-    i32x4 d0, d1, d2;       // loaded only for gather ops, read only by them
-    if (w0 & kOpGather) fetch_desc<ITEM_THREADS>(st0, r0, lane, d0, d1, d2);
-    const QB_GLB int32_t* st1 = st0 + (w0 >> 16);
-    int32_t r1 = 0;
-    if (n_ops > 1) r1 = fetch_rec(st1, lane);
-    for (int32_t k = 0; k < n_ops; ++k) {
-        uint32_t w1 = 0;
-        i32x4 e0, e1, e2;       // loaded only for gather ops, read only by them
-        const QB_GLB int32_t* st2 = st1;
-        int32_t r2 = 0;
-        if (k + 1 < n_ops) {
-            w1 = (uint32_t)__builtin_amdgcn_readfirstlane(r1);
-            if (w1 & kOpGather) fetch_desc<ITEM_THREADS>(st1, r1, lane, e0, e1, e2);
-            st2 = st1 + (w1 >> 16);
-            if (k + 2 < n_ops) r2 = fetch_rec(st2, lane);
-        }
-        op_sync<ITEM_THREADS>(w0);
-        if (w0 & kOpGather)
-            gather_op<R>(r0, d0, d1, d2, lds, lane);
-        else
-            step_op<R, ITEM_THREADS>(r0, w0, lds, srd, lane);
-        w0 = w1;
-        r0 = r1;
-        d0 = e0;
-        d1 = e1;
-        d2 = e2;
-        st1 = st2;
-        r1 = r2;
+    uint32_t wa = (uint32_t)__builtin_amdgcn_readfirstlane(r0), wb = 0;
+    int32_t ra = r0, rb = 0;
+    i32x4 a0, a1, a2, b0, b1, b2;   // descriptors, loaded and read only for gather ops
+    if (wa & kOpGather) fetch_desc<ITEM_THREADS>(st0, ra, lane, a0, a1, a2);
+    const QB_GLB int32_t* st1 = st0 + (wa >> 16);
+    if (n_ops > 1) rb = fetch_rec(st1, lane);
+    // STAGE: run op K described by (WC, RC, C0..C2); prefetch the next op's
+    // descriptors into N0..N2 from its record RN, and the record after it into RC
+#define QB_STAGE(K, WC, RC, C0, C1, C2, WN, RN, N0, N1, N2)                          \
+    {                                                                              \
+        int32_t rnext = 0;                                                         \
+        const QB_GLB int32_t* st2 = st1;                                           \
+        if ((K) + 1 < n_ops) {                                                     \
+            WN = (uint32_t)__builtin_amdgcn_readfirstlane(RN);                     \
+            if (WN & kOpGather) fetch_desc<ITEM_THREADS>(st1, RN, lane, N0, N1, N2); \
+            st2 = st1 + (WN >> 16);                                                \
+            if ((K) + 2 < n_ops) rnext = fetch_rec(st2, lane);                     \
+        }                                                                          \
+        op_sync<ITEM_THREADS>(WC);                                                 \
+        if (WC & kOpGather)                                                        \
+            gather_op<R>(RC, C0, C1, C2, lds, lane);                               \
+        else                                                                       \
+            step_op<R, ITEM_THREADS>(RC, WC, lds, srd, lane);                      \
+        RC = rnext;                                                                \
+        st1 = st2;                                                                 \
     }
+    for (int32_t k = 0; k < n_ops; k += 2) {
+        QB_STAGE(k, wa, ra, a0, a1, a2, wb, rb, b0, b1, b2)
+        if (k + 1 >= n_ops) break;
+        QB_STAGE(k + 1, wb, rb, b0, b1, b2, wa, ra, a0, a1, a2)
+    }
+#undef QB_STAGE
 }
 
 template <typename R, int ITEM_THREADS>
