@@ -300,6 +300,7 @@ __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
 constexpr int kFusedMaxIn = 8;
 constexpr int kFusedHdr = 8;
 constexpr uint32_t kOpFused = 1u << 14;
+constexpr uint32_t kOpReduce = 1u << 13;
 
 struct FusedArgs {
     const char* in[kFusedMaxIn];  // input base addresses in the arena
@@ -359,11 +360,17 @@ __global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a) {
 //     runs = n_runs << 30 | run0 | run1 << 15, run = src | width << 5 | dst << 10;
 //     n_runs == 3 means "more than two runs": the offset is pext(o, mask).
 // Gather op (several small steps, one output element per lane, all in LDS):
-//   w0 = kOpGather | n_steps | len << 16, w1 = active lanes, w2 = descriptor offset,
-//   w3 = largest input count of its steps
-//   w4...: lane -> step table, one byte per lane
-//   per step, 12 words at w2 + 12 s: out_loc, first_lane | sum << 12 | n_in << 16,
-//     then per input (loc | top << 16, runs); unused inputs read element 0 (1+0i)
+//   w0 = kOpGather | n_steps | len << 16, w1 = active lanes n, w2 = lane table
+//   offset, w3 = NI, the largest input count of its steps
+//   lane table: NI + 1 blocks of n words; lane l reads word l of each block:
+//     block 0: the LDS element it writes; block 1 + j: the LDS elements of input
+//     j at the summed variable's 0 (bits 0-15) and 1 (bits 16-31).  A step
+//     without a sum names element 1 (0+0i) as its first input's second element;
+//     unused inputs name element 0 (1+0i).
+// Reduction op (a chain of single-input sums over an LDS tensor's top k
+// variables, one wave):
+//   w0 = kOpReduce | rank_out << 4 | sync_glb << 11 | 4 << 16, w1 = k,
+//   w2 = out_loc, w3 = in_loc; out[o] = sum over x < 2^k of in[o + x 2^rank_out]
 constexpr int kPlanHdr = 12;
 constexpr int kGateWords = 16;
 constexpr int kInWords = 4;
@@ -372,7 +379,6 @@ constexpr int kExecMaxIn = 4;
 constexpr uint32_t kGlobal = 0x80000000u;
 constexpr uint32_t kOpGather = 1u << 15;
 constexpr int kGatherHdr = 4;
-constexpr int kGatherDesc = 12;
 // gate kinds (shared with qtensor_amd/gates.py and qtensor_amd/plan.py)
 enum GateKind : int32_t {
     kConst = 0, kZPhase = 1, kXPhase = 2, kZZ = 3, kYPhase = 4,
@@ -559,14 +565,6 @@ __device__ __forceinline__ uint32_t in_offset(uint32_t o, uint32_t mask, uint32_
 
 #define QB_RL(r, k) ((uint32_t)__builtin_amdgcn_readlane((r), (k)))
 
-template <typename V>
-__device__ __forceinline__ V gather_in(const QB_LDS V* lds, uint32_t locw, uint32_t runs, uint32_t o,
-                                       uint32_t sum, V* y) {
-    const uint32_t a = (locw & 0xffffu) + run_offset(o, runs);
-    *y = lds[a + (sum ? (locw >> 16) : 0u)];
-    return lds[a];
-}
-
 // Op records are read from global memory (they stay cached) ahead of use.
 // Lane l fetches word (l & 63) of the record (the program buffer is padded so
 // the read stays inside it for any op).
@@ -574,43 +572,35 @@ __device__ __forceinline__ int32_t fetch_rec(const QB_GLB int32_t* st, int lane)
     return st[lane & 63];
 }
 
-// The lane's step descriptor of a gather op (lanes past the active ones read
-// step 0).  Byte l of the lane -> step table is record word 4 + l/4: for items of
-// up to 236 lanes that word is already in lane 4 + l/4 of the wave (its record
-// word), fetched by a cross-lane permute; larger items load it.
-template <int ITEM_THREADS>
-__device__ __forceinline__ void fetch_desc(const QB_GLB int32_t* st, int32_t r, int lane,
-                                           i32x4& d0, i32x4& d1, i32x4& d2) {
-    static_assert(sizeof(i32x4) == 16, "descriptor rows are 16 bytes");
-    uint32_t tw;
-    if constexpr (ITEM_THREADS <= 4 * (64 - kGatherHdr))
-        tw = (uint32_t)__builtin_amdgcn_ds_bpermute((kGatherHdr + (lane >> 2)) << 2, r);
-    else
-        tw = (uint32_t)st[kGatherHdr + (lane >> 2)];
-    const uint32_t n_act = QB_RL(r, 1);
-    const uint32_t sidx = (uint32_t)lane < n_act ? (tw >> (8 * (lane & 3))) & 255u : 0u;
-    const QB_GLB i32x4* d = (const QB_GLB i32x4*)(st + QB_RL(r, 2) + kGatherDesc * sidx);
+// A lane's words of a gather op's lane table (lanes past the active ones read
+// the last active lane's words and do not use them).
+struct GatherLane {
+    uint32_t w[kExecMaxIn + 1];
+};
+
+__device__ __forceinline__ void fetch_lane(const QB_GLB int32_t* st, int32_t r, int lane, GatherLane& g) {
+    const uint32_t n = QB_RL(r, 1);
     const uint32_t ni = QB_RL(r, 3);
-    d0 = d[0];
-    if (ni > 1) d1 = d[1];
-    if (ni > 3) d2 = d[2];
+    const uint32_t l = (uint32_t)lane < n ? (uint32_t)lane : n - 1u;
+    const QB_GLB uint32_t* t = (const QB_GLB uint32_t*)st + QB_RL(r, 2) + l;
+    g.w[0] = t[0];
+    g.w[1] = t[n];
+    if (ni > 1) g.w[2] = t[2 * n];
+    if (ni > 2) g.w[3] = t[3 * n];
+    if (ni > 3) g.w[4] = t[4 * n];
 }
 
-// One gather op: lane l computes element o = l - first of its step.  NI is
-// the op's largest input count (record word 3); steps with fewer inputs name
-// element 0 (1+0i) in the unused slots.
+// One gather op: lane l computes its table entry's element,
+// x_0 ... x_{NI-1} + y_0 ... y_{NI-1} (y_0 = 0 for a step without a sum).
 template <typename R, int NI>
-__device__ __forceinline__ void gather_op_n(const i32x4& d0, const i32x4& d1, const i32x4& d2,
-                                            QB_LDS typename Cx<R>::V* lds, int lane) {
+__device__ __forceinline__ void gather_op_n(const GatherLane& g, QB_LDS typename Cx<R>::V* lds) {
     using V = typename Cx<R>::V;
-    const uint32_t meta = (uint32_t)d0.y;
-    const uint32_t o = (uint32_t)lane - (meta & 0xfffu);
-    const uint32_t sum = (meta >> 12) & 1u;
-    V x[4], y[4];
-    x[0] = gather_in(lds, (uint32_t)d0.z, (uint32_t)d0.w, o, sum, &y[0]);
-    if (NI > 1) x[1] = gather_in(lds, (uint32_t)d1.x, (uint32_t)d1.y, o, sum, &y[1]);
-    if (NI > 2) x[2] = gather_in(lds, (uint32_t)d1.z, (uint32_t)d1.w, o, sum, &y[2]);
-    if (NI > 3) x[3] = gather_in(lds, (uint32_t)d2.x, (uint32_t)d2.y, o, sum, &y[3]);
+    V x[NI], y[NI];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        x[j] = lds[g.w[1 + j] & 0xffffu];
+        y[j] = lds[g.w[1 + j] >> 16];
+    }
     V p, q;
     if (NI == 1) {
         p = x[0];
@@ -625,49 +615,90 @@ __device__ __forceinline__ void gather_op_n(const i32x4& d0, const i32x4& d1, co
         p = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
         q = cmul(cmul(y[0], y[1]), cmul(y[2], y[3]));
     }
-    if (sum) p = cadd(p, q);
-    lds[(uint32_t)d0.x + o] = p;
+    lds[g.w[0]] = cadd(p, q);
 }
 
 template <typename R>
-__device__ __forceinline__ void gather_op(int32_t r, const i32x4& d0, const i32x4& d1, const i32x4& d2,
-                                          QB_LDS typename Cx<R>::V* lds, int lane) {
+__device__ __forceinline__ void gather_op(int32_t r, const GatherLane& g, QB_LDS typename Cx<R>::V* lds,
+                                          int lane) {
     if ((uint32_t)lane >= QB_RL(r, 1)) return;
     switch (QB_RL(r, 3)) {
-    case 1: gather_op_n<R, 1>(d0, d1, d2, lds, lane); break;
-    case 2: gather_op_n<R, 2>(d0, d1, d2, lds, lane); break;
-    case 3: gather_op_n<R, 3>(d0, d1, d2, lds, lane); break;
-    default: gather_op_n<R, 4>(d0, d1, d2, lds, lane); break;
+    case 1: gather_op_n<R, 1>(g, lds); break;
+    case 2: gather_op_n<R, 2>(g, lds); break;
+    case 3: gather_op_n<R, 3>(g, lds); break;
+    default: gather_op_n<R, 4>(g, lds); break;
     }
 }
 
-// Output blocks [base, base + U*ITEM_THREADS) of an all-LDS step, for every
-// base below n_out in steps of U*ITEM_THREADS (n_out is a multiple of that
-// when U > 1).
-template <typename R, int ITEM_THREADS, int U>
-__device__ __forceinline__ void step_lds_pass(uint32_t n_out, uint32_t n_in, uint32_t n_sum,
-                                              const uint32_t* loc, const uint32_t* top, const uint32_t* mask,
-                                              const uint32_t* runs, uint32_t out_loc,
-                                              QB_LDS typename Cx<R>::V* lds, int lane) {
+// Input offsets of a step split at the item's lane bits: output element
+// o = base + lane with base a multiple of ITEM_THREADS (a power of two) and
+// lane < ITEM_THREADS share no bit, so pext(o, mask) = pext(lane, mask) |
+// pext(base, mask).  The lane part is computed once per step; the base part is
+// uniform and is computed by the scalar unit once per block of outputs.
+
+// One reduction op, run by the item's first wave (n_out = 2^rank_out <= 64):
+// lane l takes output o = l mod n_out and every (64 / n_out)-th term from
+// l / n_out on; the lanes sharing an output then combine in a butterfly.
+template <typename R>
+__device__ __forceinline__ void reduce_op(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds, int lane) {
     using V = typename Cx<R>::V;
+    if (lane >= 64) return;
+    const uint32_t ro = (w0 >> 4) & 31u;
+    const uint32_t terms = 1u << (QB_RL(r, 1) & 31u);
+    const uint32_t out_loc = QB_RL(r, 2);
+    const uint32_t in_loc = QB_RL(r, 3);
+    const uint32_t n_out = 1u << ro;
+    const uint32_t o = (uint32_t)lane & (n_out - 1u);
+    const uint32_t g = (uint32_t)lane >> ro;
+    const uint32_t step = ro < 6u ? 64u >> ro : 1u;   // plans keep rank_out <= 6
+    V s = V{0, 0};
+#pragma unroll 4
+    for (uint32_t x = g; x < terms; x += step) s = cadd(s, lds[in_loc + o + x * n_out]);
+    for (uint32_t b = n_out; b < 64u; b <<= 1) {
+        s.x += __shfl_xor(s.x, (int)b, 64);
+        s.y += __shfl_xor(s.y, (int)b, 64);
+    }
+    if (g == 0) lds[out_loc + o] = s;
+}
+
+// Output blocks [base, base + U*ITEM_THREADS) of an all-LDS step with NI inputs,
+// for every base below n_out in steps of U*ITEM_THREADS (n_out is a multiple of
+// that when U > 1).
+template <typename R, int ITEM_THREADS, int U, int NI>
+__device__ __forceinline__ void step_lds_pass(uint32_t n_out, uint32_t n_sum, const uint32_t* loc,
+                                              const uint32_t* top, const uint32_t* mask, const uint32_t* runs,
+                                              uint32_t out_loc, QB_LDS typename Cx<R>::V* lds, int lane) {
+    using V = typename Cx<R>::V;
+    uint32_t la[NI];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) la[j] = loc[j] + in_offset((uint32_t)lane, mask[j], runs[j]);
     for (uint32_t base = 0; base < n_out; base += U * ITEM_THREADS) {
         V p[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t o = base + u * ITEM_THREADS + (uint32_t)lane;
-            uint32_t ad[kExecMaxIn];
+            const uint32_t hb = base + u * ITEM_THREADS;
+            V x[NI], y[NI];
 #pragma unroll
-            for (int j = 0; j < kExecMaxIn; ++j)
-                ad[j] = (uint32_t)j < n_in ? loc[j] + in_offset(o, mask[j], runs[j]) : 0u;
-            V x[kExecMaxIn], y[kExecMaxIn];
-#pragma unroll
-            for (int j = 0; j < kExecMaxIn; ++j) x[j] = lds[ad[j]];
-            if (n_sum == 2) {
-#pragma unroll
-                for (int j = 0; j < kExecMaxIn; ++j) y[j] = lds[ad[j] + top[j]];
+            for (int j = 0; j < NI; ++j) {
+                const uint32_t a = la[j] + in_offset(hb, mask[j], runs[j]);
+                x[j] = lds[a];
+                if (n_sum == 2) y[j] = lds[a + top[j]];
             }
-            p[u] = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
-            if (n_sum == 2) p[u] = cadd(p[u], cmul(cmul(y[0], y[1]), cmul(y[2], y[3])));
+            V q;
+            if (NI == 1) {
+                p[u] = x[0];
+                if (n_sum == 2) q = y[0];
+            } else if (NI == 2) {
+                p[u] = cmul(x[0], x[1]);
+                if (n_sum == 2) q = cmul(y[0], y[1]);
+            } else if (NI == 3) {
+                p[u] = cmul(cmul(x[0], x[1]), x[2]);
+                if (n_sum == 2) q = cmul(cmul(y[0], y[1]), y[2]);
+            } else {
+                p[u] = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
+                if (n_sum == 2) q = cmul(cmul(y[0], y[1]), cmul(y[2], y[3]));
+            }
+            if (n_sum == 2) p[u] = cadd(p[u], q);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -675,6 +706,18 @@ __device__ __forceinline__ void step_lds_pass(uint32_t n_out, uint32_t n_in, uin
             if (U > 1 || o < n_out) lds[out_loc + o] = p[u];
         }
     }
+}
+
+template <typename R, int ITEM_THREADS, int NI>
+__device__ __forceinline__ void step_lds_n(uint32_t n_out, uint32_t n_sum, const uint32_t* loc,
+                                           const uint32_t* top, const uint32_t* mask, const uint32_t* runs,
+                                           uint32_t out_loc, QB_LDS typename Cx<R>::V* lds, int lane) {
+    if (NI < 4 && n_out >= 4u * ITEM_THREADS)
+        step_lds_pass<R, ITEM_THREADS, 4, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
+    else if (n_out >= 2u * ITEM_THREADS)
+        step_lds_pass<R, ITEM_THREADS, 2, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
+    else
+        step_lds_pass<R, ITEM_THREADS, 1, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
 }
 
 // Output blocks of a step whose inputs or output live in global scratch
@@ -686,18 +729,22 @@ __device__ __forceinline__ void step_mixed_pass(uint32_t n_out, uint32_t n_in, u
                                                 QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd,
                                                 int lane) {
     using V = typename Cx<R>::V;
+    uint32_t la[kExecMaxIn];
+#pragma unroll
+    for (int j = 0; j < kExecMaxIn; ++j)
+        la[j] = (uint32_t)j < n_in ? (loc[j] & ~kGlobal) + in_offset((uint32_t)lane, mask[j], runs[j]) : 0u;
     for (uint32_t base = 0; base < n_out; base += U * ITEM_THREADS) {
         V p[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t o = base + u * ITEM_THREADS + (uint32_t)lane;
+            const uint32_t hb = base + u * ITEM_THREADS;
             V x[kExecMaxIn], y[kExecMaxIn];
 #pragma unroll
             for (int j = 0; j < kExecMaxIn; ++j) {
                 x[j] = V{1, 0};
                 y[j] = V{1, 0};
                 if ((uint32_t)j < n_in) {
-                    const uint32_t a = (loc[j] & ~kGlobal) + in_offset(o, mask[j], runs[j]);
+                    const uint32_t a = la[j] + in_offset(hb, mask[j], runs[j]);
                     if (loc[j] & kGlobal) {
                         x[j] = buf_load(srd, a, (V*)nullptr);
                         if (n_sum == 2) y[j] = buf_load(srd, a + top[j], (V*)nullptr);
@@ -750,13 +797,13 @@ __device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename 
     if (w0 & (1u << 12)) {
         // all-LDS: up to four blocks of ITEM_THREADS outputs per pass, every read
         // of a pass issued before its first product and its stores (outputs
-        // never alias inputs); unused input slots read element 0 (1+0i)
-        if (n_out >= 4u * ITEM_THREADS)
-            step_lds_pass<R, ITEM_THREADS, 4>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, lane);
-        else if (n_out >= 2u * ITEM_THREADS)
-            step_lds_pass<R, ITEM_THREADS, 2>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, lane);
-        else
-            step_lds_pass<R, ITEM_THREADS, 1>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, lane);
+        // never alias inputs); specialised by input count
+        switch (n_in) {
+        case 1: step_lds_n<R, ITEM_THREADS, 1>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+        case 2: step_lds_n<R, ITEM_THREADS, 2>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+        case 3: step_lds_n<R, ITEM_THREADS, 3>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+        default: step_lds_n<R, ITEM_THREADS, 4>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+        }
         return;
     }
     // global scratch involved: every input is uniformly in LDS or in the
@@ -793,50 +840,53 @@ __device__ __forceinline__ void op_sync(uint32_t w0) {
 }
 
 // All ops of one plan.  Software pipeline over the records: while op k runs,
-// op k+1's gather descriptors and op k+2's record word are in flight.  r0 is op
-// 0's record word, fetched by the caller.  The loop is unrolled twice so the
-// current and the next op's descriptors alternate between two register sets
-// instead of being copied every op.
+// op k+1's lane table words (gather ops) and op k+2's record word are in
+// flight.  r0 is op 0's record word, fetched by the caller.  One copy of the
+// op bodies keeps the kernel inside the instruction cache.
 template <typename R, int ITEM_THREADS>
-__device__ void run_ops(const QB_GLB int32_t* st0, int32_t n_ops, int32_t r0,
-                        QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
+__device__ __forceinline__ void run_ops(const QB_GLB int32_t* st0, int32_t n_ops, int32_t r0,
+                                        QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
     if (n_ops <= 0) return;
-    uint32_t wa = (uint32_t)__builtin_amdgcn_readfirstlane(r0), wb = 0;
-    int32_t ra = r0, rb = 0;
-    i32x4 a0, a1, a2, b0, b1, b2;   // descriptors, loaded and read only for gather ops
-    if (wa & kOpGather) fetch_desc<ITEM_THREADS>(st0, ra, lane, a0, a1, a2);
-    const QB_GLB int32_t* st1 = st0 + (wa >> 16);
-    if (n_ops > 1) rb = fetch_rec(st1, lane);
-    // STAGE: run op K described by (WC, RC, C0..C2); prefetch the next op's
-    // descriptors into N0..N2 from its record RN, and the record after it into RC
-#define QB_STAGE(K, WC, RC, C0, C1, C2, WN, RN, N0, N1, N2)                          \
-    {                                                                              \
-        int32_t rnext = 0;                                                         \
-        const QB_GLB int32_t* st2 = st1;                                           \
-        if ((K) + 1 < n_ops) {                                                     \
-            WN = (uint32_t)__builtin_amdgcn_readfirstlane(RN);                     \
-            if (WN & kOpGather) fetch_desc<ITEM_THREADS>(st1, RN, lane, N0, N1, N2); \
-            st2 = st1 + (WN >> 16);                                                \
-            if ((K) + 2 < n_ops) rnext = fetch_rec(st2, lane);                     \
-        }                                                                          \
-        op_sync<ITEM_THREADS>(WC);                                                 \
-        if (WC & kOpGather)                                                        \
-            gather_op<R>(RC, C0, C1, C2, lds, lane);                               \
-        else                                                                       \
-            step_op<R, ITEM_THREADS>(RC, WC, lds, srd, lane);                      \
-        RC = rnext;                                                                \
-        st1 = st2;                                                                 \
+    uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(r0);
+    GatherLane g0;   // loaded and read only for gather ops
+    if (w0 & kOpGather) fetch_lane(st0, r0, lane, g0);
+    const QB_GLB int32_t* st1 = st0 + (w0 >> 16);
+    int32_t r1 = 0;
+    if (n_ops > 1) r1 = fetch_rec(st1, lane);
+    for (int32_t k = 0; k < n_ops; ++k) {
+        uint32_t w1 = 0;
+        GatherLane g1;
+        const QB_GLB int32_t* st2 = st1;
+        int32_t r2 = 0;
+        if (k + 1 < n_ops) {
+            w1 = (uint32_t)__builtin_amdgcn_readfirstlane(r1);
+            if (w1 & kOpGather) fetch_lane(st1, r1, lane, g1);
+            st2 = st1 + (w1 >> 16);
+            if (k + 2 < n_ops) r2 = fetch_rec(st2, lane);
+        }
+        op_sync<ITEM_THREADS>(w0);
+        if (w0 & kOpGather)
+            gather_op<R>(r0, g0, lds, lane);
+        else if (w0 & kOpReduce)
+            reduce_op<R>(r0, w0, lds, lane);
+        else
+            step_op<R, ITEM_THREADS>(r0, w0, lds, srd, lane);
+        w0 = w1;
+        r0 = r1;
+        g0 = g1;
+        st1 = st2;
+        r1 = r2;
     }
-    for (int32_t k = 0; k < n_ops; k += 2) {
-        QB_STAGE(k, wa, ra, a0, a1, a2, wb, rb, b0, b1, b2)
-        if (k + 1 >= n_ops) break;
-        QB_STAGE(k + 1, wb, rb, b0, b1, b2, wa, ra, a0, a1, a2)
-    }
-#undef QB_STAGE
 }
 
+// complex64 items keep at most 96 VGPRs, so five waves fit on each SIMD: with
+// 128-thread items and 16 KiB of LDS each, that is the ten blocks per CU the
+// LDS allows.
+template <typename R>
+constexpr int kExecWavesPerEU = sizeof(R) == 4 ? 5 : 1;
+
 template <typename R, int ITEM_THREADS>
-__global__ __launch_bounds__(ITEM_THREADS) void exec_kernel(
+__global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
     const int32_t* __restrict__ program, const int32_t* __restrict__ items, int32_t n_items,
     int32_t lds_elems, const double* __restrict__ params, const void* __restrict__ pool_v,
     void* __restrict__ scratch, int64_t scratch_elems, double* __restrict__ item_out,
@@ -1393,7 +1443,7 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
     for (int32_t k = 0; k < n_ops; ++k) {
         if (pos + kStepHdr > d->plan_words) return set_error(QB_ERR_INVALID, "op %d outside the plan", k);
         const uint32_t w0 = (uint32_t)w[pos];
-        if (w0 & kOpGather) return set_error(QB_ERR_INVALID, "wide plans take step ops only");
+        if (w0 & (kOpGather | kOpReduce)) return set_error(QB_ERR_INVALID, "wide plans take step ops only");
         if (w0 & kOpFused) {
             const int rc = launch_fused(w, d->plan_words, d->plan_dev, pos, k, arena, elem, cap, d->dtype, st);
             if (rc) return rc;

@@ -48,8 +48,8 @@ GATE_WORDS = 16
 IN_WORDS = 4
 STEP_HDR = 3
 MAX_IN = 4
-# zero words after the last plan of a program: the executor fetches each op's
-# record word (lane & 63) and lane -> step byte (up to 4 + 1024/4 words) speculatively
+# zero words after the last plan of a program: the executor fetches words 0-63
+# of each op's record (one per lane) whatever the record's length
 PROGRAM_PAD = 320
 # zero angles after the last item's: the executor loads 8 angles per item at once
 PARAM_PAD = 8
@@ -58,12 +58,15 @@ GLOBAL = 0x80000000
 OP_GATHER = 1 << 15
 # fused step (wide plans): several bucket steps contracted in one pass
 OP_FUSED = 1 << 14
+# reduction op (item plans): a chain of single-input sums in one op, one wave
+OP_REDUCE = 1 << 13
+REDUCE_HDR = 4
+REDUCE_MAX_OUT = 64
 FUSED_MAX_IN = 8
 FUSED_MAX_SUM = 8
 FUSED_HDR = 8
 FUSED_IN_WORDS = 4
 GATHER_HDR = 4
-GATHER_DESC = 12
 GATHER_MAX_STEPS = 16
 # LDS element offsets of the constants every plan reserves ahead of its gates
 LOC_ONE, LOC_ZERO, LOC_SINK = 0, 1, 2
@@ -151,9 +154,7 @@ class SymStep:
     def gatherable(self, lanes, lds_limit):
         if self.out.n > lanes or len(self.inputs) > MAX_IN:
             return False
-        if any(nd.n > lds_limit and not nd.buf.leaf for nd in self.inputs):
-            return False
-        return all(len(mask_runs(m)) <= MAX_RUNS for m in self.lows)
+        return not any(nd.n > lds_limit and not nd.buf.leaf for nd in self.inputs)
 
 
 class FusedStep:
@@ -174,6 +175,83 @@ class FusedStep:
 
     def gatherable(self, lanes, lds_limit):
         return False
+
+
+class ReduceStep:
+    """A chain of single-input steps, each summing the top (most significant)
+    variable of the previous one's output, contracted in one op:
+    out[o] = sum over x < 2**k of in[o + x * n_out], for the chain's k summed
+    variables.  Run by one wave (n_out <= REDUCE_MAX_OUT)."""
+
+    def __init__(self, inp, out, k, rank_out, members):
+        self.inputs = [inp]
+        self.out = out
+        self.k = k
+        self.rank_out = rank_out
+        self.members = members
+
+    def gatherable(self, lanes, lds_limit):
+        return False
+
+
+def _simple_sum(st):
+    """True for a single-input step summing its input's top variable with the
+    other variables kept in order (out[o] = in[o] + in[o + n_out])."""
+    if len(st.inputs) != 1 or not st.sum_flag:
+        return False
+    n_out = 1 << st.rank_out
+    return st.lows[0] == n_out - 1 and st.tops[0] == n_out
+
+
+def reduce_chains(steps, result_step, lds_limit):
+    """Replace chains of at least two simple sums whose intermediates have no
+    other reader by ReduceSteps.  Returns the new step list, with every
+    node's producer index renumbered into it."""
+    n = len(steps)
+    readers = [0] * n
+    for st in steps:
+        for nd in st.inputs:
+            if nd.step >= 0:
+                readers[nd.step] += 1
+    nxt = [-1] * n           # the simple-sum step reading step i's output alone
+    for i, st in enumerate(steps):
+        if _simple_sum(st):
+            j = st.inputs[0].step
+            if j >= 0 and readers[j] == 1 and j != result_step and _simple_sum(steps[j]):
+                nxt[j] = i
+    has_prev = [False] * n
+    for j in range(n):
+        if nxt[j] >= 0:
+            has_prev[nxt[j]] = True
+    replaced = {}
+    for i in range(n):
+        if has_prev[i] or nxt[i] < 0 or not _simple_sum(steps[i]):
+            continue
+        chain = [i]
+        while nxt[chain[-1]] >= 0:
+            chain.append(nxt[chain[-1]])
+        first, last = steps[chain[0]], steps[chain[-1]]
+        if (1 << last.rank_out) > REDUCE_MAX_OUT or first.inputs[0].n > lds_limit:
+            continue
+        replaced[chain[-1]] = ReduceStep(first.inputs[0], last.out, len(chain), last.rank_out, chain)
+        for c in chain[:-1]:
+            replaced[c] = None
+    if not replaced:
+        return steps
+    new, index = [], {}
+    for i, st in enumerate(steps):
+        r = replaced.get(i, st)
+        if r is None:
+            continue
+        index[i] = len(new)
+        new.append(r)
+    seen = set()
+    for st in new:
+        for nd in list(st.inputs) + [st.out]:
+            if id(nd) not in seen and nd.step >= 0:
+                seen.add(id(nd))
+                nd.step = index[nd.step]
+    return new
 
 
 def _pdep(x, mask):
@@ -275,6 +353,7 @@ class Plan:
     n_barriers: int
     n_gather: int = 0
     n_fused: int = 0
+    n_reduce: int = 0
     step_sizes: list = field(default_factory=list)
     steps_words: int = 0
     dag: list = field(default_factory=list)   # per step: (producer steps of inputs, n_out, n_in, sum)
@@ -385,7 +464,7 @@ class PlanCompiler:
     """
 
     def __init__(self, lds_tensor_limit=1 << 12, lds_capacity=None, elem_bytes=8, lanes=64,
-                 gather=True, window=None, gate_space="lds", schedule="list", fuse=False):
+                 gather=True, window=None, gate_space="lds", schedule="list", fuse=False, reduce=True):
         self.lds_tensor_limit = lds_tensor_limit
         self.lds_capacity = lds_capacity      # elements available per item, or None
         self.elem_bytes = elem_bytes
@@ -399,6 +478,8 @@ class PlanCompiler:
         self.schedule = schedule
         # wide plans: contract chains of steps in one pass (see fuse_steps)
         self.fuse = fuse
+        # item plans: chains of single-input sums become one reduction op each
+        self.reduce = reduce
         # "global": gates and every intermediate live in global memory (wide plans)
         if gate_space not in ("lds", "global"):
             raise ValueError("gate_space is 'lds' or 'global'")
@@ -520,8 +601,22 @@ class PlanCompiler:
         if result.dev:
             raise ValueError("network result is not a scalar")
 
+        # step-model figures of the bucket steps (before any fusion renumbers them)
+        total_bytes, total_flops, max_rank = 0, 0, 0
+        step_sizes, dag = [], []
+        for st in steps:
+            n_out = st.out.n
+            rank_u = st.rank_out + st.sum_flag
+            max_rank = max(max_rank, rank_u)
+            total_bytes += 8 * (sum(nd.n for nd in st.inputs) + n_out)
+            n_union = 1 << rank_u
+            total_flops += 6 * (len(st.inputs) - 1) * n_union + 2 * (n_union - n_out)
+            step_sizes.append(rank_u)
+            dag.append(([nd.step for nd in st.inputs], n_out, len(st.inputs), st.sum_flag))
+
         # phase 2: schedule steps into ops.  Wide plans run one grid-wide launch
-        # per op in bucket order, optionally with fused step chains.
+        # per op in bucket order, optionally with fused step chains; item plans
+        # contract chains of single-input sums in one reduction op each.
         lanes = self.lanes
         lim = self.lds_tensor_limit
         exec_steps = steps
@@ -529,12 +624,15 @@ class PlanCompiler:
             if self.fuse:
                 exec_steps = fuse_steps(steps)
             ops, op_kind = [[i] for i in range(len(exec_steps))], [False] * len(exec_steps)
-        elif self.schedule == "first_fit":
-            ops, op_kind = self._schedule_first_fit(steps, lanes, lim)
-        elif self.schedule == "list":
-            ops, op_kind = self._schedule_list(steps, lanes, lim)
         else:
-            ops, op_kind = self._schedule_levels(steps, lanes, lim, alap=self.schedule == "alap")
+            if self.reduce:
+                exec_steps = reduce_chains(steps, result.step, lim)
+            if self.schedule == "first_fit":
+                ops, op_kind = self._schedule_first_fit(exec_steps, lanes, lim)
+            elif self.schedule == "list":
+                ops, op_kind = self._schedule_list(exec_steps, lanes, lim)
+            else:
+                ops, op_kind = self._schedule_levels(exec_steps, lanes, lim, alap=self.schedule == "alap")
         op_of = [0] * len(exec_steps)
         for k, members in enumerate(ops):
             for i in members:
@@ -579,6 +677,7 @@ class PlanCompiler:
         barriers = 0
         n_gather = 0
         n_fused = 0
+        n_reduce = 0
         for k, members in enumerate(ops):
             reads_glb = any(nd.buf.is_global for i in members for nd in exec_steps[i].inputs)
             writes_glb = any(exec_steps[i].out.buf.is_global for i in members)
@@ -590,25 +689,16 @@ class PlanCompiler:
             if writes_glb:
                 glb_written_since.add(k)
             if op_kind[k]:
-                op_words.append(self._encode_gather([steps[i] for i in members]))
+                op_words.append(self._encode_gather([exec_steps[i] for i in members]))
                 n_gather += 1
+            elif isinstance(exec_steps[members[0]], ReduceStep):
+                op_words.append(self._encode_reduce(exec_steps[members[0]], sync_glb))
+                n_reduce += 1
             elif getattr(exec_steps[members[0]], "fused", False):
                 op_words.append(self._encode_fused(exec_steps[members[0]], sync_glb))
                 n_fused += 1
             else:
                 op_words.append(self._encode_step(exec_steps[members[0]], sync_glb))
-
-        total_bytes, total_flops, max_rank = 0, 0, 0
-        step_sizes, dag = [], []
-        for st in steps:
-            n_out = st.out.n
-            rank_u = st.rank_out + st.sum_flag
-            max_rank = max(max_rank, rank_u)
-            total_bytes += 8 * (sum(nd.n for nd in st.inputs) + n_out)
-            n_union = 1 << rank_u
-            total_flops += 6 * (len(st.inputs) - 1) * n_union + 2 * (n_union - n_out)
-            step_sizes.append(rank_u)
-            dag.append(([nd.step for nd in st.inputs], n_out, len(st.inputs), st.sum_flag))
 
         n_ops = len(op_words)
         gates_off = PLAN_HDR
@@ -625,7 +715,8 @@ class PlanCompiler:
                     n_gates=len(gates), treewidth=treewidth, max_rank=max_rank,
                     lds_need=lds.peak, glb_need=glb.peak, n_sliced=len(par_index),
                     bytes_step_model=total_bytes, flops=total_flops, n_barriers=barriers,
-                    n_gather=n_gather, n_fused=n_fused, step_sizes=step_sizes, steps_words=steps_words,
+                    n_gather=n_gather, n_fused=n_fused, n_reduce=n_reduce, step_sizes=step_sizes,
+                    steps_words=steps_words,
                     dag=dag)
 
     def _gather_ok(self, st, lanes, lim):
@@ -788,6 +879,14 @@ class PlanCompiler:
         return words
 
     @staticmethod
+    def _encode_reduce(rs, sync_glb):
+        """Reduction op: w0 = OP_REDUCE | rank_out << 4 | sync << 11 | len << 16,
+        then k (summed variables), the output and the input locations."""
+        assert not rs.out.buf.is_global and not rs.inputs[0].buf.is_global
+        w0 = OP_REDUCE | (rs.rank_out << 4) | (sync_glb << 11) | (REDUCE_HDR << 16)
+        return [w0, rs.k, _i32(rs.out.buf.loc), _i32(rs.inputs[0].buf.loc)]
+
+    @staticmethod
     def _encode_step(st, sync_glb):
         rec_len = STEP_HDR + IN_WORDS * len(st.inputs)
         all_lds = not st.out.buf.is_global and not any(nd.buf.is_global for nd in st.inputs)
@@ -799,38 +898,36 @@ class PlanCompiler:
         return words
 
     def _encode_gather(self, members):
-        """Gather op: header, lane -> step table, one 12-word descriptor per step."""
-        lanes = self.lanes
-        table_words = -(-lanes // 4)
-        n = len(members)
-        first = []
-        lane_step = [255] * (table_words * 4)
-        pos = 0
-        for s_idx, st in enumerate(members):
-            first.append(pos)
+        """Gather op: a 4-word header, then a lane table of NI + 1 blocks of
+        ``n_act`` words (NI = the members' largest input count).  Lane l of the
+        op computes one output element of one member: block 0 holds its output
+        element, block 1 + j the LDS elements of input j at the summed
+        variable's 0 (low half) and 1 (high half).  A member without a sum reads
+        element LOC_ZERO as its first input's high half, so its second product
+        is zero; missing inputs read LOC_ONE (1 + 0i)."""
+        ni = max(len(st.inputs) for st in members)
+        cols = [[] for _ in range(ni + 1)]
+        for st in members:
+            n_in = len(st.inputs)
             for o in range(st.out.n):
-                lane_step[pos + o] = s_idx
-            pos += st.out.n
-        hdr_len = GATHER_HDR + table_words
-        desc_off = -(-hdr_len // 4) * 4      # 16-byte aligned descriptors
-        length = desc_off + GATHER_DESC * n
-        words = [OP_GATHER | n | (length << 16), pos, desc_off, max(len(st.inputs) for st in members)]
-        for t in range(table_words):
-            b = lane_step[4 * t: 4 * t + 4]
-            words.append(_i32(b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24)))
-        words += [0] * (desc_off - len(words))
-        for s_idx, st in enumerate(members):
-            assert first[s_idx] < (1 << 12)
-            d = [_i32(st.out.buf.loc), first[s_idx] | (st.sum_flag << 12) | (len(st.inputs) << 16)]
-            for j in range(MAX_IN):
-                if j < len(st.inputs):
-                    loc, top, low = st.inputs[j].buf.loc, st.tops[j], st.lows[j]
-                    assert 0 <= loc < (1 << 16) and 0 <= top < (1 << 16)
-                    d += [_i32(loc | (top << 16)), _i32(_runs_word(mask_runs(low)))]
-                else:
-                    d += [_i32(LOC_ONE), 0]
-            d += [0, 0]
-            words += d
+                cols[0].append(_i32(st.out.buf.loc + o))
+                for j in range(ni):
+                    if j < n_in:
+                        x = st.inputs[j].buf.loc + _pext(o, st.lows[j])
+                        if st.sum_flag:
+                            y = x + st.tops[j]
+                        else:
+                            y = LOC_ZERO if j == 0 else LOC_ONE
+                    else:
+                        x = y = LOC_ONE
+                    assert 0 <= x < (1 << 16) and 0 <= y < (1 << 16)
+                    cols[1 + j].append(_i32(x | (y << 16)))
+        n_act = len(cols[0])
+        assert 1 <= n_act <= self.lanes
+        length = GATHER_HDR + (ni + 1) * n_act
+        words = [OP_GATHER | len(members) | (length << 16), n_act, GATHER_HDR, ni]
+        for c in cols:
+            words += c
         return words
 
     def _leaf(self, t, data_dict, fixed, par_index, dst):

@@ -153,33 +153,27 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
                 acc = acc + prod
             buf = glb if out_loc & P.GLOBAL else lds
             buf[(out_loc & ~P.GLOBAL) + o] = acc
+        elif w0 & P.OP_REDUCE:
+            rank_out, k = (w0 >> 4) & 31, w[pos + 1]
+            out_loc, in_loc = w[pos + 2], w[pos + 3]
+            assert length == P.REDUCE_HDR and (1 << rank_out) <= P.REDUCE_MAX_OUT and k >= 1
+            n_out = 1 << rank_out
+            block = lds[in_loc: in_loc + (n_out << k)].reshape(1 << k, n_out)
+            lds[out_loc: out_loc + n_out] = block.sum(axis=0)
         elif w0 & P.OP_GATHER:
-            n_steps, n_act, desc_off, ni = w0 & 0x1F, w[pos + 1], w[pos + 2], w[pos + 3]
-            assert n_act <= lanes and 1 <= n_steps <= P.GATHER_MAX_STEPS and desc_off % 4 == 0
-            assert 1 <= ni <= P.MAX_IN
-            table = []
-            for t in range(-(-lanes // 4)):
-                word = w[pos + P.GATHER_HDR + t] & 0xFFFFFFFF
-                table += [(word >> (8 * b)) & 255 for b in range(4)]
+            n_steps, n_act, tab, ni = w0 & 0x7FFF, w[pos + 1], w[pos + 2], w[pos + 3]
+            assert 1 <= n_steps <= n_act <= lanes and tab == P.GATHER_HDR
+            assert 1 <= ni <= P.MAX_IN and length == P.GATHER_HDR + (ni + 1) * n_act
+            cols = [[x & 0xFFFFFFFF for x in w[pos + tab + c * n_act: pos + tab + (c + 1) * n_act]]
+                    for c in range(ni + 1)]
             results = []
             for lane in range(n_act):
-                sidx = table[lane]
-                assert sidx < n_steps
-                d = [x & 0xFFFFFFFF for x in w[pos + desc_off + P.GATHER_DESC * sidx:
-                                              pos + desc_off + P.GATHER_DESC * (sidx + 1)]]
-                out_loc, meta = d[0], d[1]
-                o = lane - (meta & 0xfff)
-                n_in, sm = (meta >> 16) & 15, (meta >> 12) & 1
-                assert 1 <= n_in <= P.MAX_IN and o >= 0
                 x, y = 1 + 0j, 1 + 0j
-                for j in range(ni):           # the device reads the op's first ni inputs
-                    locw, rw = d[2 + 2 * j], d[3 + 2 * j]
-                    a = (locw & 0xFFFF) + int(_runs_offset(o, rw, 0)[0])
-                    if j >= n_in:
-                        assert (locw & 0xFFFF) == P.LOC_ONE and rw == 0
-                    x *= lds[a]
-                    y *= lds[a + ((locw >> 16) if sm else 0)]
-                results.append((out_loc + o, x + y if sm else x))
+                for j in range(ni):
+                    e = cols[1 + j][lane]
+                    x *= lds[e & 0xFFFF]
+                    y *= lds[e >> 16]
+                results.append((cols[0][lane], x + y))
             for addr, v in results:        # all reads of an op precede its writes
                 lds[addr] = v
         else:
