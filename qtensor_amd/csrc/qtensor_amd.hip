@@ -770,12 +770,40 @@ __device__ __forceinline__ void step_mixed_pass(uint32_t n_out, uint32_t n_in, u
     }
 }
 
+// A step op with global scratch: every input is uniformly in LDS or in the
+// item's scratch slot; reads of a pass are issued before its products.  Out of
+// line, so that its scalar registers do not crowd the op loop (all-LDS steps
+// and gather ops do not pay for it).
+template <typename R, int ITEM_THREADS>
+__device__ __attribute__((noinline)) void step_mixed(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
+                                                     __amdgpu_buffer_rsrc_t srd, int lane) {
+    const uint32_t out_loc = QB_RL(r, 2);
+    const uint32_t n_in = w0 & 15u;
+    const uint32_t n_out = 1u << ((w0 >> 4) & 31u);
+    const uint32_t n_sum = ((w0 >> 9) & 1u) + 1u;
+    uint32_t loc[kExecMaxIn], top[kExecMaxIn], mask[kExecMaxIn], runs[kExecMaxIn];
+#pragma unroll
+    for (int j = 0; j < kExecMaxIn; ++j) {
+        if ((uint32_t)j < n_in) {
+            loc[j] = QB_RL(r, 3 + 4 * j);
+            top[j] = QB_RL(r, 4 + 4 * j);
+            mask[j] = QB_RL(r, 5 + 4 * j);
+            runs[j] = QB_RL(r, 6 + 4 * j);
+        } else {
+            loc[j] = 0u; top[j] = 0u; mask[j] = 0u; runs[j] = 0u;
+        }
+    }
+    if (n_out >= 4u * ITEM_THREADS)
+        step_mixed_pass<R, ITEM_THREADS, 4>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
+    else
+        step_mixed_pass<R, ITEM_THREADS, 1>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
+}
+
 // One step op: all lanes of the item cover the step's output elements.  Lane l
 // of each wave holds word l of the record; fields reach scalar registers by
 // readlane.
 template <typename R, int ITEM_THREADS>
-__device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
-                                        __amdgpu_buffer_rsrc_t srd, int lane) {
+__device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds, int lane) {
     using V = typename Cx<R>::V;
     const uint32_t out_loc = QB_RL(r, 2);
     const uint32_t n_in = w0 & 15u;
@@ -794,24 +822,15 @@ __device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename 
         }
     }
     const uint32_t n_out = 1u << rank_out;
-    if (w0 & (1u << 12)) {
-        // all-LDS: up to four blocks of ITEM_THREADS outputs per pass, every read
-        // of a pass issued before its first product and its stores (outputs
-        // never alias inputs); specialised by input count
-        switch (n_in) {
-        case 1: step_lds_n<R, ITEM_THREADS, 1>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
-        case 2: step_lds_n<R, ITEM_THREADS, 2>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
-        case 3: step_lds_n<R, ITEM_THREADS, 3>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
-        default: step_lds_n<R, ITEM_THREADS, 4>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
-        }
-        return;
+    // all-LDS: up to four blocks of ITEM_THREADS outputs per pass, every read
+    // of a pass issued before its first product and its stores (outputs never
+    // alias inputs); specialised by input count
+    switch (n_in) {
+    case 1: step_lds_n<R, ITEM_THREADS, 1>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+    case 2: step_lds_n<R, ITEM_THREADS, 2>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+    case 3: step_lds_n<R, ITEM_THREADS, 3>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+    default: step_lds_n<R, ITEM_THREADS, 4>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
     }
-    // global scratch involved: every input is uniformly in LDS or in the
-    // item's scratch slot; reads of a pass are issued before its products
-    if (n_out >= 4u * ITEM_THREADS)
-        step_mixed_pass<R, ITEM_THREADS, 4>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
-    else
-        step_mixed_pass<R, ITEM_THREADS, 1>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
 }
 
 // Synchronisation before an op.  Only LDS data flows between ops unless the op
@@ -869,8 +888,10 @@ __device__ __forceinline__ void run_ops(const QB_GLB int32_t* st0, int32_t n_ops
             gather_op<R>(r0, g0, lds, lane);
         else if (w0 & kOpReduce)
             reduce_op<R>(r0, w0, lds, lane);
+        else if (w0 & (1u << 12))
+            step_op<R, ITEM_THREADS>(r0, w0, lds, lane);
         else
-            step_op<R, ITEM_THREADS>(r0, w0, lds, srd, lane);
+            step_mixed<R, ITEM_THREADS>(r0, w0, lds, srd, lane);
         w0 = w1;
         r0 = r1;
         g0 = g1;
@@ -924,8 +945,12 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
         const QB_GLB int32_t* st0 = (const QB_GLB int32_t*)(plan + plan[3]);
         const int32_t r0 = fetch_rec(st0, lane);
         if (lane == 0) {
-            lds[0] = V{1, 0};
-            lds[1] = V{0, 0};
+            // materialised per item: a copy held across the item loop would be
+            // spilled under the register cap
+            R one = 1, zero = 0;
+            asm volatile("" : "+v"(one), "+v"(zero));
+            lds[0] = V{one, zero};
+            lds[1] = V{zero, zero};
         }
         double th[kParamRegs];
 #pragma unroll
