@@ -117,7 +117,9 @@ typedef struct qb_exec_desc {
     void* scratch;               /* device: n_slots * scratch_elems elements */
     int64_t scratch_elems;       /* elements per slot (>= every plan's global need) */
     int32_t n_slots;             /* slots available in scratch */
-    int32_t reserved;
+    int32_t program_words;       /* words of program, at most 2^30: op records and
+                                    lane tables are read through a buffer of this
+                                    size (reads past it return 0); 0 = unbounded */
     double* item_out;            /* device: 2 * n_items doubles */
     uint32_t* queue;             /* device: one uint32 (reset by the call when the
                                     items outnumber the resident blocks) */

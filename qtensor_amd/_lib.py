@@ -39,7 +39,7 @@ class ExecDesc(ctypes.Structure):
                 ("lds_elems", ctypes.c_int32), ("dtype", ctypes.c_int32),
                 ("params", ctypes.c_void_p), ("const_pool", ctypes.c_void_p),
                 ("scratch", ctypes.c_void_p), ("scratch_elems", ctypes.c_int64),
-                ("n_slots", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("n_slots", ctypes.c_int32), ("program_words", ctypes.c_int32),
                 ("item_out", ctypes.c_void_p), ("queue", ctypes.c_void_p)]
 
 

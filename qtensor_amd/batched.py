@@ -360,6 +360,7 @@ class Batch:
         d.scratch = self.scratch.data_ptr()
         d.scratch_elems = self.scratch_elems
         d.n_slots = self.scratch_slots if self.scratch_elems else self.n_slots
+        d.program_words = self.program.numel()
         d.item_out = self.item_out.data_ptr()
         d.queue = self.queue.data_ptr()
         return d

@@ -489,6 +489,7 @@ __device__ __forceinline__ void build_gate(const int32_t* g, const double* param
 
 struct ExecArgs {
     const int32_t* program;
+    uint32_t program_bytes;
     const int32_t* items;
     int32_t n_items;
     int32_t lds_elems;
@@ -565,11 +566,12 @@ __device__ __forceinline__ uint32_t in_offset(uint32_t o, uint32_t mask, uint32_
 
 #define QB_RL(r, k) ((uint32_t)__builtin_amdgcn_readlane((r), (k)))
 
-// Op records are read from global memory (they stay cached) ahead of use.
-// Lane l fetches word (l & 63) of the record (the program buffer is padded so
-// the read stays inside it for any op).
-__device__ __forceinline__ int32_t fetch_rec(const QB_GLB int32_t* st, int lane) {
-    return st[lane & 63];
+// Op records are read from global memory (they stay cached) ahead of use,
+// through a buffer resource over the program (`prog`): offsets are 32-bit and
+// reads past the program return 0.  `st` is an op's word offset in the program.
+// Lane l fetches word (l & 63) of the record.
+__device__ __forceinline__ int32_t fetch_rec(__amdgpu_buffer_rsrc_t prog, uint32_t st, int lane) {
+    return __builtin_amdgcn_raw_buffer_load_b32(prog, (uint32_t)(lane & 63) * 4u, st * 4u, 0);
 }
 
 // A lane's words of a gather op's lane table (lanes past the active ones read
@@ -578,16 +580,18 @@ struct GatherLane {
     uint32_t w[kExecMaxIn + 1];
 };
 
-__device__ __forceinline__ void fetch_lane(const QB_GLB int32_t* st, int32_t r, int lane, GatherLane& g) {
+// Every block is read whatever the op's input count (blocks past the table
+// belong to later records or lie past the program, where reads return 0);
+// the gather op uses blocks 0..NI only.
+__device__ __forceinline__ void fetch_lane(__amdgpu_buffer_rsrc_t prog, uint32_t st, int32_t r, int lane,
+                                           GatherLane& g) {
     const uint32_t n = QB_RL(r, 1);
-    const uint32_t ni = QB_RL(r, 3);
     const uint32_t l = (uint32_t)lane < n ? (uint32_t)lane : n - 1u;
-    const QB_GLB uint32_t* t = (const QB_GLB uint32_t*)st + QB_RL(r, 2) + l;
-    g.w[0] = t[0];
-    g.w[1] = t[n];
-    if (ni > 1) g.w[2] = t[2 * n];
-    if (ni > 2) g.w[3] = t[3 * n];
-    if (ni > 3) g.w[4] = t[4 * n];
+    const uint32_t vo = l * 4u;
+    const uint32_t so = (st + QB_RL(r, 2)) * 4u;
+#pragma unroll
+    for (int j = 0; j <= kExecMaxIn; ++j)
+        g.w[j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(prog, vo, so + (uint32_t)j * n * 4u, 0);
 }
 
 // One gather op: lane l computes its table entry's element,
@@ -863,25 +867,25 @@ __device__ __forceinline__ void op_sync(uint32_t w0) {
 // flight.  r0 is op 0's record word, fetched by the caller.  One copy of the
 // op bodies keeps the kernel inside the instruction cache.
 template <typename R, int ITEM_THREADS>
-__device__ __forceinline__ void run_ops(const QB_GLB int32_t* st0, int32_t n_ops, int32_t r0,
+__device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st0, int32_t n_ops, int32_t r0,
                                         QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
     if (n_ops <= 0) return;
     uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(r0);
     GatherLane g0;   // loaded and read only for gather ops
-    if (w0 & kOpGather) fetch_lane(st0, r0, lane, g0);
-    const QB_GLB int32_t* st1 = st0 + (w0 >> 16);
+    if (w0 & kOpGather) fetch_lane(prog, st0, r0, lane, g0);
+    uint32_t st1 = st0 + (w0 >> 16);
     int32_t r1 = 0;
-    if (n_ops > 1) r1 = fetch_rec(st1, lane);
+    if (n_ops > 1) r1 = fetch_rec(prog, st1, lane);
     for (int32_t k = 0; k < n_ops; ++k) {
         uint32_t w1 = 0;
         GatherLane g1;
-        const QB_GLB int32_t* st2 = st1;
+        uint32_t st2 = st1;
         int32_t r2 = 0;
         if (k + 1 < n_ops) {
             w1 = (uint32_t)__builtin_amdgcn_readfirstlane(r1);
-            if (w1 & kOpGather) fetch_lane(st1, r1, lane, g1);
+            if (w1 & kOpGather) fetch_lane(prog, st1, r1, lane, g1);
             st2 = st1 + (w1 >> 16);
-            if (k + 2 < n_ops) r2 = fetch_rec(st2, lane);
+            if (k + 2 < n_ops) r2 = fetch_rec(prog, st2, lane);
         }
         op_sync<ITEM_THREADS>(w0);
         if (w0 & kOpGather)
@@ -908,8 +912,8 @@ constexpr int kExecWavesPerEU = sizeof(R) == 4 ? 5 : 1;
 
 template <typename R, int ITEM_THREADS>
 __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
-    const int32_t* __restrict__ program, const int32_t* __restrict__ items, int32_t n_items,
-    int32_t lds_elems, const double* __restrict__ params, const void* __restrict__ pool_v,
+    const int32_t* __restrict__ program, uint32_t program_bytes, const int32_t* __restrict__ items,
+    int32_t n_items, int32_t lds_elems, const double* __restrict__ params, const void* __restrict__ pool_v,
     void* __restrict__ scratch, int64_t scratch_elems, double* __restrict__ item_out,
     uint32_t* __restrict__ queue) {
     using V = typename Cx<R>::V;
@@ -923,6 +927,8 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
     // slot below 4 GiB)
     const __amdgpu_buffer_rsrc_t srd = __builtin_amdgcn_make_buffer_rsrc(
         (void*)glb, (short)0, (int)(uint32_t)(scratch_elems * (int64_t)sizeof(V)), (int)kBufferDword3);
+    const __amdgpu_buffer_rsrc_t prog = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)program, (short)0, (int)program_bytes, (int)kBufferDword3);
     const V* pool = reinterpret_cast<const V*>(pool_v);
     (void)lds_elems;
     // the first item of every block is static (item = block index), so the
@@ -942,8 +948,8 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
         const int32_t n_gates = plan[0];
         const int32_t* __restrict__ gates = plan + plan[2];
         // op 0's record is in flight while the gates are built
-        const QB_GLB int32_t* st0 = (const QB_GLB int32_t*)(plan + plan[3]);
-        const int32_t r0 = fetch_rec(st0, lane);
+        const uint32_t st0 = (uint32_t)(it[0] + plan[3]);
+        const int32_t r0 = fetch_rec(prog, st0, lane);
         if (lane == 0) {
             // materialised per item: a copy held across the item loop would be
             // spilled under the register cap
@@ -966,7 +972,7 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
         for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
             build_gate<R>(gates + (size_t)g * kGateWords, prm, my_angles, kParamRegs, slice_bits, lds, pool);
         group_sync<ITEM_THREADS>();
-        run_ops<R, ITEM_THREADS>(st0, plan[1], r0, lds, srd, lane);
+        run_ops<R, ITEM_THREADS>(prog, st0, plan[1], r0, lds, srd, lane);
         group_sync<ITEM_THREADS>();
         if (lane == 0) {
             const V r = load_loc((uint32_t)plan[4], 0u, lds, glb);
@@ -1072,7 +1078,7 @@ int exec_occupancy_dispatch(int item_threads, int lds_bytes, int* blocks) {
 
 template <typename R, int T>
 int exec_launch(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
-    hipLaunchKernelGGL((exec_kernel<R, T>), dim3(grid), dim3(T), lds_bytes, st, a.program, a.items,
+    hipLaunchKernelGGL((exec_kernel<R, T>), dim3(grid), dim3(T), lds_bytes, st, a.program, a.program_bytes, a.items,
                        a.n_items, a.lds_elems, a.params, a.pool, a.scratch, a.scratch_elems, a.item_out,
                        a.queue);
     QB_HIP(hipGetLastError());
@@ -1413,8 +1419,11 @@ int qb_execute(const qb_exec_desc* d, void* stream) {
     if (d->scratch_elems < 0 || d->scratch_elems * (d->dtype == QB_C128 ? 16 : 8) > (int64_t)0xFFFFFFFF)
         return set_error(QB_ERR_UNSUPPORTED, "scratch slot of %lld elements exceeds 4 GiB",
                          (long long)d->scratch_elems);
+    if (d->program_words < 0 || d->program_words > (1 << 30))
+        return set_error(QB_ERR_INVALID, "program_words %d outside 0..2^30", d->program_words);
     ExecArgs a;
     a.program = d->program;
+    a.program_bytes = d->program_words ? 4u * (uint32_t)d->program_words : 0xFFFFFFFFu;
     a.items = d->items;
     a.n_items = d->n_items;
     a.lds_elems = d->lds_elems;
@@ -1555,6 +1564,7 @@ struct qb_plan {
     int32_t n_params = 0;
     int32_t n_slots = 0;
     int64_t scratch_elems = 0;
+    int64_t program_words = 0;
     int32_t* program = nullptr;
     int32_t* items = nullptr;
     int32_t* seg = nullptr;
@@ -1586,6 +1596,8 @@ int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
         d->n_params < 0 || d->scratch_elems < 0)
         return set_error(QB_ERR_INVALID, "empty or negative program, items, segments or sizes");
     if (d->n_terms > (1 << 20)) return set_error(QB_ERR_INVALID, "n_terms=%d > 2^20", d->n_terms);
+    if (d->program_words > (1 << 30))
+        return set_error(QB_ERR_INVALID, "program_words=%lld > 2^30", (long long)d->program_words);
     // host-side checks of what the kernels will index
     for (int32_t t = 0; t < d->n_terms; ++t)
         if (d->seg[t] > d->seg[t + 1]) return set_error(QB_ERR_INVALID, "segments must not decrease");
@@ -1612,6 +1624,7 @@ int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
     p->n_params = d->n_params;
     p->n_slots = n_slots < d->n_items ? n_slots : d->n_items;
     p->scratch_elems = d->scratch_elems;
+    p->program_words = d->program_words;
     const size_t elem = d->dtype == QB_C128 ? 16 : 8;
     hipError_t e = hipSuccess;
 #define QB_PLAN_TRY(call)                                                          \
@@ -1657,7 +1670,7 @@ int qb_plan_execute(qb_plan* p, const double* params, double* term_out, double* 
     d.scratch = p->scratch;
     d.scratch_elems = p->scratch_elems;
     d.n_slots = p->n_slots;
-    d.reserved = 0;
+    d.program_words = (int32_t)p->program_words;
     d.item_out = p->item_out;
     d.queue = p->queue;
     int rc = qb_execute(&d, stream);

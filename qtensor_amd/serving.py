@@ -389,6 +389,7 @@ class EnergyServer:
             d.scratch = d_scr.data_ptr()
             d.scratch_elems = glb_need
             d.n_slots = used_slots if glb_need else slots_c.value
+            d.program_words = self.store.buf.numel()
             d.item_out = d_out.data_ptr()
             d.queue = d_queue.data_ptr()
             h = ctypes.c_void_p(st.cuda_stream)
