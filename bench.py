@@ -27,10 +27,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per executor launch on the default workload, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE runs (profiles/exec_kernel_qaoa_p3_n256.md): 6341.2 KiB
-# fetched x 2.0 (a 1 GiB device copy reads as 524306.6 KiB) + 78.6 KiB written
+# FETCH_SIZE / WRITE_SIZE runs (profiles/exec_kernel_qaoa_p3_n256.md): 18219.2 KiB
+# fetched x 2.0 (a 1 GiB device copy reads as 524306.4 KiB) + 10202.7 KiB written
 # x 1.0 (the copy's 1 GiB writes read as 1048576 KiB).
-EXEC_TRAFFIC_BYTES = int(6341.2 * 1024 * 2.0 + 78.6 * 1024 * 1.0)
+EXEC_TRAFFIC_BYTES = int(18219.2 * 1024 * 2.0 + 10202.7 * 1024 * 1.0)
 
 
 def workload(name):

@@ -141,3 +141,35 @@ def test_plan_handle_matches_batch(gpu, shape, max_tw):
         assert abs(z - sv[e]) <= _tol(sv[e])
     h.close()
     h.close()
+
+
+@pytest.mark.parametrize("n_seg,max_len", [(1, 1), (384, 12), (1500, 3), (3000, 40), (9000, 2)])
+def test_reduce_terms_vs_fsum(gpu, n_seg, max_len):
+    """qb_reduce_terms (compensated FP64 sums of item results per term, then of
+    the terms) against math.fsum, for segment counts from 1 to 9000 and
+    segments of up to 40 items, with values spanning ten decades."""
+    import ctypes
+    import math
+
+    import torch
+
+    from qtensor_amd import _lib
+    lib = _lib.init(0)
+    rng = np.random.default_rng(n_seg)
+    lens = rng.integers(1, max_len + 1, n_seg)
+    seg = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    vals = rng.standard_normal((int(seg[-1]), 2)) * 10.0 ** rng.integers(-8, 3, (int(seg[-1]), 1))
+    d_items = torch.tensor(vals, dtype=torch.float64, device=gpu)
+    d_seg = torch.tensor(seg, dtype=torch.int32, device=gpu)
+    d_terms = torch.zeros((n_seg, 2), dtype=torch.float64, device=gpu)
+    d_total = torch.zeros(2, dtype=torch.float64, device=gpu)
+    _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(d_items.data_ptr()), ctypes.c_void_p(d_seg.data_ptr()),
+                                   n_seg, ctypes.c_void_p(d_terms.data_ptr()),
+                                   ctypes.c_void_p(d_total.data_ptr()), None))
+    torch.cuda.synchronize()
+    terms = d_terms.cpu().numpy()
+    for c in range(2):
+        ref = np.array([math.fsum(vals[seg[s]:seg[s + 1], c]) for s in range(n_seg)])
+        assert np.max(np.abs(terms[:, c] - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+        tot = math.fsum(vals[:, c])
+        assert abs(d_total[c].item() - tot) <= 1e-12 * max(1.0, np.sum(np.abs(vals[:, c])))
