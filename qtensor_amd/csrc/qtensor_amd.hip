@@ -910,6 +910,14 @@ __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st
 template <typename R>
 constexpr int kExecWavesPerEU = sizeof(R) == 4 ? 5 : 1;
 
+// LDS of an executor block, all dynamic so that the data region starts at
+// address 0 (an element's byte address is then its index times the element
+// size, one shift): lds_elems data elements, then each wave's copy of the
+// item's first kParamRegs angles, then the queue ticket.
+__host__ __device__ constexpr int exec_aux_bytes(int item_threads) {
+    return ((item_threads + 63) / 64) * kParamRegs * 8 + 16;
+}
+
 template <typename R, int ITEM_THREADS>
 __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
     const int32_t* __restrict__ program, uint32_t program_bytes, const int32_t* __restrict__ items,
@@ -918,10 +926,13 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
     uint32_t* __restrict__ queue) {
     using V = typename Cx<R>::V;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ int32_t item_sh;
-    __shared__ double angles_sh[(ITEM_THREADS + 63) / 64][kParamRegs];
+    // The kernel declares no static LDS, so its dynamic LDS starts at address
+    // 0 (the launch checks that); element e of the data region is at byte
+    // e * sizeof(V), which the compiler then folds into one shift.
+    QB_LDS V* lds = (QB_LDS V*)(uintptr_t)0u;
+    QB_LDS double* angles_sh = (QB_LDS double*)(smem_raw + (size_t)lds_elems * sizeof(V));
+    QB_LDS int32_t* item_sh = (QB_LDS int32_t*)(angles_sh + ((ITEM_THREADS + 63) / 64) * kParamRegs);
     const int lane = threadIdx.x;
-    QB_LDS V* lds = (QB_LDS V*)(smem_raw);
     QB_GLB V* glb = (QB_GLB V*)(scratch) + (int64_t)blockIdx.x * scratch_elems;
     // the item's scratch slot as a bounds-checked buffer (qb_execute keeps a
     // slot below 4 GiB)
@@ -930,7 +941,6 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
     const __amdgpu_buffer_rsrc_t prog = __builtin_amdgcn_make_buffer_rsrc(
         (void*)program, (short)0, (int)program_bytes, (int)kBufferDword3);
     const V* pool = reinterpret_cast<const V*>(pool_v);
-    (void)lds_elems;
     // the first item of every block is static (item = block index), so the
     // launch does not start with a burst of atomics on one address; later
     // items come from the queue.  A ticket is taken only when an item is done
@@ -961,7 +971,7 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
         double th[kParamRegs];
 #pragma unroll
         for (int q = 0; q < kParamRegs; ++q) th[q] = prm[q];
-        QB_LDS double* my_angles = (QB_LDS double*)&angles_sh[lane >> 6][0];
+        QB_LDS double* my_angles = angles_sh + (lane >> 6) * kParamRegs;
         if ((lane & 63) == 0) {
 #pragma unroll
             for (int q = 0; q < kParamRegs; ++q) my_angles[q] = th[q];
@@ -980,9 +990,9 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
             item_out[2 * (size_t)slot + 1] = (double)r.y;
         }
         if ((int64_t)gridDim.x >= (int64_t)n_items) break;
-        if (lane == 0) item_sh = (int32_t)(gridDim.x + atomicAdd(queue, 1u));
+        if (lane == 0) *item_sh = (int32_t)(gridDim.x + atomicAdd(queue, 1u));
         group_sync<ITEM_THREADS>();
-        item = __builtin_amdgcn_readfirstlane(item_sh);
+        item = __builtin_amdgcn_readfirstlane(*item_sh);
         // item_sh is rewritten only after the next item's barriers
     }
 }
@@ -1078,6 +1088,16 @@ int exec_occupancy_dispatch(int item_threads, int lds_bytes, int* blocks) {
 
 template <typename R, int T>
 int exec_launch(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
+    // the kernel addresses its LDS data from address 0: it must declare no
+    // static LDS (checked once per instantiation)
+    static int static_lds = -1;
+    if (static_lds < 0) {
+        hipFuncAttributes attr;
+        QB_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&exec_kernel<R, T>)));
+        static_lds = (int)attr.sharedSizeBytes;
+    }
+    if (static_lds != 0)
+        return set_error(QB_ERR_UNSUPPORTED, "executor kernel declares %d bytes of static LDS", static_lds);
     hipLaunchKernelGGL((exec_kernel<R, T>), dim3(grid), dim3(T), lds_bytes, st, a.program, a.program_bytes, a.items,
                        a.n_items, a.lds_elems, a.params, a.pool, a.scratch, a.scratch_elems, a.item_out,
                        a.queue);
@@ -1102,7 +1122,7 @@ int slots_for(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n
     if (lds_elems < 0) return set_error(QB_ERR_INVALID, "lds_elems < 0");
     const int elem = dtype == QB_C128 ? 16 : 8;
     const int groups = 1;
-    const int64_t lds_bytes = (int64_t)lds_elems * elem;
+    const int64_t lds_bytes = (int64_t)lds_elems * elem + exec_aux_bytes(item_threads);
     if (lds_bytes > 160 * 1024)
         return set_error(QB_ERR_UNSUPPORTED, "LDS request %lld bytes exceeds 160 KiB", (long long)lds_bytes);
     int blocks = 0;
@@ -1439,7 +1459,7 @@ int qb_execute(const qb_exec_desc* d, void* stream) {
         QB_HIP(hipGetLastError());
     }
     const int elem = d->dtype == QB_C128 ? 16 : 8;
-    const int lds_bytes = d->lds_elems * elem * groups;
+    const int lds_bytes = (d->lds_elems * elem + exec_aux_bytes(d->item_threads)) * groups;
     return d->dtype == QB_C128 ? exec_launch_dispatch<double>(d->item_threads, a, grid, lds_bytes, st)
                                : exec_launch_dispatch<float>(d->item_threads, a, grid, lds_bytes, st);
 }
