@@ -34,6 +34,7 @@ released, so no op writes memory it also reads.
 
 The program format is shared with ``csrc/qtensor_amd.hip`` (constants below).
 """
+import functools
 import struct
 from dataclasses import dataclass, field
 
@@ -274,6 +275,12 @@ def _pext(x, mask):
         bit += 1
         mask &= mask - 1
     return r
+
+
+@functools.lru_cache(maxsize=4096)
+def _pext_table(mask, n):
+    """(_pext(o, mask) for o in range(n)), increasing in o."""
+    return tuple(_pext(o, mask) for o in range(n))
 
 
 def _step_time(in_ranks, out_rank, union_rank, bw=4e12, rate=40e12, launch=4e-6):
@@ -906,28 +913,31 @@ class PlanCompiler:
         element LOC_ZERO as its first input's high half, so its second product
         is zero; missing inputs read LOC_ONE (1 + 0i)."""
         ni = max(len(st.inputs) for st in members)
+        one = LOC_ONE | (LOC_ONE << 16)
         cols = [[] for _ in range(ni + 1)]
         for st in members:
-            n_in = len(st.inputs)
-            for o in range(st.out.n):
-                cols[0].append(_i32(st.out.buf.loc + o))
-                for j in range(ni):
-                    if j < n_in:
-                        x = st.inputs[j].buf.loc + _pext(o, st.lows[j])
-                        if st.sum_flag:
-                            y = x + st.tops[j]
-                        else:
-                            y = LOC_ZERO if j == 0 else LOC_ONE
+            n_in, n = len(st.inputs), st.out.n
+            base = st.out.buf.loc
+            cols[0].extend(range(base, base + n))
+            for j in range(ni):
+                if j < n_in:
+                    loc = st.inputs[j].buf.loc
+                    offs = _pext_table(st.lows[j], n)
+                    assert 0 <= loc and loc + offs[-1] + (st.tops[j] if st.sum_flag else 0) < (1 << 16)
+                    if st.sum_flag:
+                        hi = (loc + st.tops[j]) << 16
+                        cols[1 + j].extend(loc + t + ((t << 16) + hi) for t in offs)
                     else:
-                        x = y = LOC_ONE
-                    assert 0 <= x < (1 << 16) and 0 <= y < (1 << 16)
-                    cols[1 + j].append(_i32(x | (y << 16)))
+                        hi = (LOC_ZERO if j == 0 else LOC_ONE) << 16
+                        cols[1 + j].extend(loc + t + hi for t in offs)
+                else:
+                    cols[1 + j].extend([one] * n)
         n_act = len(cols[0])
         assert 1 <= n_act <= self.lanes
         length = GATHER_HDR + (ni + 1) * n_act
         words = [OP_GATHER | len(members) | (length << 16), n_act, GATHER_HDR, ni]
         for c in cols:
-            words += c
+            words += [_i32(v) for v in c]
         return words
 
     def _leaf(self, t, data_dict, fixed, par_index, dst):
