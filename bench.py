@@ -340,6 +340,8 @@ def main():
                          "1 GiB device copy that calibrates the memory counters")
     ap.add_argument("--slice-tw", type=int, default=None,
                     help="with --shape: slice lightcones whose treewidth reaches this value")
+    ap.add_argument("--slicer", default="greedy", choices=("greedy", "tree_trim"),
+                    help="with --shape: slice-variable rule (max-degree greedy or tree trimming)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -384,16 +386,16 @@ def main():
     t0 = time.perf_counter()
     if args.shape:
         it, cap, lim = (int(x) for x in args.shape.split(","))
-        shape, max_tw, table = LaunchShape(it, cap, lim), args.slice_tw, []
+        shape, max_tw, slicer, table = LaunchShape(it, cap, lim), args.slice_tw, args.slicer, []
     else:
-        tuner = Autotuner(verbose=True)
-        shape, max_tw, table = tuner.tune(G, p, gamma, beta)
+        tuner = Autotuner(verbose=True, workers=16)
+        shape, max_tw, slicer, table = tuner.tune(G, p, gamma, beta)
     tune_s = time.perf_counter() - t0
-    print(f"[bench] tuned: {shape.key()} max_tw={max_tw} ({tune_s:.1f} s)", file=sys.stderr, flush=True)
+    print(f"[bench] tuned: {shape.key()} max_tw={max_tw} {slicer} ({tune_s:.1f} s)", file=sys.stderr, flush=True)
 
     from qtensor_amd.batched import clear_plan_cache
     clear_plan_cache()                  # the autotuner compiled these plans already
-    eng = LightconeEngine(shape=shape, max_tw=max_tw)
+    eng = LightconeEngine(shape=shape, max_tw=max_tw, slicer=slicer)
     t0 = time.perf_counter()
     wl = eng.workload(G, p)
     plan_cold = time.perf_counter() - t0
@@ -444,7 +446,7 @@ def main():
                           "higher_is_better": False, "n_gpus": 1, "dtype": "complex64", "data": "synthetic",
                           "config": {"workload": desc, "item_threads": shape.item_threads,
                                      "lds_capacity": shape.lds_capacity,
-                                     "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw,
+                                     "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw, "slicer": slicer,
                                      "work_items": batch.n_items, "resident_items": batch.n_slots},
                           "step_model_bytes_per_launch": bytes_per_launch,
                           "terms_per_s": round(value, 1), "calibration_copy_bytes": 2 << 30}), flush=True)
@@ -468,7 +470,7 @@ def main():
                    "ordering": "greedy", "form": "batched persistent executor, HIP graph",
                    "max_treewidth": wl_alg.max_treewidth(), "distinct_plans": len(set(map(id, wl.plans.values()))),
                    "item_threads": shape.item_threads, "lds_capacity": shape.lds_capacity,
-                   "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw,
+                   "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw, "slicer": slicer,
                    "work_items": batch.n_items},
         "autotune": {"seconds": round(tune_s, 1), "candidates": len(table)},
         "max_rel_err_vs_fp64": err_fp64,

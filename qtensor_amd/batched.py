@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib
 from .composer import QtreeQAOAComposer
 from .lightcone import canonical_graph, canonical_lightcone
-from .ordering import OrderingOptimizer, SlicesOptimizer
+from .ordering import OrderingOptimizer, SlicesOptimizer, TreeTrimSplitter
 from .plan import PARAM_PAD, PROGRAM_PAD, PlanCompiler
 
 
@@ -116,10 +116,10 @@ def _keys_task(args):
 
 
 def _plan_task(args):
-    composer_name, dtype, shape, max_tw, tw_bias, tasks = args
+    composer_name, dtype, shape, max_tw, tw_bias, slicer, tasks = args
     from . import composer as _composer
     eng = LightconeEngine(composer=getattr(_composer, composer_name), dtype=dtype, shape=shape,
-                          max_tw=max_tw, tw_bias=tw_bias)
+                          max_tw=max_tw, tw_bias=tw_bias, slicer=slicer)
     out = []
     for key, n, ce, i, j, p in tasks:
         try:
@@ -134,17 +134,23 @@ class LightconeEngine:
 
     ``composer``: QAOA composer class (default ``QtreeQAOAComposer``, the one
     ``qtensor.QAOA_energy`` uses).  ``max_tw``: lightcones whose greedy
-    treewidth reaches it are sliced (``SlicesOptimizer``) into 2^s work items.
+    treewidth reaches it are sliced into 2^s work items, the slice variables
+    chosen by ``slicer``: "greedy" (``SlicesOptimizer``, max-degree variables,
+    qtensor/optimisation/Optimizer.py:90-159) or "tree_trim"
+    (``TreeTrimSplitter``, Optimizer.py:178-228).
     """
 
     def __init__(self, composer=QtreeQAOAComposer, device=0, dtype="complex64", shape=None,
-                 max_tw=None, tw_bias=0):
+                 max_tw=None, tw_bias=0, slicer="greedy"):
+        if slicer not in ("greedy", "tree_trim"):
+            raise ValueError(f"unknown slicer {slicer!r}")
         self.composer = composer
         self.device_index = device
         self.dtype = dtype
         self.shape = shape
         self.max_tw = max_tw
         self.tw_bias = tw_bias
+        self.slicer = slicer
         self._plans = _PLAN_CACHE
         self._lock = _PLAN_LOCK
 
@@ -152,6 +158,8 @@ class LightconeEngine:
     def _optimizer(self):
         if self.max_tw is None:
             return OrderingOptimizer()
+        if self.slicer == "tree_trim":
+            return TreeTrimSplitter(max_tw=self.max_tw, tw_bias=self.tw_bias)
         return SlicesOptimizer(max_tw=self.max_tw, tw_bias=self.tw_bias)
 
     def shape_for(self, p):
@@ -160,7 +168,7 @@ class LightconeEngine:
         return DEFAULT_SHAPES["small"] if p <= 2 else DEFAULT_SHAPES["large"]
 
     def plan(self, key, n, edges, i, j, p, shape):
-        ck = (key, p, shape.key(), self.composer.__name__, self.max_tw)
+        ck = (key, p, shape.key(), self.composer.__name__, self.max_tw, self.slicer)
         with self._lock:
             hit = self._plans.get(ck)
         if hit is not None:
@@ -219,7 +227,7 @@ class LightconeEngine:
         todo, seen = [], set()
         for e in edges:
             key, n, ce, i, j = keyed[e]
-            ck = (key, p, shape.key(), self.composer.__name__, self.max_tw)
+            ck = (key, p, shape.key(), self.composer.__name__, self.max_tw, self.slicer)
             if key in seen:
                 continue
             seen.add(key)
@@ -229,13 +237,14 @@ class LightconeEngine:
             todo.append((key, n, ce, i, j, p))
         if todo:
             chunk = max(1, -(-len(todo) // (2 * workers)))
-            jobs = [(self.composer.__name__, self.dtype, shape, self.max_tw, self.tw_bias, todo[k:k + chunk])
+            jobs = [(self.composer.__name__, self.dtype, shape, self.max_tw, self.tw_bias, self.slicer,
+                     todo[k:k + chunk])
                     for k in range(0, len(todo), chunk)]
             for part in pool.map(_plan_task, jobs):
                 for key, plan, err in part:
                     if plan is not None:
                         with self._lock:
-                            self._plans[(key, p, shape.key(), self.composer.__name__, self.max_tw)] = plan
+                            self._plans[(key, p, shape.key(), self.composer.__name__, self.max_tw, self.slicer)] = plan
         if not hasattr(self, "_keyed"):
             self._keyed = {}
         self._keyed[(id(G), p)] = keyed

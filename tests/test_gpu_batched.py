@@ -56,13 +56,17 @@ def test_p3_vs_oracle(gpu, dtype, tol):
         assert abs(z - ref) <= tol * max(1.0, abs(ref))
 
 
-def test_sliced_items_sum_to_unsliced(gpu):
+@pytest.mark.parametrize("slicer", ["greedy", "tree_trim"])
+def test_sliced_items_sum_to_unsliced(gpu, slicer):
+    """Sliced work items (max-degree or tree-trimming slice variables) sum to
+    the unsliced terms (the sliced == unsliced invariant of
+    tests/test_qaoa_energy.py:106-121)."""
     G = nx.random_regular_graph(3, 40, seed=7)
     g, b = [0.3, 0.7, 0.2], [0.5, 0.1, 0.9]
     edges = list(G.edges())[:4]
     shape = LaunchShape(256, 8192, 512)
     _, ref = LightconeEngine(shape=shape, dtype="complex128").energy(G, g, b, edges=edges)
-    eng = LightconeEngine(shape=shape, max_tw=9, dtype="complex128")
+    eng = LightconeEngine(shape=shape, max_tw=9, dtype="complex128", slicer=slicer)
     wl = eng.workload(G, 3, edges)
     assert any(w.n_slices > 1 for w in wl.works)
     batch = eng.batch([(wl, g, b)])

@@ -296,31 +296,36 @@ def test_fuse_steps_limits():
 
 
 def test_autotuner_two_stage_search(monkeypatch):
-    """Stage 1 times every slicing threshold under the base shape, stage 2 the
-    other shapes at the two best thresholds; the fastest candidate wins."""
+    """Stage 1 times every slicing threshold with each slicer under the base
+    shape, stage 2 the other shapes at the best (threshold, slicer) pairs; the
+    fastest candidate wins."""
     import networkx as nx
     from qtensor_amd import autotune
     from qtensor_amd.batched import LaunchShape
     G = nx.random_regular_graph(3, 16, seed=7)
     calls = []
 
-    def fake_time(self, G, p, gamma, beta, shape, mtw):
-        calls.append((shape.key(), mtw))
+    def fake_time(self, G, p, gamma, beta, shape, mtw, slicer="greedy"):
+        calls.append((shape.key(), mtw, slicer))
         if shape.key() == (64, 4096, 2048):
             raise ValueError("LDS capacity exhausted")
-        cost = {None: 5.0, 13: 4.0, 12: 3.0, 11: 2.0, 10: 2.5}[mtw]
+        cost = {None: 5.0, 13: 4.0, 12: 3.0, 11: 2.0, 10: 2.5}[mtw] - 0.6 * (slicer == "tree_trim" and mtw == 10)
         return cost - 0.1 * (shape.item_threads == 256) - 0.01 * (shape.lds_capacity == 2048), 0.0, None, None
 
     monkeypatch.setattr(autotune.Autotuner, "time_config", fake_time)
     monkeypatch.setattr(autotune, "slicing_candidates", lambda max_tw: [None, 13, 12, 11, 10])
-    shape, mtw, table = autotune.Autotuner().tune(G, 2, [0.1, 0.2], [0.3, 0.4])
+    shape, mtw, slicer, table = autotune.Autotuner().tune(G, 2, [0.1, 0.2], [0.3, 0.4])
     base = [c for c in calls if c[0] == autotune.BASE_SHAPE.key()]
-    assert [m for _, m in base] == [None, 13, 12, 11, 10]
+    # every threshold with each slicer, unsliced once
+    assert [(m, sl) for _, m, sl in base] == [(None, "greedy")] + [(m, sl) for m in (13, 12, 11, 10)
+                                                                  for sl in autotune.SLICERS]
     rest = [c for c in calls if c[0] != autotune.BASE_SHAPE.key()]
-    assert sorted({m for _, m in rest}, key=str) == sorted({11, 10}, key=str)
-    assert len(rest) == 2 * len(autotune.OTHER_SHAPES)
-    assert shape == LaunchShape(256, 2048, 1024) and mtw == 11
-    assert sum("error" in row for row in table) == 2
+    # the best (threshold, slicer) pairs of stage 1 under every other shape
+    # (10, tree_trim) costs 1.9, then (11, greedy) and (11, tree_trim) at 2.0
+    assert sorted({(m, sl) for _, m, sl in rest}) == [(10, "tree_trim"), (11, "greedy"), (11, "tree_trim")]
+    assert len(rest) == autotune.STAGE2_PAIRS * len(autotune.OTHER_SHAPES)
+    assert shape == LaunchShape(256, 2048, 1024) and mtw == 10 and slicer == "tree_trim"
+    assert sum("error" in row for row in table) == autotune.STAGE2_PAIRS    # the (64, ...) shape
 
 
 def test_native_greedy_matches_oracle():
