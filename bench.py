@@ -26,11 +26,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per executor launch on the default workload, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE runs (profiles/exec_kernel_qaoa_p3_n256.md): 18219.2 KiB
-# fetched x 2.0 (a 1 GiB device copy reads as 524306.4 KiB) + 10202.7 KiB written
-# x 1.0 (the copy's 1 GiB writes read as 1048576 KiB).
-EXEC_TRAFFIC_BYTES = int(18219.2 * 1024 * 2.0 + 10202.7 * 1024 * 1.0)
+# HBM bytes per executor launch on the default workload under the launch
+# configuration the autotuner picks for it (256-thread items, 8192-element LDS
+# budget, tensors <= 4096 elements in LDS, tree-trimming slices below
+# treewidth 10), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+# (profiles/exec_kernel_qaoa_p3_n256.md): 19101.5 KiB fetched x 2.0 (a 1 GiB
+# device copy reads as 524306.9 KiB) + 10231.3 KiB written x 1.0 (the copy's
+# 1 GiB writes read as 1048576 KiB).  Reported only for that configuration.
+EXEC_TRAFFIC_CONFIG = ((256, 8192, 4096), 10, "tree_trim")
+EXEC_TRAFFIC_BYTES = int(19101.5 * 1024 * 2.0 + 10231.3 * 1024 * 1.0)
 
 
 def workload(name):
@@ -478,7 +482,8 @@ def main():
         "plan_seconds": {"cold": round(plan_cold, 3), "warm": round(plan_warm, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": EXEC_TRAFFIC_BYTES if args.workload == "c2" else None,
+                     "traffic": EXEC_TRAFFIC_BYTES if (args.workload == "c2" and (shape.key(), max_tw, slicer)
+                                                       == EXEC_TRAFFIC_CONFIG) else None,
                      "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
                      "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,
                      "executed_step_bytes_per_launch": batch.bytes_step_model},
