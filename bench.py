@@ -270,10 +270,14 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
             "per_set": per_set}
 
 
-def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4)):
+def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4),
+                slicer="tree_trim", shape4=None, max_tw4=16):
     """Poisson stream of energy requests (C5) through the continuous-batching server.
-    Depths 1-3 use 64-thread items sliced below treewidth 11; depth 4 uses
-    256-thread items sliced below 16, with larger tensors in global scratch."""
+    Depths 1-3 use ``shape`` (default 64-thread items) sliced below treewidth
+    ``max_tw``; depth 4 uses ``shape4`` (default 256-thread items) sliced below
+    ``max_tw4``, with larger tensors in global scratch; ``slicer`` picks the
+    slice variables of both (tree trimming: about half the work items of the
+    max-degree rule on these graphs)."""
     import networkx as nx
     import numpy as np
     from qtensor_amd.batched import LaunchShape, LightconeEngine
@@ -283,8 +287,8 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
     for k in range(pool_size):
         n = int(rng.integers(10, 1001)) * 2
         pool.append((f"rr3_n{n}_s{k}", nx.random_regular_graph(3, n, seed=100 + k)))
-    eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw)
-    eng4 = LightconeEngine(shape=LaunchShape(256, 4096, 2048), max_tw=16)
+    eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw, slicer=slicer)
+    eng4 = LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=max_tw4, slicer=slicer)
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
     server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=workers).start()
     cold = {}
