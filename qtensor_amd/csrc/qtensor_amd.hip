@@ -865,8 +865,10 @@ __device__ __forceinline__ void op_sync(uint32_t w0) {
 // All ops of one plan.  Software pipeline over the records: while op k runs,
 // op k+1's lane table words (gather ops) and op k+2's record word are in
 // flight.  r0 is op 0's record word, fetched by the caller.  One copy of the
-// op bodies keeps the kernel inside the instruction cache.
-template <typename R, int ITEM_THREADS>
+// op bodies keeps the kernel inside the instruction cache.  GLB = false is the
+// variant for batches without global scratch (every op all-LDS): no
+// global-scratch step and no global fences.
+template <typename R, int ITEM_THREADS, bool GLB>
 __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st0, int32_t n_ops, int32_t r0,
                                         QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
     if (n_ops <= 0) return;
@@ -887,12 +889,12 @@ __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st
             st2 = st1 + (w1 >> 16);
             if (k + 2 < n_ops) r2 = fetch_rec(prog, st2, lane);
         }
-        op_sync<ITEM_THREADS>(w0);
+        op_sync<ITEM_THREADS>(GLB ? w0 : 0u);
         if (w0 & kOpGather)
             gather_op<R>(r0, g0, lds, lane);
         else if (w0 & kOpReduce)
             reduce_op<R>(r0, w0, lds, lane);
-        else if (w0 & (1u << 12))
+        else if (!GLB || (w0 & (1u << 12)))
             step_op<R, ITEM_THREADS>(r0, w0, lds, lane);
         else
             step_mixed<R, ITEM_THREADS>(r0, w0, lds, srd, lane);
@@ -918,7 +920,7 @@ __host__ __device__ constexpr int exec_aux_bytes(int item_threads) {
     return ((item_threads + 63) / 64) * kParamRegs * 8 + 16;
 }
 
-template <typename R, int ITEM_THREADS>
+template <typename R, int ITEM_THREADS, bool GLB>
 __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
     const int32_t* __restrict__ program, uint32_t program_bytes, const int32_t* __restrict__ items,
     int32_t n_items, int32_t lds_elems, const double* __restrict__ params, const void* __restrict__ pool_v,
@@ -982,7 +984,7 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
         for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
             build_gate<R>(gates + (size_t)g * kGateWords, prm, my_angles, kParamRegs, slice_bits, lds, pool);
         group_sync<ITEM_THREADS>();
-        run_ops<R, ITEM_THREADS>(prog, st0, plan[1], r0, lds, srd, lane);
+        run_ops<R, ITEM_THREADS, GLB>(prog, st0, plan[1], r0, lds, srd, lane);
         group_sync<ITEM_THREADS>();
         if (lane == 0) {
             const V r = load_loc((uint32_t)plan[4], 0u, lds, glb);
@@ -1070,7 +1072,7 @@ __global__ __launch_bounds__(256) void reduce_total(const double* __restrict__ t
 
 template <typename R, int T>
 int exec_occupancy(int lds_bytes, int* blocks) {
-    QB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, exec_kernel<R, T>, T, lds_bytes));
+    QB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, exec_kernel<R, T, true>, T, lds_bytes));
     return QB_OK;
 }
 
@@ -1086,23 +1088,31 @@ int exec_occupancy_dispatch(int item_threads, int lds_bytes, int* blocks) {
     }
 }
 
-template <typename R, int T>
-int exec_launch(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
+template <typename R, int T, bool GLB>
+int exec_launch_variant(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
     // the kernel addresses its LDS data from address 0: it must declare no
     // static LDS (checked once per instantiation)
     static int static_lds = -1;
     if (static_lds < 0) {
         hipFuncAttributes attr;
-        QB_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&exec_kernel<R, T>)));
+        QB_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&exec_kernel<R, T, GLB>)));
         static_lds = (int)attr.sharedSizeBytes;
     }
     if (static_lds != 0)
         return set_error(QB_ERR_UNSUPPORTED, "executor kernel declares %d bytes of static LDS", static_lds);
-    hipLaunchKernelGGL((exec_kernel<R, T>), dim3(grid), dim3(T), lds_bytes, st, a.program, a.program_bytes, a.items,
-                       a.n_items, a.lds_elems, a.params, a.pool, a.scratch, a.scratch_elems, a.item_out,
+    hipLaunchKernelGGL((exec_kernel<R, T, GLB>), dim3(grid), dim3(T), lds_bytes, st, a.program, a.program_bytes,
+                       a.items, a.n_items, a.lds_elems, a.params, a.pool, a.scratch, a.scratch_elems, a.item_out,
                        a.queue);
     QB_HIP(hipGetLastError());
     return QB_OK;
+}
+
+// Batches whose plans need no global scratch (scratch_elems == 0) run the
+// variant without the global-scratch path.
+template <typename R, int T>
+int exec_launch(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
+    if (a.scratch_elems > 0) return exec_launch_variant<R, T, true>(a, grid, lds_bytes, st);
+    return exec_launch_variant<R, T, false>(a, grid, lds_bytes, st);
 }
 
 template <typename R>
