@@ -5,7 +5,9 @@ Workload (default): QAOA p=3 MaxCut on a random 3-regular graph, N=256
 numpy default_rng(2), per-edge lightcone networks (QtreeQAOAComposer,
 greedy ordering), complex64 with FP64 gate construction and term sums.
 One step = one full energy evaluation: all 384 edge terms, plans and
-buffers resident on the device, launched as one captured HIP graph.
+buffers resident on the device: the executor launch and the FP64 term
+reductions issued on one stream (the same launches replayed from a captured
+HIP graph are reported beside the value; they are slower on this stack).
 
 Reported beside the value: the plain per-bucket GPU backend on a sample of
 edges, the error of every term against a complex128 run of the same
@@ -414,8 +416,9 @@ def main():
     wl_alg = LightconeEngine(shape=shape).workload(G, p)
     alg_bytes = wl_alg.step_model_bytes()
     batch = eng.batch([(wl, gamma, beta)])
-    if not args.kernel_only:          # counter runs see plain launches, not graph nodes
-        batch.capture()
+    # a step is the executor launch and the FP64 reductions issued on the
+    # stream (a captured HIP graph replays the same launches more slowly here:
+    # its rate is reported beside the value)
     for _ in range(args.warmup):
         batch.run()
     torch.cuda.synchronize()
@@ -428,6 +431,18 @@ def main():
     value = n_terms * args.steps / elapsed
     terms64 = np.array(batch.terms()[0])
     print(f"[bench] batched: {value:.1f} terms/s, {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+    graph_ms = None
+    if not args.kernel_only:          # counter runs see plain launches only
+        batch.capture()
+        for _ in range(args.warmup):
+            batch.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.run(sync=False)
+        torch.cuda.synchronize()
+        graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        batch.graph = None
 
     # executor kernel duration, HIP events on the launch stream
     stream = torch.cuda.current_stream()
@@ -475,7 +490,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "complex64",
         "data": "synthetic (seeded networkx graph and angles)",
         "config": {"workload": desc, "p": p, "edges": n_terms, "composer": "QtreeQAOAComposer",
-                   "ordering": "greedy", "form": "batched persistent executor, HIP graph",
+                   "ordering": "greedy", "form": "batched persistent executor, stream launches",
                    "max_treewidth": wl_alg.max_treewidth(), "distinct_plans": len(set(map(id, wl.plans.values()))),
                    "item_threads": shape.item_threads, "lds_capacity": shape.lds_capacity,
                    "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw, "slicer": slicer,
@@ -484,6 +499,7 @@ def main():
         "max_rel_err_vs_fp64": err_fp64,
         "energy": energy,
         "plan_seconds": {"cold": round(plan_cold, 3), "warm": round(plan_warm, 3)},
+        "graph_replay_ms_per_step": None if graph_ms is None else round(graph_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": EXEC_TRAFFIC_BYTES if (args.workload == "c2" and (shape.key(), max_tw, slicer)

@@ -157,7 +157,8 @@ int qb_wide_execute(const qb_wide_desc* desc, void* stream);
 /*
  * FP64 reductions: segment s covers items [seg[s], seg[s+1]); term_out[s] is
  * the (Neumaier-compensated) complex sum of its items' values, and total[0..1]
- * the complex sum of all terms.  n_segments <= 2^20.
+ * the complex sum of all terms.  n_segments <= 2^20.  seg = NULL means one
+ * item per segment (item s is term s): one launch instead of two.
  */
 int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segments,
                     double* term_out, double* total, void* stream);

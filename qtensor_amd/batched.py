@@ -330,6 +330,9 @@ class Batch:
             self.term_slices.append((start, len(seg) - 1))
         self.n_items = len(items)
         self.n_terms = len(seg) - 1
+        # no slicing anywhere: item slot s holds term s and the reduction needs
+        # no segment table (one launch)
+        self.one_item_per_term = self.n_items == self.n_terms
         self.lds_elems = max(lds_need, 2)
         self.scratch_elems = int(glb_need)
         slots = ctypes.c_int32(0)
@@ -395,7 +398,8 @@ class Batch:
         h = ctypes.c_void_p(st.cuda_stream)
         _lib.check(self.lib.qb_execute(ctypes.byref(self._desc), h))
         _lib.check(self.lib.qb_reduce_terms(ctypes.c_void_p(self.item_out.data_ptr()),
-                                            ctypes.c_void_p(self.seg.data_ptr()), self.n_terms,
+                                            None if self.one_item_per_term else ctypes.c_void_p(self.seg.data_ptr()),
+                                            self.n_terms,
                                             ctypes.c_void_p(self.term_out.data_ptr()),
                                             ctypes.c_void_p(self.total.data_ptr()), h))
 

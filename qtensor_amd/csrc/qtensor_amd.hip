@@ -1039,13 +1039,19 @@ __global__ __launch_bounds__(64) void reduce_segments(const double* __restrict__
     }
 }
 
+// copy_to (optional): the terms are also written there (one item per term).
 __global__ __launch_bounds__(256) void reduce_total(const double* __restrict__ term_out, int32_t n_seg,
-                                                    double* __restrict__ total) {
+                                                    double* __restrict__ total, double* __restrict__ copy_to) {
     __shared__ double sh[4][4];
     double sr = 0, cr = 0, si = 0, ci = 0;
     for (int32_t s = threadIdx.x; s < n_seg; s += blockDim.x) {
-        neumaier(sr, cr, term_out[2 * (size_t)s]);
-        neumaier(si, ci, term_out[2 * (size_t)s + 1]);
+        const double re = term_out[2 * (size_t)s], im = term_out[2 * (size_t)s + 1];
+        if (copy_to) {
+            copy_to[2 * (size_t)s] = re;
+            copy_to[2 * (size_t)s + 1] = im;
+        }
+        neumaier(sr, cr, re);
+        neumaier(si, ci, im);
     }
     wave_combine(sr, cr);
     wave_combine(si, ci);
@@ -1567,14 +1573,21 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
 int qb_reduce_terms(const double* item_out, const int32_t* seg, int32_t n_segments,
                     double* term_out, double* total, void* stream) {
     if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
-    if (!item_out || !seg || !term_out || !total) return set_error(QB_ERR_INVALID, "null pointer");
+    if (!item_out || !term_out || !total) return set_error(QB_ERR_INVALID, "null pointer");
     if (n_segments < 1 || n_segments > (1 << 20))
         return set_error(QB_ERR_INVALID, "n_segments=%d outside [1, 2^20]", n_segments);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!seg) {
+        // one item per term: the terms are the item results (one launch)
+        hipLaunchKernelGGL(reduce_total, dim3(1), dim3(256), 0, st, item_out, n_segments, total, term_out);
+        QB_HIP(hipGetLastError());
+        return QB_OK;
+    }
     hipLaunchKernelGGL(reduce_segments, dim3(n_segments), dim3(64), 0, st, item_out, seg, n_segments,
                        term_out);
     QB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(reduce_total, dim3(1), dim3(256), 0, st, term_out, n_segments, total);
+    hipLaunchKernelGGL(reduce_total, dim3(1), dim3(256), 0, st, term_out, n_segments, total,
+                       (double*)nullptr);
     QB_HIP(hipGetLastError());
     return QB_OK;
 }
@@ -1595,6 +1608,7 @@ struct qb_plan {
     int32_t n_slots = 0;
     int64_t scratch_elems = 0;
     int64_t program_words = 0;
+    bool unit_seg = false;       // every term has exactly one item
     int32_t* program = nullptr;
     int32_t* items = nullptr;
     int32_t* seg = nullptr;
@@ -1633,6 +1647,8 @@ int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
         if (d->seg[t] > d->seg[t + 1]) return set_error(QB_ERR_INVALID, "segments must not decrease");
     if (d->seg[0] != 0 || d->seg[d->n_terms] != d->n_items)
         return set_error(QB_ERR_INVALID, "segments must cover items 0..n_items");
+    bool unit_seg = true;
+    for (int32_t t = 0; t < d->n_terms; ++t) unit_seg = unit_seg && d->seg[t + 1] - d->seg[t] == 1;
     for (int32_t i = 0; i < d->n_items; ++i) {
         const int32_t* it = d->items + (size_t)i * QB_ITEM_WORDS;
         if (it[0] < 0 || (int64_t)it[0] + kPlanHdr > d->program_words)
@@ -1655,6 +1671,7 @@ int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
     p->n_slots = n_slots < d->n_items ? n_slots : d->n_items;
     p->scratch_elems = d->scratch_elems;
     p->program_words = d->program_words;
+    p->unit_seg = unit_seg;
     const size_t elem = d->dtype == QB_C128 ? 16 : 8;
     hipError_t e = hipSuccess;
 #define QB_PLAN_TRY(call)                                                          \
@@ -1705,7 +1722,7 @@ int qb_plan_execute(qb_plan* p, const double* params, double* term_out, double* 
     d.queue = p->queue;
     int rc = qb_execute(&d, stream);
     if (rc) return rc;
-    return qb_reduce_terms(p->item_out, p->seg, p->n_terms, term_out, total, stream);
+    return qb_reduce_terms(p->item_out, p->unit_seg ? nullptr : p->seg, p->n_terms, term_out, total, stream);
 }
 
 int qb_plan_destroy(qb_plan* p) {

@@ -395,7 +395,8 @@ class EnergyServer:
             h = ctypes.c_void_p(st.cuda_stream)
             _lib.check(self.lib.qb_execute(ctypes.byref(d), h))
             _lib.check(self.lib.qb_reduce_terms(ctypes.c_void_p(d_out.data_ptr()),
-                                                ctypes.c_void_p(d_seg.data_ptr()), n_terms,
+                                                None if n_items == n_terms else ctypes.c_void_p(d_seg.data_ptr()),
+                                                n_terms,
                                                 ctypes.c_void_p(d_terms.data_ptr()),
                                                 ctypes.c_void_p(d_total.data_ptr()), h))
             h_terms[:2 * n_terms].copy_(d_terms[:2 * n_terms], non_blocking=True)

@@ -177,3 +177,27 @@ def test_reduce_terms_vs_fsum(gpu, n_seg, max_len):
         assert np.max(np.abs(terms[:, c] - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
         tot = math.fsum(vals[:, c])
         assert abs(d_total[c].item() - tot) <= 1e-12 * max(1.0, np.sum(np.abs(vals[:, c])))
+
+
+@pytest.mark.parametrize("n", [1, 1500, 5000])
+def test_reduce_terms_one_item_per_term(gpu, n):
+    """qb_reduce_terms without a segment table (item s is term s): the terms
+    are the item values unchanged, the total matches math.fsum."""
+    import ctypes
+    import math
+
+    import torch
+
+    from qtensor_amd import _lib
+    lib = _lib.init(0)
+    rng = np.random.default_rng(n)
+    vals = rng.standard_normal((n, 2)) * 10.0 ** rng.integers(-8, 3, (n, 1))
+    d_items = torch.tensor(vals, dtype=torch.float64, device=gpu)
+    d_terms = torch.zeros((n, 2), dtype=torch.float64, device=gpu)
+    d_total = torch.zeros(2, dtype=torch.float64, device=gpu)
+    _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(d_items.data_ptr()), None, n,
+                                   ctypes.c_void_p(d_terms.data_ptr()), ctypes.c_void_p(d_total.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert np.array_equal(d_terms.cpu().numpy(), vals)
+    for c in range(2):
+        assert abs(d_total[c].item() - math.fsum(vals[:, c])) <= 1e-12 * max(1.0, np.sum(np.abs(vals[:, c])))
