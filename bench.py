@@ -29,14 +29,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per executor launch on the default workload under the launch
-# configuration the autotuner picks for it (256-thread items, 8192-element LDS
-# budget, tensors <= 4096 elements in LDS, tree-trimming slices below
-# treewidth 10), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-# (profiles/exec_kernel_qaoa_p3_n256.md): 19101.5 KiB fetched x 2.0 (a 1 GiB
-# device copy reads as 524306.9 KiB) + 10231.3 KiB written x 1.0 (the copy's
-# 1 GiB writes read as 1048576 KiB).  Reported only for that configuration.
-EXEC_TRAFFIC_CONFIG = ((256, 8192, 4096), 10, "tree_trim")
-EXEC_TRAFFIC_BYTES = int(19101.5 * 1024 * 2.0 + 10231.3 * 1024 * 1.0)
+# configuration the autotuner picks for it (256-thread items, tree-trimming
+# slices below treewidth 10: 1274 work items, no global scratch; the LDS
+# budgets 4096 and 8192 compile the same plans), from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs (profiles/exec_kernel_qaoa_p3_n256.md): 13882.2 KiB fetched
+# x 2.0 (a 1 GiB device copy reads as 524306.4 KiB) + 39.25 KiB written x 1.0
+# (the copy's 1 GiB writes read as 1048576 KiB).  Reported only for that
+# configuration: (item threads, slicing threshold, slicer, work items).
+EXEC_TRAFFIC_CONFIG = (256, 10, "tree_trim", 1274)
+EXEC_TRAFFIC_BYTES = int(13882.2 * 1024 * 2.0 + 39.25 * 1024 * 1.0)
 
 
 def workload(name):
@@ -502,7 +503,8 @@ def main():
         "graph_replay_ms_per_step": None if graph_ms is None else round(graph_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": EXEC_TRAFFIC_BYTES if (args.workload == "c2" and (shape.key(), max_tw, slicer)
+                     "traffic": EXEC_TRAFFIC_BYTES if (args.workload == "c2" and batch.scratch_elems == 0 and
+                                                       (shape.item_threads, max_tw, slicer, batch.n_items)
                                                        == EXEC_TRAFFIC_CONFIG) else None,
                      "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
                      "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,
