@@ -586,6 +586,8 @@ struct GatherLane {
 __device__ __forceinline__ void fetch_lane(__amdgpu_buffer_rsrc_t prog, uint32_t st, int32_t r, int lane,
                                            GatherLane& g) {
     const uint32_t n = QB_RL(r, 1);
+    // a wave none of whose lanes is active loads nothing (the op skips it)
+    if (((uint32_t)__builtin_amdgcn_readfirstlane(lane) & ~63u) >= n) return;
     const uint32_t l = (uint32_t)lane < n ? (uint32_t)lane : n - 1u;
     const uint32_t vo = l * 4u;
     const uint32_t so = (st + QB_RL(r, 2)) * 4u;
@@ -720,7 +722,7 @@ __device__ __forceinline__ void step_lds_n(uint32_t n_out, uint32_t n_sum, const
         step_lds_pass<R, ITEM_THREADS, 4, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
     else if (n_out >= 2u * ITEM_THREADS)
         step_lds_pass<R, ITEM_THREADS, 2, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
-    else
+    else if (((uint32_t)__builtin_amdgcn_readfirstlane(lane) & ~63u) < n_out)   // waves past the outputs idle
         step_lds_pass<R, ITEM_THREADS, 1, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
 }
 
