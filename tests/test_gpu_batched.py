@@ -201,3 +201,17 @@ def test_reduce_terms_one_item_per_term(gpu, n):
     assert np.array_equal(d_terms.cpu().numpy(), vals)
     for c in range(2):
         assert abs(d_total[c].item() - math.fsum(vals[:, c])) <= 1e-12 * max(1.0, np.sum(np.abs(vals[:, c])))
+
+
+def test_autotuner_pick_matches_statevector(gpu):
+    """The autotuner's staged search on a small p=2 workload returns a
+    configuration whose energy terms match the statevector."""
+    from qtensor_amd.autotune import Autotuner
+    G = nx.random_regular_graph(3, 14, seed=11)
+    g, b = [0.4, 0.9], [0.3, 0.6]
+    shape, max_tw, slicer, table = Autotuner(reps=3).tune(G, 2, g, b)
+    assert table and any("exec_ms" in row for row in table)
+    C, zz = LightconeEngine(shape=shape, max_tw=max_tw, slicer=slicer).energy(G, g, b)
+    sv = statevector.zz_expectations(G, g, b)
+    for e, z in zip(G.edges(), zz):
+        assert abs(z - sv[e]) <= _tol(sv[e])
