@@ -29,6 +29,8 @@
  *   qb_greedy_order   -> min-degree greedy elimination order (qtensor/utils.py:12-44,
  *                        used by OrderingOptimizer, qtensor/optimisation/Optimizer.py:46-88)
  *   qb_elimination_widths -> widths of a fixed order (the same elimination, utils.py:77-86)
+ *   qb_place_intervals -> LDS / scratch offsets of a plan's buffers (the plan compiler's
+ *                        own layout step; the reference allocates with numpy)
  *   qb_init / qb_shutdown / qb_last_error -> backend construction and the Python
  *                        exceptions of the reference (AssertionError, ValueError)
  */
@@ -211,6 +213,16 @@ int qb_greedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_
 /* Widths of eliminating order[0..n_order) in that order (same elimination rule). */
 int qb_elimination_widths(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, int32_t n_order,
                           const int32_t* order, int32_t* widths);
+
+/*
+ * Interval allocation for the plan compiler (host only; no counterpart in the
+ * reference, which allocates every intermediate with numpy): buffer i lives
+ * over ops [starts[i], ends[i]] and needs sizes[i] elements.  In `order`,
+ * each buffer gets the lowest offset >= base free over its whole interval
+ * among the buffers already placed; *peak = the highest end (>= base).
+ */
+int qb_place_intervals(int32_t m, const int64_t* sizes, const int64_t* starts, const int64_t* ends,
+                       const int64_t* order, int64_t base, int64_t* offsets, int64_t* peak);
 
 #ifdef __cplusplus
 }

@@ -1,4 +1,5 @@
-// planner.cpp -- host-side elimination ordering for the bucket-elimination engine.
+// planner.cpp -- host-side planning code for the bucket-elimination engine:
+// elimination ordering and the plan compiler's interval allocator.
 //
 // Min-degree greedy ordering of an interaction (line) graph, the rule of
 // qtensor/utils.py:12-44 (get_neighbours_peo_vars) used by
@@ -8,6 +9,7 @@
 // count at elimination.  The same elimination with a fixed order gives the
 // widths of that order.  Plain host code: no device work.
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -99,6 +101,49 @@ int qb_greedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_
         order[step] = best;
         widths[step] = g.eliminate(best);
     }
+    return QB_OK;
+}
+
+// Offline interval allocation of a plan's buffers (the plan compiler's own
+// layout step, no counterpart in the reference): buffer i lives over ops
+// [starts[i], ends[i]] and needs sizes[i] elements; buffers are placed in
+// `order`, each at the lowest offset >= base that is free over its whole
+// interval (first fit over the placed buffers whose intervals overlap, sorted
+// by offset, ties in placement order).
+int qb_place_intervals(int32_t m, const int64_t* sizes, const int64_t* starts, const int64_t* ends,
+                       const int64_t* order, int64_t base, int64_t* offsets, int64_t* peak) {
+    if (m < 0 || (m > 0 && (!sizes || !starts || !ends || !order || !offsets)) || !peak)
+        return qb_internal::fail(QB_ERR_INVALID, "qb_place_intervals: bad count or null pointer");
+    std::vector<char> seen(m, 0);
+    for (int32_t k = 0; k < m; ++k) {
+        const int64_t i = order[k];
+        if (i < 0 || i >= m || seen[i] || sizes[i] < 0 || ends[i] < starts[i])
+            return qb_internal::fail(QB_ERR_INVALID, "qb_place_intervals: bad order, size or interval");
+        seen[i] = 1;
+    }
+    struct Placed { int64_t off, end, s, e; };
+    std::vector<Placed> placed;
+    placed.reserve(m);
+    std::vector<int32_t> live;
+    int64_t top = base;
+    for (int32_t k = 0; k < m; ++k) {
+        const int64_t i = order[k];
+        const int64_t n = sizes[i], s = starts[i], e = ends[i];
+        live.clear();
+        for (int32_t q = 0; q < k; ++q)
+            if (placed[q].s <= e && s <= placed[q].e) live.push_back(q);
+        std::stable_sort(live.begin(), live.end(),
+                         [&](int32_t a, int32_t b) { return placed[a].off < placed[b].off; });
+        int64_t off = base;
+        for (int32_t q : live) {
+            if (off + n <= placed[q].off) break;
+            if (placed[q].end > off) off = placed[q].end;
+        }
+        offsets[i] = off;
+        placed.push_back({off, off + n, s, e});
+        top = std::max(top, off + n);
+    }
+    *peak = top;
     return QB_OK;
 }
 

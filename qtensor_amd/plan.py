@@ -34,6 +34,7 @@ released, so no op writes memory it also reads.
 
 The program format is shared with ``csrc/qtensor_amd.hip`` (constants below).
 """
+import ctypes
 import functools
 import struct
 from dataclasses import dataclass, field
@@ -414,30 +415,17 @@ class _Arena:
 
 def _place_order(sizes, starts, ends, order, base):
     """Offsets for buffers placed in ``order``, each at the lowest offset free
-    over its whole op interval; returns (offsets, peak)."""
+    over its whole op interval (native ``qb_place_intervals``); returns
+    (offsets, peak)."""
+    from . import _lib
     m = len(sizes)
-    p_off = np.zeros(m, dtype=np.int64)
-    p_end = np.zeros(m, dtype=np.int64)
-    p_s = np.zeros(m, dtype=np.int64)
-    p_e = np.zeros(m, dtype=np.int64)
-    offsets = np.zeros(m, dtype=np.int64)
-    peak = base
-    for k, idx in enumerate(order):
-        n, s, e = sizes[idx], starts[idx], ends[idx]
-        off = base
-        if k:
-            live = (p_s[:k] <= e) & (s <= p_e[:k])
-            offs, tops = p_off[:k][live], p_end[:k][live]
-            srt = np.argsort(offs, kind="stable")
-            for a, z in zip(offs[srt].tolist(), tops[srt].tolist()):
-                if off + n <= a:
-                    break
-                if z > off:
-                    off = z
-        offsets[idx] = off
-        p_off[k], p_end[k], p_s[k], p_e[k] = off, off + n, s, e
-        peak = max(peak, int(off + n))
-    return offsets, peak
+    arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (sizes, starts, ends, order)]
+    offsets = np.zeros(max(m, 1), dtype=np.int64)
+    peak = ctypes.c_int64(0)
+    p64 = ctypes.POINTER(ctypes.c_int64)
+    _lib.check(_lib.load().qb_place_intervals(m, *(a.ctypes.data_as(p64) for a in arrs), int(base),
+                                              offsets.ctypes.data_as(p64), ctypes.byref(peak)))
+    return offsets[:m], int(peak.value)
 
 
 def _place_intervals(items, base, align):

@@ -356,3 +356,28 @@ def test_native_greedy_matches_oracle():
         assert elimination_widths(g, order) == oo.elimination_widths(g, order)
     with pytest.raises(ValueError):
         elimination_widths(nx.path_graph(3), [0, 0])
+
+
+def test_native_place_intervals_matches_restatement():
+    """qb_place_intervals against the plain-Python allocator (tests/place_ref.py)
+    on random buffer sets, and directly: buffers whose op intervals overlap get
+    disjoint element ranges, all at or above the base."""
+    import numpy as np
+    from qtensor_amd.plan import _place_order
+    from place_ref import place_order
+    rng = np.random.default_rng(21)
+    for trial in range(40):
+        m = int(rng.integers(1, 300))
+        sizes = rng.integers(1, 64, m) * 2
+        starts = rng.integers(0, 100, m)
+        ends = starts + rng.integers(0, 30, m)
+        order = rng.permutation(m)
+        base = int(rng.integers(0, 16))
+        offs, peak = _place_order(sizes, starts, ends, order, base)
+        ref_offs, ref_peak = place_order(sizes, starts, ends, order, base)
+        assert list(offs) == ref_offs and peak == ref_peak
+        assert offs.min() >= base and peak == max(base, int((offs + sizes).max()))
+        for i in range(m):
+            for j in range(i + 1, m):
+                if starts[i] <= ends[j] and starts[j] <= ends[i]:
+                    assert offs[i] + sizes[i] <= offs[j] or offs[j] + sizes[j] <= offs[i]
