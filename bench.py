@@ -274,13 +274,14 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
 
 
 def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4),
-                slicer="tree_trim", shape4=None, max_tw4=16):
+                slicer="tree_trim", shape4=None, max_tw4=16, slicer4="tree_trim"):
     """Poisson stream of energy requests (C5) through the continuous-batching server.
     Depths 1-3 use ``shape`` (default 64-thread items) sliced below treewidth
     ``max_tw``; depth 4 uses ``shape4`` (default 256-thread items) sliced below
-    ``max_tw4``, with larger tensors in global scratch; ``slicer`` picks the
-    slice variables of both (tree trimming: about half the work items of the
-    max-degree rule on these graphs)."""
+    ``max_tw4``, with larger tensors in global scratch; ``slicer`` and
+    ``slicer4`` pick their slice variables (tree trimming needs about half
+    the work items of max-degree slices on these graphs; with max-degree
+    slices at p = 4 the server saturates at about a third of the rate)."""
     import networkx as nx
     import numpy as np
     from qtensor_amd.batched import LaunchShape, LightconeEngine
@@ -291,7 +292,7 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
         n = int(rng.integers(10, 1001)) * 2
         pool.append((f"rr3_n{n}_s{k}", nx.random_regular_graph(3, n, seed=100 + k)))
     eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw, slicer=slicer)
-    eng4 = LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=max_tw4, slicer=slicer)
+    eng4 = LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=max_tw4, slicer=slicer4)
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
     server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=workers).start()
     cold = {}
