@@ -728,40 +728,48 @@ __device__ __forceinline__ void step_lds_n(uint32_t n_out, uint32_t n_sum, const
 
 // Output blocks of a step whose inputs or output live in global scratch
 // (accessed through the slot's buffer resource `srd`).
-template <typename R, int ITEM_THREADS, int U>
-__device__ __forceinline__ void step_mixed_pass(uint32_t n_out, uint32_t n_in, uint32_t n_sum,
+template <typename R, int ITEM_THREADS, int U, int NI>
+__device__ __forceinline__ void step_mixed_pass(uint32_t n_out, uint32_t n_sum,
                                                 const uint32_t* loc, const uint32_t* top, const uint32_t* mask,
                                                 const uint32_t* runs, uint32_t out_loc,
                                                 QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd,
                                                 int lane) {
     using V = typename Cx<R>::V;
-    uint32_t la[kExecMaxIn];
+    uint32_t la[NI];
 #pragma unroll
-    for (int j = 0; j < kExecMaxIn; ++j)
-        la[j] = (uint32_t)j < n_in ? (loc[j] & ~kGlobal) + in_offset((uint32_t)lane, mask[j], runs[j]) : 0u;
+    for (int j = 0; j < NI; ++j) la[j] = (loc[j] & ~kGlobal) + in_offset((uint32_t)lane, mask[j], runs[j]);
     for (uint32_t base = 0; base < n_out; base += U * ITEM_THREADS) {
         V p[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t hb = base + u * ITEM_THREADS;
-            V x[kExecMaxIn], y[kExecMaxIn];
+            V x[NI], y[NI];
 #pragma unroll
-            for (int j = 0; j < kExecMaxIn; ++j) {
-                x[j] = V{1, 0};
-                y[j] = V{1, 0};
-                if ((uint32_t)j < n_in) {
-                    const uint32_t a = la[j] + in_offset(hb, mask[j], runs[j]);
-                    if (loc[j] & kGlobal) {
-                        x[j] = buf_load(srd, a, (V*)nullptr);
-                        if (n_sum == 2) y[j] = buf_load(srd, a + top[j], (V*)nullptr);
-                    } else {
-                        x[j] = lds[a];
-                        if (n_sum == 2) y[j] = lds[a + top[j]];
-                    }
+            for (int j = 0; j < NI; ++j) {
+                const uint32_t a = la[j] + in_offset(hb, mask[j], runs[j]);
+                if (loc[j] & kGlobal) {
+                    x[j] = buf_load(srd, a, (V*)nullptr);
+                    if (n_sum == 2) y[j] = buf_load(srd, a + top[j], (V*)nullptr);
+                } else {
+                    x[j] = lds[a];
+                    if (n_sum == 2) y[j] = lds[a + top[j]];
                 }
             }
-            p[u] = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
-            if (n_sum == 2) p[u] = cadd(p[u], cmul(cmul(y[0], y[1]), cmul(y[2], y[3])));
+            V q;
+            if (NI == 1) {
+                p[u] = x[0];
+                if (n_sum == 2) q = y[0];
+            } else if (NI == 2) {
+                p[u] = cmul(x[0], x[1]);
+                if (n_sum == 2) q = cmul(y[0], y[1]);
+            } else if (NI == 3) {
+                p[u] = cmul(cmul(x[0], x[1]), x[2]);
+                if (n_sum == 2) q = cmul(cmul(y[0], y[1]), y[2]);
+            } else {
+                p[u] = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
+                if (n_sum == 2) q = cmul(cmul(y[0], y[1]), cmul(y[2], y[3]));
+            }
+            if (n_sum == 2) p[u] = cadd(p[u], q);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -776,12 +784,23 @@ __device__ __forceinline__ void step_mixed_pass(uint32_t n_out, uint32_t n_in, u
     }
 }
 
+template <typename R, int ITEM_THREADS, int NI>
+__device__ __forceinline__ void step_mixed_n(uint32_t n_out, uint32_t n_sum, const uint32_t* loc,
+                                             const uint32_t* top, const uint32_t* mask, const uint32_t* runs,
+                                             uint32_t out_loc, QB_LDS typename Cx<R>::V* lds,
+                                             __amdgpu_buffer_rsrc_t srd, int lane) {
+    if (n_out >= 4u * ITEM_THREADS)
+        step_mixed_pass<R, ITEM_THREADS, 4, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
+    else
+        step_mixed_pass<R, ITEM_THREADS, 1, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
+}
+
 // A step op with global scratch: every input is uniformly in LDS or in the
-// item's scratch slot; reads of a pass are issued before its products.  Out of
-// line, so that its scalar registers do not crowd the op loop (all-LDS steps
-// and gather ops do not pay for it).
+// item's scratch slot; reads of a pass are issued before its products.  Only
+// the executor variant for batches with global scratch contains it (inline: a
+// call would save and restore the callee-saved registers it uses every time).
 template <typename R, int ITEM_THREADS>
-__device__ __attribute__((noinline)) void step_mixed(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
+__device__ __forceinline__ void step_mixed(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
                                                      __amdgpu_buffer_rsrc_t srd, int lane) {
     const uint32_t out_loc = QB_RL(r, 2);
     const uint32_t n_in = w0 & 15u;
@@ -799,10 +818,12 @@ __device__ __attribute__((noinline)) void step_mixed(int32_t r, uint32_t w0, QB_
             loc[j] = 0u; top[j] = 0u; mask[j] = 0u; runs[j] = 0u;
         }
     }
-    if (n_out >= 4u * ITEM_THREADS)
-        step_mixed_pass<R, ITEM_THREADS, 4>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
-    else
-        step_mixed_pass<R, ITEM_THREADS, 1>(n_out, n_in, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
+    switch (n_in) {
+    case 1: step_mixed_n<R, ITEM_THREADS, 1>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
+    case 2: step_mixed_n<R, ITEM_THREADS, 2>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
+    case 3: step_mixed_n<R, ITEM_THREADS, 3>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
+    default: step_mixed_n<R, ITEM_THREADS, 4>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
+    }
 }
 
 // One step op: all lanes of the item cover the step's output elements.  Lane l
