@@ -512,24 +512,6 @@ __device__ __forceinline__ void group_sync() {
     }
 }
 
-template <typename V>
-__device__ __attribute__((noinline)) V load_glb(const QB_GLB V* glb, uint32_t idx) {
-    return glb[idx];
-}
-
-template <typename V>
-__device__ __attribute__((noinline)) void store_glb(QB_GLB V* glb, uint32_t idx, V v) {
-    glb[idx] = v;
-}
-
-// The global paths are out-of-line calls so that the optimiser cannot fold an
-// LDS access and a global access into one generic (flat) access.
-template <typename V>
-__device__ __forceinline__ V load_loc(uint32_t loc, uint32_t off, const QB_LDS V* lds, const QB_GLB V* glb) {
-    if (loc & kGlobal) return load_glb(glb, (loc & ~kGlobal) + off);
-    return lds[loc + off];
-}
-
 // Global scratch of an executor item through a buffer resource (base = the
 // item's slot, bounds = its size in bytes): out-of-range accesses read zero
 // and drop writes instead of faulting, and buffer instructions stay distinct
@@ -1010,7 +992,9 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
         run_ops<R, ITEM_THREADS, GLB>(prog, st0, plan[1], r0, lds, srd, lane);
         group_sync<ITEM_THREADS>();
         if (lane == 0) {
-            const V r = load_loc((uint32_t)plan[4], 0u, lds, glb);
+            // the result: LDS, or the item's scratch slot (through its buffer)
+            const uint32_t rl = (uint32_t)plan[4];
+            const V r = (rl & kGlobal) ? buf_load(srd, rl & ~kGlobal, (V*)nullptr) : lds[rl];
             item_out[2 * (size_t)slot] = (double)r.x;
             item_out[2 * (size_t)slot + 1] = (double)r.y;
         }
