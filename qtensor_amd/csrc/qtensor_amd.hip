@@ -911,9 +911,9 @@ __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st
     }
 }
 
-// complex64 items keep at most 96 VGPRs, so five waves fit on each SIMD: with
-// 128-thread items and 16 KiB of LDS each, that is the ten blocks per CU the
-// LDS allows.
+// complex64 items keep at most 96 VGPRs, so five waves fit on each SIMD (20
+// per CU): as many items per CU as the LDS of the usual plans allows (for
+// example ten 128-thread items of 16 KiB, or five 256-thread items of 26 KiB).
 template <typename R>
 constexpr int kExecWavesPerEU = sizeof(R) == 4 ? 5 : 1;
 
