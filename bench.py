@@ -459,6 +459,24 @@ def main():
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
     bytes_per_launch = alg_bytes
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    copy_gbs = None
+    if not args.kernel_only:
+        # the device's copy bandwidth beside the datasheet peak: a 1 GiB
+        # device-to-device copy (1 GiB read + 1 GiB written), median of 5
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        cts = []
+        for _ in range(5):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            dst.copy_(src)
+            b.record(stream)
+            torch.cuda.synchronize()
+            cts.append(a.elapsed_time(b))
+        copy_gbs = 2 * (1 << 30) / (float(np.median(cts)) * 1e-3) / 1e9
+        del src, dst
     if args.kernel_only:
         # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings give
         # the scale of those counters (1 GiB read, 1 GiB written)
@@ -509,7 +527,8 @@ def main():
                                                        == EXEC_TRAFFIC_CONFIG) else None,
                      "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
                      "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,
-                     "executed_step_bytes_per_launch": batch.bytes_step_model},
+                     "executed_step_bytes_per_launch": batch.bytes_step_model,
+                     "measured_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1)},
     }
     if args.workload == "c2":            # the extra wide edge belongs to the N=256, p=3 workload
         fe = forced_edge(G, gamma, beta, terms64[0])
