@@ -267,12 +267,22 @@ class LightconeEngine:
         return C, zz
 
 
-def order_items(items, costs):
+def _cu_count(device):
+    return _torch().cuda.get_device_properties(device).multi_processor_count
+
+
+def order_items(items, costs, band=None):
     """Work items in launch order: longest first (by ``costs``, stable), each
-    carrying its output slot (its index in the given order) in word 3."""
+    carrying its output slot (its index in the given order) in word 3.  With
+    ``band`` (the device's CU count), every other run of ``band`` items is
+    reversed: blocks b and b + band tend to share a CU, so each CU then gets a
+    long item of one run with a short one of the next."""
     items = np.array(items, dtype=np.int32).reshape(-1, 4)
     items[:, 3] = np.arange(items.shape[0], dtype=np.int32)
     order = np.argsort(-np.asarray(costs, dtype=np.int64), kind="stable")
+    if band:
+        for k in range(band, len(order), 2 * band):
+            order[k:k + band] = order[k:k + band][::-1].copy()
     return np.ascontiguousarray(items[order])
 
 
@@ -340,7 +350,7 @@ class Batch:
         _lib.check(self.lib.qb_exec_slots(self.shape.item_threads, self.lds_elems, self.qb_dtype,
                                           ctypes.byref(slots)))
         self.n_slots = slots.value
-        items_np = order_items(items, order)
+        items_np = order_items(items, order, band=_cu_count(self.device))
         self._host = (program, items_np, np.asarray(seg, dtype=np.int32), len(params))
         self.program = torch.from_numpy(program).to(dev)
         self.items = torch.from_numpy(np.ascontiguousarray(items_np)).to(dev)

@@ -344,7 +344,8 @@ class EnergyServer:
             costs.append(cost)
             sizes.append(seg_sizes)
             lds_need, glb_need = max(lds_need, ln), max(glb_need, gn)
-        items = order_items(np.concatenate(parts), np.concatenate(costs))
+        items = order_items(np.concatenate(parts), np.concatenate(costs),
+                            band=torch.cuda.get_device_properties(self.device).multi_processor_count)
         params += [0.0] * PARAM_PAD      # the executor reads 8 angles per item at once
         seg = np.concatenate([[0], np.cumsum(np.concatenate(sizes))]).astype(np.int32)
         n_items, n_terms = items.shape[0], seg.size - 1

@@ -381,3 +381,18 @@ def test_native_place_intervals_matches_restatement():
             for j in range(i + 1, m):
                 if starts[i] <= ends[j] and starts[j] <= ends[i]:
                     assert offs[i] + sizes[i] <= offs[j] or offs[j] + sizes[j] <= offs[i]
+
+
+def test_order_items_serpentine_bands():
+    """Longest first; with a band, every other run of ``band`` items reversed;
+    word 3 always carries the item's index in the given order."""
+    import numpy as np
+    from qtensor_amd.batched import order_items
+    items = [(k, 0, 0, 0) for k in range(10)]
+    costs = [5, 9, 1, 7, 3, 8, 2, 6, 4, 0]
+    plain = order_items(items, costs)
+    assert list(plain[:, 0]) == [1, 5, 3, 7, 0, 8, 4, 6, 2, 9]
+    assert list(plain[:, 3]) == list(plain[:, 0])
+    snake = order_items(items, costs, band=4)          # runs [1 5 3 7] [0 8 4 6] [2 9], the second reversed
+    assert list(snake[:, 0]) == [1, 5, 3, 7, 6, 4, 8, 0, 2, 9]
+    assert list(snake[:, 3]) == list(snake[:, 0])
