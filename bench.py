@@ -35,8 +35,8 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs (profiles/exec_kernel_qaoa_p3_n256.md): 13882.2 KiB fetched
 # x 2.0 (a 1 GiB device copy reads as 524306.4 KiB) + 39.25 KiB written x 1.0
 # (the copy's 1 GiB writes read as 1048576 KiB).  Reported only for that
-# configuration: (item threads, slicing threshold, slicer, work items).
-EXEC_TRAFFIC_CONFIG = (256, 10, "tree_trim", 1274)
+# configuration: (item threads, slicing threshold, slicer, order, work items).
+EXEC_TRAFFIC_CONFIG = (256, 10, "tree_trim", "greedy", 1274)
 EXEC_TRAFFIC_BYTES = int(13882.2 * 1024 * 2.0 + 39.25 * 1024 * 1.0)
 
 
@@ -274,14 +274,15 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
 
 
 def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4),
-                slicer="tree_trim", shape4=None, max_tw4=16, slicer4="tree_trim"):
+                slicer="tree_trim", shape4=None, max_tw4=16, slicer4="tree_trim", ordering="greedy"):
     """Poisson stream of energy requests (C5) through the continuous-batching server.
     Depths 1-3 use ``shape`` (default 64-thread items) sliced below treewidth
     ``max_tw``; depth 4 uses ``shape4`` (default 256-thread items) sliced below
     ``max_tw4``, with larger tensors in global scratch; ``slicer`` and
     ``slicer4`` pick their slice variables (tree trimming needs about half
     the work items of max-degree slices on these graphs; with max-degree
-    slices at p = 4 the server saturates at about a third of the rate)."""
+    slices at p = 4 the server saturates at about a third of the rate).
+    ``ordering`` is the lightcones' elimination order at every depth."""
     import networkx as nx
     import numpy as np
     from qtensor_amd.batched import LaunchShape, LightconeEngine
@@ -291,8 +292,10 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
     for k in range(pool_size):
         n = int(rng.integers(10, 1001)) * 2
         pool.append((f"rr3_n{n}_s{k}", nx.random_regular_graph(3, n, seed=100 + k)))
-    eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw, slicer=slicer)
-    eng4 = LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=max_tw4, slicer=slicer4)
+    eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw, slicer=slicer,
+                          ordering=ordering)
+    eng4 = LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=max_tw4, slicer=slicer4,
+                           ordering=ordering)
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
     server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=workers).start()
     cold = {}
@@ -354,6 +357,10 @@ def main():
                     help="with --shape: slice lightcones whose treewidth reaches this value")
     ap.add_argument("--slicer", default="greedy", choices=("greedy", "tree_trim"),
                     help="with --shape: slice-variable rule (max-degree greedy or tree trimming)")
+    ap.add_argument("--ordering", default=None, choices=("greedy", "rgreedy"),
+                    help="elimination order of the lightcones: min-degree greedy, or the best of "
+                         "seeded randomized-greedy samples and the greedy order (default: rgreedy "
+                         "for the batched workloads, greedy for --serve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -372,12 +379,14 @@ def main():
         return
     if args.serve:
         rates = [float(x) for x in args.serve_rates.split(",")]
-        res = serve_bench(rates, args.serve_requests, args.serve_pool)
+        ordering = args.ordering or "greedy"
+        res = serve_bench(rates, args.serve_requests, args.serve_pool, ordering=ordering)
         best = max(res["rates"], key=lambda r: r["terms_per_s"])
         print(json.dumps({"metric": "served edge-energy terms/sec and p99 request latency",
                           "value": best["terms_per_s"], "unit": "terms/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "complex64", "data": "synthetic",
-                          "config": {"workload": "poisson_energy_requests_3reg_n20_2000_p1_4"},
+                          "config": {"workload": "poisson_energy_requests_3reg_n20_2000_p1_4",
+                                     "ordering": ordering},
                           "serving": res}), flush=True)
         return
 
@@ -390,6 +399,7 @@ def main():
                           "forced_edge": fe}), flush=True)
         return
     n_terms = G.number_of_edges()
+    ordering = args.ordering or "rgreedy"
     print(f"[bench] workload {desc}: {n_terms} edge terms", file=sys.stderr, flush=True)
 
     # launch shape and slicing from the autotuner (timed outside the measured region)
@@ -400,14 +410,15 @@ def main():
         it, cap, lim = (int(x) for x in args.shape.split(","))
         shape, max_tw, slicer, table = LaunchShape(it, cap, lim), args.slice_tw, args.slicer, []
     else:
-        tuner = Autotuner(verbose=True, workers=16)
+        tuner = Autotuner(verbose=True, workers=16, ordering=ordering)
         shape, max_tw, slicer, table = tuner.tune(G, p, gamma, beta)
     tune_s = time.perf_counter() - t0
-    print(f"[bench] tuned: {shape.key()} max_tw={max_tw} {slicer} ({tune_s:.1f} s)", file=sys.stderr, flush=True)
+    print(f"[bench] tuned: {shape.key()} max_tw={max_tw} {slicer} {ordering} ({tune_s:.1f} s)",
+          file=sys.stderr, flush=True)
 
     from qtensor_amd.batched import clear_plan_cache
     clear_plan_cache()                  # the autotuner compiled these plans already
-    eng = LightconeEngine(shape=shape, max_tw=max_tw, slicer=slicer)
+    eng = LightconeEngine(shape=shape, max_tw=max_tw, slicer=slicer, ordering=ordering)
     t0 = time.perf_counter()
     wl = eng.workload(G, p)
     plan_cold = time.perf_counter() - t0
@@ -490,7 +501,8 @@ def main():
                           "config": {"workload": desc, "item_threads": shape.item_threads,
                                      "lds_capacity": shape.lds_capacity,
                                      "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw, "slicer": slicer,
-                                     "work_items": batch.n_items, "resident_items": batch.n_slots},
+                                     "ordering": ordering, "work_items": batch.n_items,
+                                     "resident_items": batch.n_slots},
                           "step_model_bytes_per_launch": bytes_per_launch,
                           "terms_per_s": round(value, 1), "calibration_copy_bytes": 2 << 30}), flush=True)
         return
@@ -510,7 +522,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "complex64",
         "data": "synthetic (seeded networkx graph and angles)",
         "config": {"workload": desc, "p": p, "edges": n_terms, "composer": "QtreeQAOAComposer",
-                   "ordering": "greedy", "form": "batched persistent executor, stream launches",
+                   "ordering": ordering, "form": "batched persistent executor, stream launches",
                    "max_treewidth": wl_alg.max_treewidth(), "distinct_plans": len(set(map(id, wl.plans.values()))),
                    "item_threads": shape.item_threads, "lds_capacity": shape.lds_capacity,
                    "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw, "slicer": slicer,
@@ -523,7 +535,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": EXEC_TRAFFIC_BYTES if (args.workload == "c2" and batch.scratch_elems == 0 and
-                                                       (shape.item_threads, max_tw, slicer, batch.n_items)
+                                                       (shape.item_threads, max_tw, slicer, ordering,
+                                                        batch.n_items)
                                                        == EXEC_TRAFFIC_CONFIG) else None,
                      "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
                      "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,

@@ -29,6 +29,8 @@
  *   qb_greedy_order   -> min-degree greedy elimination order (qtensor/utils.py:12-44,
  *                        used by OrderingOptimizer, qtensor/optimisation/Optimizer.py:46-88)
  *   qb_elimination_widths -> widths of a fixed order (the same elimination, utils.py:77-86)
+ *   qb_rgreedy_order  -> randomized greedy orders, best of several samples
+ *                        (RGreedyOptimizer, qtensor/optimisation/RGreedy.py:15-77)
  *   qb_place_intervals -> LDS / scratch offsets of a plan's buffers (the plan compiler's
  *                        own layout step; the reference allocates with numpy)
  *   qb_init / qb_shutdown / qb_last_error -> backend construction and the Python
@@ -213,6 +215,18 @@ int qb_greedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_
 /* Widths of eliminating order[0..n_order) in that order (same elimination rule). */
 int qb_elimination_widths(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, int32_t n_order,
                           const int32_t* order, int32_t* widths);
+
+/*
+ * qb_rgreedy_order (replaces RGreedyOptimizer._get_ordering_ints,
+ * qtensor/optimisation/RGreedy.py:36-77): samples 1..repeats eliminate, at
+ * every step, a remaining node drawn with probability proportional to
+ * exp(-(neighbours - fewest neighbours) / temp) (temp > 0; a splitmix64 stream
+ * seeded by `seed`); sample 0 is the qb_greedy_order elimination.  order and
+ * widths receive the sample with the smallest (largest width, sum of
+ * 2^width), the earliest on ties.  Deterministic for a given seed.
+ */
+int qb_rgreedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, double temp,
+                     uint64_t seed, int32_t repeats, int32_t* order, int32_t* widths);
 
 /*
  * Interval allocation for the plan compiler (host only; no counterpart in the

@@ -23,7 +23,7 @@ QB_ITEM_WORDS = 4
 
 EXPORTED = ("qb_version", "qb_init", "qb_shutdown", "qb_last_error", "qb_step",
             "qb_exec_slots", "qb_execute", "qb_wide_execute", "qb_reduce_terms",
-            "qb_greedy_order", "qb_elimination_widths", "qb_place_intervals",
+            "qb_greedy_order", "qb_elimination_widths", "qb_rgreedy_order", "qb_place_intervals",
             "qb_plan_create", "qb_plan_execute", "qb_plan_destroy")
 
 
@@ -93,6 +93,8 @@ def load():
         i32p = ctypes.POINTER(ctypes.c_int32)
         lib.qb_greedy_order.argtypes = [ctypes.c_int32, i32p, i32p, i32p, i32p]
         lib.qb_elimination_widths.argtypes = [ctypes.c_int32, i32p, i32p, ctypes.c_int32, i32p, i32p]
+        lib.qb_rgreedy_order.argtypes = [ctypes.c_int32, i32p, i32p, ctypes.c_double, ctypes.c_uint64,
+                                         ctypes.c_int32, i32p, i32p]
         i64p = ctypes.POINTER(ctypes.c_int64)
         lib.qb_place_intervals.argtypes = [ctypes.c_int32, i64p, i64p, i64p, i64p, ctypes.c_int64, i64p, i64p]
         lib.qb_wide_execute.argtypes = [ctypes.POINTER(WideDesc), ctypes.c_void_p]

@@ -21,9 +21,9 @@ import numpy as np
 from .batched import LaunchShape, LightconeEngine
 
 
-def _signature(G, p, composer, dtype):
+def _signature(G, p, composer, dtype, ordering):
     degs = tuple(sorted(d for _, d in G.degree()))
-    return (G.number_of_nodes(), G.number_of_edges(), hash(degs), p, composer.__name__, dtype)
+    return (G.number_of_nodes(), G.number_of_edges(), hash(degs), p, composer.__name__, dtype, ordering)
 
 
 BASE_SHAPE = LaunchShape(128, 4096, 2048)
@@ -41,7 +41,7 @@ def slicing_candidates(max_tw):
     """Slicing thresholds worth timing for a workload whose greedy lightcone
     treewidth reaches ``max_tw`` (None = no slicing)."""
     slicing = [None]
-    for t in (13, 12, 11, 10, 9):
+    for t in (13, 12, 11, 10, 9, 8):
         if t < max_tw:
             slicing.append(t)
     return slicing
@@ -58,9 +58,11 @@ def candidates(max_tw):
 
 
 class Autotuner:
-    def __init__(self, composer=None, dtype="complex64", reps=10, device=0, verbose=False, workers=None):
+    def __init__(self, composer=None, dtype="complex64", reps=10, device=0, verbose=False, workers=None,
+                 ordering="greedy"):
         from .composer import QtreeQAOAComposer
         self.composer = composer or QtreeQAOAComposer
+        self.ordering = ordering        # LightconeEngine's ``ordering`` for every candidate
         self.workers = workers          # planning processes per candidate (None: in-process)
         self.dtype = dtype
         self.reps = reps
@@ -71,7 +73,7 @@ class Autotuner:
     def time_config(self, G, p, gamma, beta, shape, max_tw, slicer="greedy"):
         import torch
         eng = LightconeEngine(composer=self.composer, device=self.device, dtype=self.dtype,
-                              shape=shape, max_tw=max_tw, slicer=slicer)
+                              shape=shape, max_tw=max_tw, slicer=slicer, ordering=self.ordering)
         t0 = time.perf_counter()
         wl = eng.workload(G, p, workers=self.workers)
         plan_s = time.perf_counter() - t0
@@ -94,7 +96,7 @@ class Autotuner:
         """Return ``(shape, max_tw, slicer, table)`` of the fastest candidate;
         ``cands`` (optional) lists (shape, max_tw, slicer) to time instead of
         the staged search."""
-        key = _signature(G, p, self.composer, self.dtype)
+        key = _signature(G, p, self.composer, self.dtype, self.ordering)
         if key in self.memo:
             return self.memo[key]
         table = []
@@ -102,7 +104,7 @@ class Autotuner:
         if staged:
             # stage 1: slicing thresholds and slicers under the base shape;
             # stage 2: the other shapes at the best (threshold, slicer) pairs
-            probe = LightconeEngine(composer=self.composer, dtype=self.dtype)
+            probe = LightconeEngine(composer=self.composer, dtype=self.dtype, ordering=self.ordering)
             max_tw = probe.workload(G, p).max_treewidth()
             cands = [(BASE_SHAPE, m, sl) for m, sl in _with_slicers(slicing_candidates(max_tw))]
         best = None

@@ -24,7 +24,8 @@ import numpy as np
 from . import _lib
 from .composer import QtreeQAOAComposer
 from .lightcone import canonical_graph, canonical_lightcone
-from .ordering import OrderingOptimizer, SlicesOptimizer, TreeTrimSplitter
+from .ordering import (OrderingOptimizer, RGreedyOrderingOptimizer, RGreedySlicesOptimizer,
+                       RGreedyTreeTrimSplitter, SlicesOptimizer, TreeTrimSplitter)
 from .plan import PARAM_PAD, PROGRAM_PAD, PlanCompiler
 
 
@@ -117,10 +118,10 @@ def _keys_task(args):
 
 
 def _plan_task(args):
-    composer_name, dtype, shape, max_tw, tw_bias, slicer, tasks = args
+    composer_name, dtype, shape, max_tw, tw_bias, slicer, ordering, tasks = args
     from . import composer as _composer
     eng = LightconeEngine(composer=getattr(_composer, composer_name), dtype=dtype, shape=shape,
-                          max_tw=max_tw, tw_bias=tw_bias, slicer=slicer)
+                          max_tw=max_tw, tw_bias=tw_bias, slicer=slicer, ordering=ordering)
     out = []
     for key, n, ce, i, j, p in tasks:
         try:
@@ -138,13 +139,20 @@ class LightconeEngine:
     treewidth reaches it are sliced into 2^s work items, the slice variables
     chosen by ``slicer``: "greedy" (``SlicesOptimizer``, max-degree variables,
     qtensor/optimisation/Optimizer.py:90-159) or "tree_trim"
-    (``TreeTrimSplitter``, Optimizer.py:178-228).
+    (``TreeTrimSplitter``, Optimizer.py:178-228).  ``ordering``: "greedy"
+    (min-degree, the reference's default ``OrderingOptimizer``) or "rgreedy"
+    (best of 16 seeded randomized-greedy samples and the greedy order,
+    ``qtensor/optimisation/RGreedy.py:15-77``), also the order the slicer
+    works from.  Orders change the work, not the values (up to rounding).
     """
 
     def __init__(self, composer=QtreeQAOAComposer, device=0, dtype="complex64", shape=None,
-                 max_tw=None, tw_bias=0, slicer="greedy"):
+                 max_tw=None, tw_bias=0, slicer="greedy", ordering="greedy"):
         if slicer not in ("greedy", "tree_trim"):
             raise ValueError(f"unknown slicer {slicer!r}")
+        if ordering not in ("greedy", "rgreedy"):
+            raise ValueError(f"unknown ordering {ordering!r}")
+        self.ordering = ordering
         self.composer = composer
         self.device_index = device
         self.dtype = dtype
@@ -157,11 +165,17 @@ class LightconeEngine:
 
     # -- host planning -----------------------------------------------------
     def _optimizer(self):
+        rg = self.ordering == "rgreedy"
         if self.max_tw is None:
-            return OrderingOptimizer()
+            return RGreedyOrderingOptimizer() if rg else OrderingOptimizer()
         if self.slicer == "tree_trim":
-            return TreeTrimSplitter(max_tw=self.max_tw, tw_bias=self.tw_bias)
-        return SlicesOptimizer(max_tw=self.max_tw, tw_bias=self.tw_bias)
+            cls = RGreedyTreeTrimSplitter if rg else TreeTrimSplitter
+        else:
+            cls = RGreedySlicesOptimizer if rg else SlicesOptimizer
+        return cls(max_tw=self.max_tw, tw_bias=self.tw_bias)
+
+    def _cache_key(self, key, p, shape):
+        return (key, p, shape.key(), self.composer.__name__, self.max_tw, self.slicer, self.ordering)
 
     def shape_for(self, p):
         if self.shape is not None:
@@ -169,7 +183,7 @@ class LightconeEngine:
         return DEFAULT_SHAPES["small"] if p <= 2 else DEFAULT_SHAPES["large"]
 
     def plan(self, key, n, edges, i, j, p, shape):
-        ck = (key, p, shape.key(), self.composer.__name__, self.max_tw, self.slicer)
+        ck = self._cache_key(key, p, shape)
         with self._lock:
             hit = self._plans.get(ck)
         if hit is not None:
@@ -228,7 +242,7 @@ class LightconeEngine:
         todo, seen = [], set()
         for e in edges:
             key, n, ce, i, j = keyed[e]
-            ck = (key, p, shape.key(), self.composer.__name__, self.max_tw, self.slicer)
+            ck = self._cache_key(key, p, shape)
             if key in seen:
                 continue
             seen.add(key)
@@ -239,13 +253,13 @@ class LightconeEngine:
         if todo:
             chunk = max(1, -(-len(todo) // (2 * workers)))
             jobs = [(self.composer.__name__, self.dtype, shape, self.max_tw, self.tw_bias, self.slicer,
-                     todo[k:k + chunk])
+                     self.ordering, todo[k:k + chunk])
                     for k in range(0, len(todo), chunk)]
             for part in pool.map(_plan_task, jobs):
                 for key, plan, err in part:
                     if plan is not None:
                         with self._lock:
-                            self._plans[(key, p, shape.key(), self.composer.__name__, self.max_tw, self.slicer)] = plan
+                            self._plans[self._cache_key(key, p, shape)] = plan
         if not hasattr(self, "_keyed"):
             self._keyed = {}
         self._keyed[(id(G), p)] = keyed

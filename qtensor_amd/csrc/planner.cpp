@@ -10,6 +10,7 @@
 // widths of that order.  Plain host code: no device work.
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <vector>
 
@@ -100,6 +101,75 @@ int qb_greedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_
         alive[best] = 0;
         order[step] = best;
         widths[step] = g.eliminate(best);
+    }
+    return QB_OK;
+}
+
+// Randomized greedy orderings, the rule of qtensor/optimisation/RGreedy.py:15-77
+// (RGreedyOptimizer): at every step pick a remaining node with probability
+// proportional to exp(-(neighbours - fewest neighbours) / temp).  Sample 0 is
+// the min-degree greedy order itself; samples 1..repeats draw from a
+// splitmix64 stream seeded by `seed`.  The kept sample is the one with the
+// smallest (largest width, sum of 2^width), the earliest on ties: the
+// reference keeps the smallest largest width; the sum breaks its ties by the
+// total size of the step results.
+int qb_rgreedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, double temp,
+                     uint64_t seed, int32_t repeats, int32_t* order, int32_t* widths) {
+    if (n_nodes < 0 || repeats < 0 || !(temp > 0.0) || (n_nodes > 0 && (!adj_ptr || !adj_idx || !order || !widths)))
+        return qb_internal::fail(QB_ERR_INVALID, "qb_rgreedy_order: bad node count, repeats, temp or null pointer");
+    Graph g0;
+    if (!g0.load(n_nodes, adj_ptr, adj_idx))
+        return qb_internal::fail(QB_ERR_INVALID, "qb_rgreedy_order: malformed adjacency");
+    uint64_t state = seed;
+    auto uniform = [&state]() {
+        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    std::vector<int32_t> cur_order(n_nodes), cur_width(n_nodes);
+    std::vector<double> weight(n_nodes);
+    int32_t best_max = -1;
+    double best_sum = 0.0;
+    for (int32_t sample = 0; sample <= repeats; ++sample) {
+        Graph g = g0;
+        std::vector<char> alive(n_nodes, 1);
+        int32_t w_max = 0;
+        double w_sum = 0.0;
+        for (int32_t step = 0; step < n_nodes; ++step) {
+            size_t lo = SIZE_MAX;
+            int32_t first = -1;
+            for (int32_t v = 0; v < n_nodes; ++v)
+                if (alive[v] && g.nb[v].size() < lo) { lo = g.nb[v].size(); first = v; }
+            int32_t pick = first;
+            if (sample > 0) {
+                double total = 0.0;
+                for (int32_t v = 0; v < n_nodes; ++v) {
+                    weight[v] = alive[v] ? std::exp(-(double)(g.nb[v].size() - lo) / temp) : 0.0;
+                    total += weight[v];
+                }
+                const double r = uniform() * total;
+                double acc = 0.0;
+                for (int32_t v = 0; v < n_nodes; ++v) {
+                    if (!alive[v]) continue;
+                    acc += weight[v];
+                    pick = v;
+                    if (acc > r) break;
+                }
+            }
+            alive[pick] = 0;
+            cur_order[step] = pick;
+            cur_width[step] = g.eliminate(pick);
+            w_max = std::max(w_max, cur_width[step]);
+            w_sum += std::ldexp(1.0, cur_width[step]);
+        }
+        if (best_max < 0 || w_max < best_max || (w_max == best_max && w_sum < best_sum)) {
+            best_max = w_max;
+            best_sum = w_sum;
+            std::copy(cur_order.begin(), cur_order.end(), order);
+            std::copy(cur_width.begin(), cur_width.end(), widths);
+        }
     }
     return QB_OK;
 }

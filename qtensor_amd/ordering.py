@@ -17,6 +17,9 @@
 * ``RGreedyOptimizer``   -- ``qtensor/optimisation/RGreedy.py:15-77``: pick the next
   node with probability proportional to exp(-(degree - min degree)/temp), keep the
   narrowest of ``repeats`` samples (seeded here).
+* ``RGreedyOrdering``    -- the same rule in the native planner (``qb_rgreedy_order``),
+  best of the samples and the greedy order, as the order under a slicer
+  (``RGreedySlicesOptimizer``, ``RGreedyTreeTrimSplitter``) or alone.
 """
 import ctypes
 import os
@@ -63,6 +66,21 @@ def greedy_peo(graph):
     lib = _lib.load()
     _lib.check(lib.qb_greedy_order(n, _i32p(ptr), _i32p(idx if idx.size else np.zeros(1, np.int32)),
                                    _i32p(order), _i32p(widths)))
+    return [nodes[k] for k in order[:n]], [int(w) for w in widths[:n]]
+
+
+def rgreedy_best(graph, temp=0.002, repeats=16, seed=0):
+    """``(peo, degrees)`` of the best of ``repeats`` randomized-greedy samples and
+    the greedy order (``qb_rgreedy_order``): the rule of
+    ``qtensor/optimisation/RGreedy.py:36-77``, kept by (largest width, sum of
+    2^width).  Deterministic for a given ``seed``."""
+    nodes, ptr, idx = _csr(graph)
+    n = len(nodes)
+    order = np.zeros(max(n, 1), dtype=np.int32)
+    widths = np.zeros(max(n, 1), dtype=np.int32)
+    lib = _lib.load()
+    _lib.check(lib.qb_rgreedy_order(n, _i32p(ptr), _i32p(idx if idx.size else np.zeros(1, np.int32)),
+                                    float(temp), int(seed), int(repeats), _i32p(order), _i32p(widths)))
     return [nodes[k] for k in order[:n]], [int(w) for w in widths[:n]]
 
 
@@ -184,6 +202,30 @@ class SlicesOptimizer(OrderingOptimizer):
         return self.peo, self.parallel_vars, tensor_net
 
 
+class RGreedyOrdering:
+    """Ordering mixin: every order (the first one and each one after a slice
+    variable is removed) is the best of ``repeats`` randomized-greedy samples
+    and the greedy order (``rgreedy_best``, seeded), so that slicing starts
+    from a narrower order.  Combined with a slicer the way the reference
+    combines orderers and slicers (``TamakiTrimSlicing``,
+    ``qtensor/optimisation/Optimizer.py:227``)."""
+
+    temp = 0.002
+    repeats = 16
+    seed = 0
+
+    def _get_ordering_ints(self, graph):
+        return rgreedy_best(graph, self.temp, self.repeats, self.seed)
+
+
+class RGreedyOrderingOptimizer(RGreedyOrdering, OrderingOptimizer):
+    pass
+
+
+class RGreedySlicesOptimizer(RGreedyOrdering, SlicesOptimizer):
+    pass
+
+
 def _trim_costs(graph, order):
     """Tree-trimming cost of each node for an elimination order: the sum, over
     the elimination steps whose neighbourhood (with the eliminated node)
@@ -232,6 +274,10 @@ class TreeTrimSplitter(SlicesOptimizer):
             self.slice_history.append((len(result), tw))
         self.treewidth = tw
         return peo_ints, result
+
+
+class RGreedyTreeTrimSplitter(RGreedyOrdering, TreeTrimSplitter):
+    pass
 
 
 class TamakiOptimizer(OrderingOptimizer):

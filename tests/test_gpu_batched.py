@@ -56,17 +56,19 @@ def test_p3_vs_oracle(gpu, dtype, tol):
         assert abs(z - ref) <= tol * max(1.0, abs(ref))
 
 
-@pytest.mark.parametrize("slicer", ["greedy", "tree_trim"])
-def test_sliced_items_sum_to_unsliced(gpu, slicer):
-    """Sliced work items (max-degree or tree-trimming slice variables) sum to
-    the unsliced terms (the sliced == unsliced invariant of
-    tests/test_qaoa_energy.py:106-121)."""
+@pytest.mark.parametrize("slicer,ordering", [("greedy", "greedy"), ("tree_trim", "greedy"),
+                                             ("greedy", "rgreedy"), ("tree_trim", "rgreedy")])
+def test_sliced_items_sum_to_unsliced(gpu, slicer, ordering):
+    """Sliced work items (max-degree or tree-trimming slice variables, under
+    the greedy or the randomized-greedy order) sum to the unsliced greedy-order
+    terms (the sliced == unsliced invariant of tests/test_qaoa_energy.py:106-121;
+    orders change the work, not the values)."""
     G = nx.random_regular_graph(3, 40, seed=7)
     g, b = [0.3, 0.7, 0.2], [0.5, 0.1, 0.9]
     edges = list(G.edges())[:4]
     shape = LaunchShape(256, 8192, 512)
     _, ref = LightconeEngine(shape=shape, dtype="complex128").energy(G, g, b, edges=edges)
-    eng = LightconeEngine(shape=shape, max_tw=9, dtype="complex128", slicer=slicer)
+    eng = LightconeEngine(shape=shape, max_tw=9, dtype="complex128", slicer=slicer, ordering=ordering)
     wl = eng.workload(G, 3, edges)
     assert any(w.n_slices > 1 for w in wl.works)
     batch = eng.batch([(wl, g, b)])
@@ -209,9 +211,9 @@ def test_autotuner_pick_matches_statevector(gpu):
     from qtensor_amd.autotune import Autotuner
     G = nx.random_regular_graph(3, 14, seed=11)
     g, b = [0.4, 0.9], [0.3, 0.6]
-    shape, max_tw, slicer, table = Autotuner(reps=3).tune(G, 2, g, b)
+    shape, max_tw, slicer, table = Autotuner(reps=3, ordering="rgreedy").tune(G, 2, g, b)
     assert table and any("exec_ms" in row for row in table)
-    C, zz = LightconeEngine(shape=shape, max_tw=max_tw, slicer=slicer).energy(G, g, b)
+    C, zz = LightconeEngine(shape=shape, max_tw=max_tw, slicer=slicer, ordering="rgreedy").energy(G, g, b)
     sv = statevector.zz_expectations(G, g, b)
     for e, z in zip(G.edges(), zz):
         assert abs(z - sv[e]) <= _tol(sv[e])

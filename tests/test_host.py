@@ -358,6 +358,51 @@ def test_native_greedy_matches_oracle():
         elimination_widths(nx.path_graph(3), [0, 0])
 
 
+def test_native_rgreedy_rules():
+    """qb_rgreedy_order: with no samples it is the greedy order of the oracle;
+    with samples, the kept order is a permutation whose widths are the oracle's
+    widths of that order, no worse than greedy by (largest width, sum of
+    2^width), and at a temperature whose non-minimal weights underflow to 0
+    every pick has the fewest neighbours (the rule of
+    qtensor/optimisation/RGreedy.py:47-62); the same seed gives the same order."""
+    from oracle import ordering as oo
+    from qtensor_amd.composer import QtreeQAOAComposer
+    from qtensor_amd.ordering import rgreedy_best
+    from qtensor_amd.tensornet import QtreeTensorNet
+    rng = np.random.default_rng(17)
+    graphs = [nx.Graph(), nx.complete_graph(5)]
+    for k in range(12):
+        graphs.append(nx.gnp_random_graph(int(rng.integers(2, 50)), float(rng.uniform(0.05, 0.3)),
+                                          seed=int(rng.integers(1 << 30))))
+    G = nx.random_regular_graph(3, 16, seed=3)
+    for e in list(G.edges())[:3]:
+        comp = QtreeQAOAComposer(G, gamma=[0.1, 0.2], beta=[0.3, 0.4])
+        comp.energy_expectation_lightcone(e)
+        graphs.append(QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph())
+
+    def cost(w):
+        return (max(w) if w else 0, sum(2.0 ** x for x in w))
+
+    for g in graphs:
+        assert rgreedy_best(g, repeats=0) == oo.greedy_peo(g)
+        peo, w = rgreedy_best(g, temp=1e-3, repeats=8, seed=5)
+        assert sorted(peo, key=str) == sorted(g.nodes(), key=str)
+        assert w == oo.elimination_widths(g, peo)
+        assert cost(w) <= cost(oo.greedy_peo(g)[1])
+        assert rgreedy_best(g, temp=1e-3, repeats=8, seed=5) == (peo, w)
+        adj = {n: set(g[n]) - {n} for n in g.nodes()}
+        for v in peo:
+            assert len(adj[v]) == min(len(s) for s in adj.values())
+            nb = adj.pop(v)
+            for u in nb:
+                adj[u] |= nb
+                adj[u] -= {u, v}
+    with pytest.raises(ValueError):
+        rgreedy_best(nx.path_graph(3), temp=0.0)
+    with pytest.raises(ValueError):
+        rgreedy_best(nx.path_graph(3), repeats=-1)
+
+
 def test_native_place_intervals_matches_restatement():
     """qb_place_intervals against the plain-Python allocator (tests/place_ref.py)
     on random buffer sets, and directly: buffers whose op intervals overlap get
