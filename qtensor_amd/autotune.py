@@ -2,8 +2,9 @@
 
 For one (graph, p) workload it compiles the lightcone plans under each
 candidate (item threads, LDS capacity, LDS tensor limit, slicing treewidth,
-slice-variable rule) and times the executor kernel on the device with HIP
-events, keeping the fastest.  The search runs in two stages: every slicing
+slice-variable rule) and times one evaluation (the executor launch and the
+term reductions, as a caller runs it) on the device with HIP events, keeping
+the fastest.  The search runs in two stages: every slicing
 threshold below the workload's largest greedy treewidth, with each slicer,
 under a base shape; then the other shapes (item width, LDS budget per item)
 at the three best (threshold, slicer) pairs.
@@ -78,14 +79,14 @@ class Autotuner:
         wl = eng.workload(G, p, workers=self.workers)
         plan_s = time.perf_counter() - t0
         batch = eng.batch([(wl, gamma, beta)])
-        batch.launch_exec()
+        batch.launch()
         torch.cuda.synchronize()
         st = torch.cuda.current_stream()
         evs = []
         for _ in range(self.reps):
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record(st)
-            batch.launch_exec(st)
+            batch.launch(st)
             b.record(st)
             evs.append((a, b))
         torch.cuda.synchronize()
@@ -117,7 +118,7 @@ class Autotuner:
             except ValueError as exc:      # shape cannot hold these plans
                 table.append({"shape": asdict(shape), "max_tw": mtw, "slicer": sl, "error": str(exc)})
                 return
-            table.append({"shape": asdict(shape), "max_tw": mtw, "slicer": sl, "exec_ms": ms, "plan_s": plan_s})
+            table.append({"shape": asdict(shape), "max_tw": mtw, "slicer": sl, "eval_ms": ms, "plan_s": plan_s})
             timed.append((ms, (mtw, sl)))
             if self.verbose:
                 print(f"[autotune] {shape.key()} max_tw={mtw} {sl}: {ms:.4f} ms", file=sys.stderr, flush=True)
