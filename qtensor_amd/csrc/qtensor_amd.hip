@@ -556,26 +556,28 @@ __device__ __forceinline__ int32_t fetch_rec(__amdgpu_buffer_rsrc_t prog, uint32
     return __builtin_amdgcn_raw_buffer_load_b32(prog, (uint32_t)(lane & 63) * 4u, st * 4u, 0);
 }
 
-// A lane's words of a gather op's lane table (lanes past the active ones read
-// the last active lane's words and do not use them).
+// A lane's words of a gather op's lane table: entry l holds NI + 1 words.
+static_assert(kExecMaxIn == 4, "fetch_lane reads four input words per lane");
 struct GatherLane {
     uint32_t w[kExecMaxIn + 1];
 };
 
-// Every block is read whatever the op's input count (blocks past the table
-// belong to later records or lie past the program, where reads return 0);
-// the gather op uses blocks 0..NI only.
+// Lane l reads five words from its entry's start whatever the op's input
+// count (one 16-byte and one 4-byte load; words past the entry belong to later
+// entries or records, or to the zero words after the last plan, and are not
+// used); lanes past the table read later records or, past the program, zeros,
+// and skip the op.
 __device__ __forceinline__ void fetch_lane(__amdgpu_buffer_rsrc_t prog, uint32_t st, int32_t r, int lane,
                                            GatherLane& g) {
     const uint32_t n = QB_RL(r, 1);
     // a wave none of whose lanes is active loads nothing (the op skips it)
     if (((uint32_t)__builtin_amdgcn_readfirstlane(lane) & ~63u) >= n) return;
-    const uint32_t l = (uint32_t)lane < n ? (uint32_t)lane : n - 1u;
-    const uint32_t vo = l * 4u;
     const uint32_t so = (st + QB_RL(r, 2)) * 4u;
+    const uint32_t vo = __umul24((uint32_t)lane, (QB_RL(r, 3) + 1u) * 4u);   // lane < 2^10
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(prog, vo, so, 0);
 #pragma unroll
-    for (int j = 0; j <= kExecMaxIn; ++j)
-        g.w[j] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(prog, vo, so + (uint32_t)j * n * 4u, 0);
+    for (int j = 0; j < 4; ++j) g.w[j] = (uint32_t)a[j];
+    g.w[4] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(prog, vo + 16u, so, 0);
 }
 
 // One gather op: lane l computes its table entry's element,

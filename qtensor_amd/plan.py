@@ -893,11 +893,12 @@ class PlanCompiler:
         return words
 
     def _encode_gather(self, members):
-        """Gather op: a 4-word header, then a lane table of NI + 1 blocks of
-        ``n_act`` words (NI = the members' largest input count).  Lane l of the
-        op computes one output element of one member: block 0 holds its output
-        element, block 1 + j the LDS elements of input j at the summed
-        variable's 0 (low half) and 1 (high half).  A member without a sum reads
+        """Gather op: a 4-word header, then a lane table of ``n_act`` entries of
+        NI + 1 words (NI = the members' largest input count), entry l at word
+        l * (NI + 1) of the table.  Lane l of the op computes one output element
+        of one member: word 0 of its entry holds the output element, word 1 + j
+        the LDS elements of input j at the summed variable's 0 (low half) and 1
+        (high half).  A member without a sum reads
         element LOC_ZERO as its first input's high half, so its second product
         is zero; missing inputs read LOC_ONE (1 + 0i)."""
         ni = max(len(st.inputs) for st in members)
@@ -924,8 +925,7 @@ class PlanCompiler:
         assert 1 <= n_act <= self.lanes
         length = GATHER_HDR + (ni + 1) * n_act
         words = [OP_GATHER | len(members) | (length << 16), n_act, GATHER_HDR, ni]
-        for c in cols:
-            words += [_i32(v) for v in c]
+        words += [_i32(v) for entry in zip(*cols) for v in entry]
         return words
 
     def _leaf(self, t, data_dict, fixed, par_index, dst):

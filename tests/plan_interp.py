@@ -164,8 +164,8 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
             n_steps, n_act, tab, ni = w0 & 0x7FFF, w[pos + 1], w[pos + 2], w[pos + 3]
             assert 1 <= n_steps <= n_act <= lanes and tab == P.GATHER_HDR
             assert 1 <= ni <= P.MAX_IN and length == P.GATHER_HDR + (ni + 1) * n_act
-            cols = [[x & 0xFFFFFFFF for x in w[pos + tab + c * n_act: pos + tab + (c + 1) * n_act]]
-                    for c in range(ni + 1)]
+            table = [x & 0xFFFFFFFF for x in w[pos + tab: pos + tab + (ni + 1) * n_act]]
+            cols = [table[c::ni + 1] for c in range(ni + 1)]      # entry l = words l*(NI+1) ...
             results = []
             for lane in range(n_act):
                 x, y = 1 + 0j, 1 + 0j
