@@ -378,6 +378,9 @@ constexpr int kStepHdr = 3;
 constexpr int kExecMaxIn = 4;
 constexpr uint32_t kGlobal = 0x80000000u;
 constexpr uint32_t kOpGather = 1u << 15;
+// LDS elements every item holds (written before its ops; LOC_ONE / LOC_ZERO in
+// qtensor_amd/plan.py): 1 + 0i and 0 + 0i
+constexpr uint32_t kLocOne = 0u, kLocZero = 1u;
 constexpr int kGatherHdr = 4;
 // gate kinds (shared with qtensor_amd/gates.py and qtensor_amd/plan.py)
 enum GateKind : int32_t {
@@ -389,22 +392,22 @@ enum GateKind : int32_t {
 //  0 kind, 1 dst (LDS element offset), 2 n_elems, 3 src_map (4 bits per element),
 //  4 j0, 5 c0 (float bits), 6 j1, 7 c1 (float bits), 8-9 alpha0 (double bits),
 //  10 const_off, 11 n_slice | var0 << 8 | var1 << 16, 12 stride0, 13 stride1
+// One sincospi evaluation serves every kind (the lanes of a wave build gates
+// of different kinds; per-kind calls would run one after another).
 __device__ void natural_gate(int32_t kind, double alpha, double* re, double* im) {
     for (int e = 0; e < 8; ++e) { re[e] = 0.0; im[e] = 0.0; }
     const double h = 0.70710678118654752440;
+    const bool half = kind == kXPhase || kind == kYPhase || kind == kZZ;
+    double s, c;
+    sincospi(half ? alpha * 0.5 : alpha, &s, &c);
     switch (kind) {
-    case kZPhase: {
-        double s, c;
-        sincospi(alpha, &s, &c);
+    case kZPhase:
         re[0] = 1.0; re[1] = c; im[1] = s;
         break;
-    }
     case kXPhase:
     case kYPhase: {
         // e^{i pi a/2} [[c, -i s], [-i s, c]]  /  e^{i pi a/2} [[c, -s], [s, c]]
-        double s, c, gs, gc;
-        sincospi(alpha * 0.5, &s, &c);
-        gs = s; gc = c;  // global phase e^{i pi a/2} = gc + i gs
+        const double gs = s, gc = c;  // global phase e^{i pi a/2} = gc + i gs
         re[0] = gc * c; im[0] = gs * c;
         re[3] = gc * c; im[3] = gs * c;
         if (kind == kXPhase) {
@@ -417,15 +420,12 @@ __device__ void natural_gate(int32_t kind, double alpha, double* re, double* im)
         }
         break;
     }
-    case kZZ: {
-        double s, c;
-        sincospi(alpha * 0.5, &s, &c);
+    case kZZ:
         re[0] = c; im[0] = -s;  // m
         re[1] = c; im[1] = s;   // p
         re[2] = c; im[2] = s;
         re[3] = c; im[3] = -s;
         break;
-    }
     case kH:
         re[0] = h; re[1] = h; re[2] = h; re[3] = -h;
         break;
@@ -641,9 +641,16 @@ __device__ __forceinline__ void reduce_op(int32_t r, uint32_t w0, QB_LDS typenam
     const uint32_t o = (uint32_t)lane & (n_out - 1u);
     const uint32_t g = (uint32_t)lane >> ro;
     const uint32_t step = ro < 6u ? 64u >> ro : 1u;   // plans keep rank_out <= 6
+    // terms and step are powers of two: every lane makes terms / step trips,
+    // or one when terms < step, where lanes past the terms add element
+    // LOC_ZERO (0 + 0i) -- a uniform loop with no per-lane exit
+    const uint32_t trips = terms >= step ? terms / step : 1u;
     V s = V{0, 0};
 #pragma unroll 4
-    for (uint32_t x = g; x < terms; x += step) s = cadd(s, lds[in_loc + o + x * n_out]);
+    for (uint32_t t = 0; t < trips; ++t) {
+        const uint32_t x = g + t * step;
+        s = cadd(s, lds[x < terms ? in_loc + o + x * n_out : kLocZero]);
+    }
     for (uint32_t b = n_out; b < 64u; b <<= 1) {
         s.x += __shfl_xor(s.x, (int)b, 64);
         s.y += __shfl_xor(s.y, (int)b, 64);
@@ -974,8 +981,8 @@ __global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
             // spilled under the register cap
             R one = 1, zero = 0;
             asm volatile("" : "+v"(one), "+v"(zero));
-            lds[0] = V{one, zero};
-            lds[1] = V{zero, zero};
+            lds[kLocOne] = V{one, zero};
+            lds[kLocZero] = V{zero, zero};
         }
         double th[kParamRegs];
 #pragma unroll
