@@ -274,7 +274,8 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
 
 
 def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4),
-                slicer="tree_trim", shape4=None, max_tw4=16, slicer4="tree_trim", ordering="greedy"):
+                slicer="tree_trim", shape4=None, max_tw4=16, slicer4="tree_trim", ordering="greedy",
+                ordering4="greedy"):
     """Poisson stream of energy requests (C5) through the continuous-batching server.
     Depths 1-3 use ``shape`` (default 64-thread items) sliced below treewidth
     ``max_tw``; depth 4 uses ``shape4`` (default 256-thread items) sliced below
@@ -282,7 +283,8 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
     ``slicer4`` pick their slice variables (tree trimming needs about half
     the work items of max-degree slices on these graphs; with max-degree
     slices at p = 4 the server saturates at about a third of the rate).
-    ``ordering`` is the lightcones' elimination order at every depth."""
+    ``ordering`` and ``ordering4`` are the lightcones' elimination orders at
+    depths 1-3 and at depth 4."""
     import networkx as nx
     import numpy as np
     from qtensor_amd.batched import LaunchShape, LightconeEngine
@@ -295,7 +297,7 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
     eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw, slicer=slicer,
                           ordering=ordering)
     eng4 = LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=max_tw4, slicer=slicer4,
-                           ordering=ordering)
+                           ordering=ordering4)
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
     server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=workers).start()
     cold = {}
@@ -361,6 +363,9 @@ def main():
                     help="elimination order of the lightcones: min-degree greedy, or the best of "
                          "seeded randomized-greedy samples and the greedy order (default: rgreedy "
                          "for the batched workloads, greedy for --serve)")
+    ap.add_argument("--serve-ordering4", default="greedy", choices=("greedy", "rgreedy"),
+                    help="with --serve: elimination order of the depth-4 requests (--ordering sets "
+                         "depths 1-3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -380,13 +385,14 @@ def main():
     if args.serve:
         rates = [float(x) for x in args.serve_rates.split(",")]
         ordering = args.ordering or "greedy"
-        res = serve_bench(rates, args.serve_requests, args.serve_pool, ordering=ordering)
+        res = serve_bench(rates, args.serve_requests, args.serve_pool, ordering=ordering,
+                          ordering4=args.serve_ordering4)
         best = max(res["rates"], key=lambda r: r["terms_per_s"])
         print(json.dumps({"metric": "served edge-energy terms/sec and p99 request latency",
                           "value": best["terms_per_s"], "unit": "terms/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "complex64", "data": "synthetic",
                           "config": {"workload": "poisson_energy_requests_3reg_n20_2000_p1_4",
-                                     "ordering": ordering},
+                                     "ordering": ordering, "ordering_p4": args.serve_ordering4},
                           "serving": res}), flush=True)
         return
 
