@@ -149,7 +149,8 @@ def test_plan_handle_matches_batch(gpu, shape, max_tw):
     h.close()
 
 
-@pytest.mark.parametrize("n_seg,max_len", [(1, 1), (384, 12), (1500, 3), (3000, 40), (9000, 2)])
+@pytest.mark.parametrize("n_seg,max_len", [(1, 1), (384, 12), (1500, 3), (3000, 40), (4096, 5), (4097, 1),
+                                           (9000, 2)])
 def test_reduce_terms_vs_fsum(gpu, n_seg, max_len):
     """qb_reduce_terms (compensated FP64 sums of item results per term, then of
     the terms) against math.fsum, for segment counts from 1 to 9000 and
