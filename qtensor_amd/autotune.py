@@ -7,7 +7,9 @@ term reductions, as a caller runs it) on the device with HIP events, keeping
 the fastest.  The search runs in two stages: every slicing
 threshold below the workload's largest greedy treewidth, with each slicer,
 under a base shape; then the other shapes (item width, LDS budget per item)
-at the three best (threshold, slicer) pairs.
+at the three best (threshold, slicer) pairs.  Every candidate is compiled under
+the tuner's elimination order (``ordering``: "greedy" or "rgreedy", as in
+LightconeEngine).
 
 Results are memoised per (graph signature, p, composer, dtype) in-process and
 can be saved to / loaded from a JSON file.
