@@ -362,9 +362,9 @@ __global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a) {
 // Gather op (several small steps, one output element per lane, all in LDS):
 //   w0 = kOpGather | n_steps | len << 16, w1 = active lanes n, w2 = lane table
 //   offset, w3 = NI, the largest input count of its steps
-//   lane table: NI + 1 blocks of n words; lane l reads word l of each block:
-//     block 0: the LDS element it writes; block 1 + j: the LDS elements of input
-//     j at the summed variable's 0 (bits 0-15) and 1 (bits 16-31).  A step
+//   lane table: n entries of NI + 1 words, entry l at word l * (NI + 1):
+//     word 0: the LDS element lane l writes; word 1 + j: the LDS elements of
+//     input j at the summed variable's 0 (bits 0-15) and 1 (bits 16-31).  A step
 //     without a sum names element 1 (0+0i) as its first input's second element;
 //     unused inputs name element 0 (1+0i).
 // Reduction op (a chain of single-input sums over an LDS tensor's top k
