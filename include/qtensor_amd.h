@@ -17,6 +17,9 @@
  *                        tensors that get_sliced_buckets materialises from data_dict
  *                        (ProcessingFrameworks.py:17-18,33-34)
  *   qb_exec_slots     -> (no reference counterpart: sizes the executor's scratch)
+ *   qb_check_program  -> the structural checks of bucket elimination
+ *                        (AssertionError at ProcessingFrameworks.py:74,137), on a
+ *                        compiled program before it runs
  *   qb_wide_execute   -> bucket_elimination(process_bucket) over one large network
  *                        (qtensor/Simulate.py:90-93) or one slice of it
  *                        (qtensor/FeynmanSimulator.py:49-58), all steps on the device
@@ -133,7 +136,29 @@ typedef struct qb_exec_desc {
  * launch shape; qb_execute uses at most this many slots. */
 int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n_slots);
 
+/* qb_execute checks the descriptor's sizes, but it reads the items and the
+ * program through raw device pointers and does NOT validate their contents:
+ * a plan offset outside the program, a plan needing more LDS or scratch than
+ * the descriptor reserves, or an output slot outside item_out makes the
+ * kernel read or write out of bounds.  Run qb_check_program on host copies of
+ * the same program and items first (qb_plan_create always does). */
 int qb_execute(const qb_exec_desc* desc, void* stream);
+
+/*
+ * Host check of a batched program and its items (no device work; usable
+ * without qb_init): every item's plan offset, angle offset and output slot
+ * (slots must be a permutation of 0..n_items-1), and for every plan its
+ * header, gate table (device-built kinds only, inside the LDS data, angle
+ * indices inside the item's angle vector), every op record and lane table
+ * (inside the program, every LDS element below the plan's LDS need, which
+ * must not exceed lds_elems, every scratch location below its scratch need,
+ * which must not exceed scratch_elems, every mask consistent with its run
+ * word and summed-variable offset) and its result location.
+ * Replaces the structural assertions the reference makes while eliminating
+ * (ProcessingFrameworks.py:74,137).
+ */
+int qb_check_program(const int32_t* program, int64_t program_words, const int32_t* items, int32_t n_items,
+                     int32_t n_params, int32_t item_threads, int32_t lds_elems, int64_t scratch_elems);
 
 /*
  * Wide plan: one compiled lightcone plan whose tensors (gates included) all

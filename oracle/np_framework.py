@@ -12,15 +12,16 @@ ordering assertion at ``:137``).
 and apply the slice dict (``ProcessingFrameworks.py:33-34``; size-1 retention at
 ``examples/Qtree Basic usage.ipynb:330,468``).
 
-The functions take and return objects of the engine's data model
-(``qtensor_amd.model``) but compute with numpy only.
+The functions take and return tensors of any class with the reference data
+model's attributes (``name``, ``indices``, ``data_key``, ``data`` and a
+``copy(indices=..., data=...)`` method; variables with ``int(v)``, ``size`` and
+``copy(size=...)``, ``qtensor/ProcessingFrameworks.py:53,69-72,96``).  Results
+are built with the input tensors' own class.  This module imports nothing from
+the engine: the transpose and slice below are its own.
 """
 import string
 
 import numpy as np
-
-from qtensor_amd.circuit import slice_tensor_data
-from qtensor_amd.model import Tensor
 
 _LETTERS = string.ascii_letters
 
@@ -34,6 +35,7 @@ def _einsum_multiply(ix_a, a, ix_b, b):
 
 
 def process_bucket(bucket, no_sum=False, dtype=np.complex128):
+    tensor_cls = type(bucket[0])
     ix = [int(v) for v in bucket[0].indices]
     vars_by_id = {int(v): v for t in bucket for v in t.indices}
     data = np.asarray(bucket[0].data, dtype=dtype)
@@ -48,7 +50,36 @@ def process_bucket(bucket, no_sum=False, dtype=np.complex128):
             indices = indices[1:]
     else:
         tag = "f"
-    return Tensor(f"E{tag}", indices, data=data)
+    return tensor_cls(f"E{tag}", indices, data=data)
+
+
+def transpose_and_slice(indices, data, slice_dict):
+    """Axes of ``data`` (one per variable of ``indices``, in that order) sorted
+    by ascending variable id, then ``slice_dict`` applied axis by axis: a
+    ``slice`` keeps the axis with the sliced size, an ``int`` selects one
+    entry and drops the axis (``qtensor/ProcessingFrameworks.py:33-34``; the
+    kept size-1 axes of ``examples/Qtree Basic usage.ipynb:330,468``).
+    Returns ``(new indices, data)``."""
+    data = np.asarray(data)
+    perm = sorted(range(len(indices)), key=lambda a: int(indices[a]))
+    data = np.transpose(data, perm)
+    out_vars = []
+    axis = 0
+    for a in perm:
+        v = indices[a]
+        sel = slice_dict.get(v, None)
+        if sel is None:
+            out_vars.append(v)
+            axis += 1
+        elif isinstance(sel, slice):
+            data = np.take(data, np.arange(data.shape[axis])[sel], axis=axis)
+            out_vars.append(v.copy(size=data.shape[axis]))
+            axis += 1
+        else:
+            data = np.take(data, int(sel), axis=axis)
+    if len(out_vars) != data.ndim:
+        raise AssertionError("sliced data and indices disagree")
+    return tuple(out_vars), data
 
 
 def get_sliced_buckets(buckets, data_dict, slice_dict):
@@ -56,7 +87,7 @@ def get_sliced_buckets(buckets, data_dict, slice_dict):
     for bucket in buckets:
         nb = []
         for t in bucket:
-            indices, data = slice_tensor_data(t, data_dict[t.data_key], slice_dict)
+            indices, data = transpose_and_slice(t.indices, data_dict[t.data_key], slice_dict)
             nb.append(t.copy(indices=indices, data=data))
         out.append(nb)
     return out

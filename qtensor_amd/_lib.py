@@ -24,7 +24,7 @@ QB_ITEM_WORDS = 4
 EXPORTED = ("qb_version", "qb_init", "qb_shutdown", "qb_last_error", "qb_step",
             "qb_exec_slots", "qb_execute", "qb_wide_execute", "qb_reduce_terms",
             "qb_greedy_order", "qb_elimination_widths", "qb_rgreedy_order", "qb_place_intervals",
-            "qb_plan_create", "qb_plan_execute", "qb_plan_destroy")
+            "qb_plan_create", "qb_plan_execute", "qb_plan_destroy", "qb_check_program")
 
 
 class StepDesc(ctypes.Structure):
@@ -98,10 +98,26 @@ def load():
         i64p = ctypes.POINTER(ctypes.c_int64)
         lib.qb_place_intervals.argtypes = [ctypes.c_int32, i64p, i64p, i64p, i64p, ctypes.c_int64, i64p, i64p]
         lib.qb_wide_execute.argtypes = [ctypes.POINTER(WideDesc), ctypes.c_void_p]
+        lib.qb_check_program.argtypes = [i32p, ctypes.c_int64, i32p, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int64]
         lib.qb_reduce_terms.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib = lib
         return lib
+
+
+def check_program(program, items, n_params, item_threads, lds_elems, scratch_elems):
+    """Host check of a batched program and its items (``qb_check_program``)
+    before ``qb_execute`` reads them on the device; raises ``ValueError``."""
+    import numpy as np
+    lib = load()
+    program = np.ascontiguousarray(program, dtype=np.int32)
+    items = np.ascontiguousarray(items, dtype=np.int32).reshape(-1)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    check(lib.qb_check_program(program.ctypes.data_as(i32p), program.size,
+                               items.ctypes.data_as(i32p) if items.size else ctypes.cast(program.ctypes.data, i32p),
+                               items.size // QB_ITEM_WORDS, int(n_params), int(item_threads), int(lds_elems),
+                               int(scratch_elems)))
 
 
 def check(code):

@@ -11,8 +11,8 @@ at the three best (threshold, slicer) pairs.  Every candidate is compiled under
 the tuner's elimination order (``ordering``: "greedy" or "rgreedy", as in
 LightconeEngine).
 
-Results are memoised per (graph signature, p, composer, dtype) in-process and
-can be saved to / loaded from a JSON file.
+Results are memoised per (graph, p, composer, dtype, order) in-process and
+can be saved to a JSON file.
 """
 import json
 import sys
@@ -25,8 +25,13 @@ from .batched import LaunchShape, LightconeEngine
 
 
 def _signature(G, p, composer, dtype, ordering):
-    degs = tuple(sorted(d for _, d in G.degree()))
-    return (G.number_of_nodes(), G.number_of_edges(), hash(degs), p, composer.__name__, dtype, ordering)
+    """Memo key of a tuning result: the exact graph (node order and edge set,
+    so two different graphs of one size and degree sequence are tuned
+    separately), depth, composer, precision and order."""
+    nodes = list(G.nodes())
+    pos = {v: k for k, v in enumerate(nodes)}
+    edges = tuple(sorted(tuple(sorted((pos[a], pos[b]))) for a, b in G.edges()))
+    return (len(nodes), edges, p, composer.__name__, dtype, ordering)
 
 
 BASE_SHAPE = LaunchShape(128, 4096, 2048)

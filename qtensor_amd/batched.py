@@ -175,7 +175,17 @@ class LightconeEngine:
         return cls(max_tw=self.max_tw, tw_bias=self.tw_bias)
 
     def _cache_key(self, key, p, shape):
-        return (key, p, shape.key(), self.composer.__name__, self.max_tw, self.slicer, self.ordering)
+        return (key, p, shape.key(), self.composer.__name__, self.max_tw, self.tw_bias, self.slicer,
+                self.ordering)
+
+    def _check_fits(self, plan, shape):
+        """The plan's LDS data fit the launch shape in this engine's precision."""
+        if plan.lds_need > shape.lds_capacity:
+            raise ValueError(f"plan needs {plan.lds_need} LDS elements (> {shape.lds_capacity}); "
+                             "its gate tensors alone do not fit the launch shape")
+        elem = 16 if self.dtype == "complex128" else 8
+        if plan.lds_elems(elem) * elem > 160 * 1024:
+            raise ValueError("plan data exceed 160 KiB of LDS")
 
     def shape_for(self, p):
         if self.shape is not None:
@@ -187,6 +197,8 @@ class LightconeEngine:
         with self._lock:
             hit = self._plans.get(ck)
         if hit is not None:
+            # plans are shared across precisions: the LDS check depends on it
+            self._check_fits(hit, shape)
             return hit
         g = canonical_graph(n, edges)
         comp = self.composer(g, gamma=[0.0] * p, beta=[0.0] * p)
@@ -203,12 +215,7 @@ class LightconeEngine:
                 if "LDS capacity" not in str(exc) or lim <= shape.item_threads:
                     raise
                 lim //= 2
-        if plan.lds_need > shape.lds_capacity:
-            raise ValueError(f"plan needs {plan.lds_need} LDS elements (> {shape.lds_capacity}); "
-                             "its gate tensors alone do not fit the launch shape")
-        if plan.lds_elems(16 if self.dtype == "complex128" else 8) * (16 if self.dtype == "complex128" else 8) \
-                > 160 * 1024:
-            raise ValueError("plan data exceed 160 KiB of LDS")
+        self._check_fits(plan, shape)
         with self._lock:
             self._plans[ck] = plan
         return plan
@@ -365,6 +372,9 @@ class Batch:
                                           ctypes.byref(slots)))
         self.n_slots = slots.value
         items_np = order_items(items, order, band=_cu_count(self.device))
+        # qb_execute trusts the program and items: check them on the host first
+        _lib.check_program(program, items_np, len(params), self.shape.item_threads, self.lds_elems,
+                           self.scratch_elems)
         self._host = (program, items_np, np.asarray(seg, dtype=np.int32), len(params))
         self.program = torch.from_numpy(program).to(dev)
         self.items = torch.from_numpy(np.ascontiguousarray(items_np)).to(dev)

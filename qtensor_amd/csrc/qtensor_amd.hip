@@ -24,7 +24,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <vector>
 #include <string>
 
 #include "../../include/qtensor_amd.h"
@@ -382,6 +384,7 @@ constexpr uint32_t kOpGather = 1u << 15;
 // qtensor_amd/plan.py): 1 + 0i and 0 + 0i
 constexpr uint32_t kLocOne = 0u, kLocZero = 1u;
 constexpr int kGatherHdr = 4;
+constexpr int kReduceHdr = 4;
 // gate kinds (shared with qtensor_amd/gates.py and qtensor_amd/plan.py)
 enum GateKind : int32_t {
     kConst = 0, kZPhase = 1, kXPhase = 2, kZZ = 3, kYPhase = 4,
@@ -1368,6 +1371,172 @@ namespace qb_internal {
 int fail(int code, const char* msg) { return set_error(code, "%s", msg); }
 }  // namespace qb_internal
 
+// Host check of one executor plan at word `off` of a program: every record and
+// lane table lies inside the program, and every LDS element an op, a gate or the
+// result touches lies inside the item's data region (lds_elems elements: the
+// per-wave angle copies and the queue ticket sit right after it), every global
+// location inside the item's scratch slot (scratch_elems elements), every angle
+// index inside the item's angle vector (n_angles doubles from the item's
+// offset, plus the kParamRegs words of padding the executor may read), and
+// every offset the kernel derives from a mask agrees with that mask.
+// `const_ok`: whether gates of kind kConst (copied from the constant pool) are
+// allowed.
+static int check_item_plan(const int32_t* prog, int64_t words, int64_t off, int32_t item_threads, int32_t lds_elems,
+                           int64_t scratch_elems, int64_t n_angles, bool const_ok) {
+    if (off < 0 || off + kPlanHdr > words) return set_error(QB_ERR_INVALID, "plan at %lld: header outside the program", (long long)off);
+    const int32_t* p = prog + off;
+    const int32_t n_gates = p[0], n_ops = p[1], gates_off = p[2], ops_off = p[3];
+    const int64_t lds_need = (uint32_t)p[5];
+    const int64_t glb_need = (int64_t)(uint32_t)p[6] | ((int64_t)(uint32_t)p[9] << 32);
+    if (n_gates < 0 || n_ops < 0 || gates_off < kPlanHdr || ops_off < kPlanHdr)
+        return set_error(QB_ERR_INVALID, "plan at %lld: bad header", (long long)off);
+    if (lds_need > lds_elems || lds_need > 65536)
+        return set_error(QB_ERR_INVALID, "plan at %lld needs %lld LDS elements, the launch reserves %d",
+                         (long long)off, (long long)lds_need, lds_elems);
+    if (glb_need > scratch_elems)
+        return set_error(QB_ERR_INVALID, "plan at %lld needs %lld scratch elements, the slot holds %lld",
+                         (long long)off, (long long)glb_need, (long long)scratch_elems);
+    if (off + gates_off + (int64_t)n_gates * kGateWords > words)
+        return set_error(QB_ERR_INVALID, "plan at %lld: gate table outside the program", (long long)off);
+    auto lds_ok = [&](int64_t first, int64_t last) { return first >= 0 && first <= last && last < lds_need; };
+    auto loc_ok = [&](uint32_t loc, int64_t n) {
+        if (loc & kGlobal) return (int64_t)(loc & ~kGlobal) + n <= glb_need;
+        return lds_ok((int64_t)loc, (int64_t)loc + n - 1);
+    };
+    for (int32_t g = 0; g < n_gates; ++g) {
+        const int32_t* e = p + gates_off + (int64_t)g * kGateWords;
+        const int32_t kind = e[0], dst = e[1], n = e[2];
+        if (kind < kConst || kind > kM || (kind == kConst && !const_ok))
+            return set_error(QB_ERR_INVALID, "plan at %lld gate %d: kind %d has no device builder", (long long)off, g, kind);
+        if (n < 1 || n > 8 || dst < 4 || !lds_ok(dst, (int64_t)dst + n - 1))
+            return set_error(QB_ERR_INVALID, "plan at %lld gate %d outside the LDS data", (long long)off, g);
+        for (int k = 0; k < 2; ++k) {
+            const int32_t j = e[4 + 2 * k];
+            if (j < -1 || (int64_t)j >= n_angles)
+                return set_error(QB_ERR_INVALID, "plan at %lld gate %d: angle index %d", (long long)off, g, j);
+        }
+        if ((e[11] & 0xff) > 2) return set_error(QB_ERR_INVALID, "plan at %lld gate %d: slice word", (long long)off, g);
+    }
+    int64_t pos = off + ops_off;
+    for (int32_t k = 0; k < n_ops; ++k) {
+        if (pos + 4 > words) return set_error(QB_ERR_INVALID, "plan at %lld op %d outside the program", (long long)off, k);
+        const uint32_t w0 = (uint32_t)prog[pos];
+        const int64_t len = w0 >> 16;
+        if (pos + len > words || len < 4)
+            return set_error(QB_ERR_INVALID, "plan at %lld op %d: record length %lld", (long long)off, k, (long long)len);
+        const int32_t* r = prog + pos;
+        if (w0 & kOpGather) {
+            const uint32_t n_act = (uint32_t)r[1], tab = (uint32_t)r[2], ni = (uint32_t)r[3];
+            if (ni < 1 || ni > (uint32_t)kExecMaxIn || n_act < 1 || n_act > (uint32_t)item_threads ||
+                (int64_t)tab + (int64_t)n_act * (ni + 1) > len)
+                return set_error(QB_ERR_INVALID, "plan at %lld op %d: gather table", (long long)off, k);
+            for (uint32_t l = 0; l < n_act; ++l) {
+                const int32_t* t = r + tab + (int64_t)l * (ni + 1);
+                if (!lds_ok((uint32_t)t[0], (uint32_t)t[0]) || (uint32_t)t[0] < 2)
+                    return set_error(QB_ERR_INVALID, "plan at %lld op %d lane %u writes outside the LDS data", (long long)off, k, l);
+                for (uint32_t j = 0; j < ni; ++j) {
+                    const uint32_t w = (uint32_t)t[1 + j];
+                    if (!lds_ok(w & 0xffffu, w & 0xffffu) || !lds_ok(w >> 16, w >> 16))
+                        return set_error(QB_ERR_INVALID, "plan at %lld op %d lane %u reads outside the LDS data", (long long)off, k, l);
+                }
+            }
+        } else if (w0 & kOpReduce) {
+            const uint32_t ro = (w0 >> 4) & 31u, kk = (uint32_t)r[1];
+            if (ro > 6 || kk > 24 || len != kReduceHdr)
+                return set_error(QB_ERR_INVALID, "plan at %lld op %d: reduction shape", (long long)off, k);
+            const uint32_t out_loc = (uint32_t)r[2], in_loc = (uint32_t)r[3];
+            if ((out_loc & kGlobal) || (in_loc & kGlobal) || !lds_ok(out_loc, (int64_t)out_loc + (1ll << ro) - 1) ||
+                !lds_ok(in_loc, (int64_t)in_loc + (1ll << (ro + kk)) - 1) || out_loc < 2)
+                return set_error(QB_ERR_INVALID, "plan at %lld op %d: reduction outside the LDS data", (long long)off, k);
+        } else if (w0 & kOpFused) {
+            return set_error(QB_ERR_INVALID, "plan at %lld op %d: fused ops run on the wide path only", (long long)off, k);
+        } else {
+            const uint32_t n_in = w0 & 15u, rank_out = (w0 >> 4) & 31u, sum = (w0 >> 9) & 1u;
+            const bool all_lds = (w0 >> 12) & 1u;
+            if (n_in < 1 || n_in > (uint32_t)kExecMaxIn || len != kStepHdr + kInWords * (int64_t)n_in || rank_out > 24)
+                return set_error(QB_ERR_INVALID, "plan at %lld op %d: step shape", (long long)off, k);
+            const uint32_t out_loc = (uint32_t)r[2];
+            bool any_glb = (out_loc & kGlobal) != 0;
+            if (!loc_ok(out_loc, 1ll << rank_out) || (!(out_loc & kGlobal) && out_loc < 2))
+                return set_error(QB_ERR_INVALID, "plan at %lld op %d: output outside its memory", (long long)off, k);
+            for (uint32_t j = 0; j < n_in; ++j) {
+                const uint32_t loc = (uint32_t)r[3 + 4 * j], top = (uint32_t)r[4 + 4 * j];
+                const uint32_t mask = (uint32_t)r[5 + 4 * j], runs = (uint32_t)r[6 + 4 * j];
+                const int64_t n_mask = 1ll << __builtin_popcount(mask);
+                if ((mask >> rank_out) != 0u || runs != runs_word(mask) ||
+                    !(top == 0u || (sum && top == (uint32_t)n_mask)))
+                    return set_error(QB_ERR_INVALID, "plan at %lld op %d input %u: mask, runs or top", (long long)off, k, j);
+                if (!loc_ok(loc, n_mask * (top ? 2 : 1)))
+                    return set_error(QB_ERR_INVALID, "plan at %lld op %d input %u outside its memory", (long long)off, k, j);
+                any_glb = any_glb || (loc & kGlobal);
+            }
+            if (all_lds && any_glb)
+                return set_error(QB_ERR_INVALID, "plan at %lld op %d: marked all-LDS but touches scratch", (long long)off, k);
+            if (any_glb && scratch_elems <= 0)
+                return set_error(QB_ERR_INVALID, "plan at %lld op %d touches scratch, the launch has none", (long long)off, k);
+        }
+        pos += len;
+    }
+    const uint32_t rl = (uint32_t)p[4];
+    if (!loc_ok(rl, 1)) return set_error(QB_ERR_INVALID, "plan at %lld: result outside its memory", (long long)off);
+    return QB_OK;
+}
+
+namespace qb_internal {
+// The executor launch behind qb_execute: `scratch` holds `n_slots_avail` slots
+// of d->scratch_elems elements, `queue` one uint32.
+int execute_impl(const qb_exec_desc* d, void* scratch, int32_t n_slots_avail, uint32_t* queue, void* stream) {
+    if (d->dtype != QB_C64 && d->dtype != QB_C128) return set_error(QB_ERR_INVALID, "dtype %d", d->dtype);
+    if (d->n_items < 0) return set_error(QB_ERR_INVALID, "n_items < 0");
+    if (!d->program || !d->items || !d->params || !d->item_out || !queue)
+        return set_error(QB_ERR_INVALID, "null device pointer in descriptor");
+    int32_t n_slots = 0;
+    int per_cu = 0;
+    int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, &n_slots, &per_cu);
+    if (rc) return rc;
+    int grid = per_cu * g_num_cus;
+    if (d->n_items < grid) grid = d->n_items;
+    if (grid < 1) grid = 1;
+    if ((int64_t)grid > n_slots_avail && d->scratch_elems > 0)
+        return set_error(QB_ERR_INVALID, "scratch holds %d slots, launch needs %d", n_slots_avail, grid);
+    if (d->scratch_elems > 0 && !scratch) return set_error(QB_ERR_INVALID, "scratch is null");
+    if (d->scratch_elems < 0 || d->scratch_elems * (d->dtype == QB_C128 ? 16 : 8) > (int64_t)0xFFFFFFFF)
+        return set_error(QB_ERR_UNSUPPORTED, "scratch slot of %lld elements exceeds 4 GiB",
+                         (long long)d->scratch_elems);
+    if (d->program_words < 0 || d->program_words > (1 << 30))
+        return set_error(QB_ERR_INVALID, "program_words %d outside 0..2^30", d->program_words);
+    ExecArgs a;
+    a.program = d->program;
+    a.program_bytes = d->program_words ? 4u * (uint32_t)d->program_words : 0xFFFFFFFFu;
+    a.items = d->items;
+    a.n_items = d->n_items;
+    a.lds_elems = d->lds_elems;
+    a.params = d->params;
+    a.pool = d->const_pool;
+    a.scratch = scratch;
+    a.scratch_elems = d->scratch_elems;
+    a.item_out = d->item_out;
+    a.queue = queue;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (grid < d->n_items) {
+        // a fill kernel, not a memset: the launch may be captured in a graph
+        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, st, queue);
+        QB_HIP(hipGetLastError());
+    }
+    const int elem = d->dtype == QB_C128 ? 16 : 8;
+    const int lds_bytes = d->lds_elems * elem + exec_aux_bytes(d->item_threads);
+    return d->dtype == QB_C128 ? exec_launch_dispatch<double>(d->item_threads, a, grid, lds_bytes, st)
+                               : exec_launch_dispatch<float>(d->item_threads, a, grid, lds_bytes, st);
+}
+
+int resident_items(const qb_exec_desc* d, int32_t* n_slots) {
+    int per_cu = 0;
+    return slots_for(d->item_threads, d->lds_elems, d->dtype, n_slots, &per_cu);
+}
+
+int current_device() { return g_device; }
+}  // namespace qb_internal
+
 extern "C" {
 
 int qb_version(void) { return 100; }
@@ -1452,48 +1621,8 @@ int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_
 
 int qb_execute(const qb_exec_desc* d, void* stream) {
     if (!d) return set_error(QB_ERR_INVALID, "null descriptor");
-    if (d->dtype != QB_C64 && d->dtype != QB_C128) return set_error(QB_ERR_INVALID, "dtype %d", d->dtype);
-    if (d->n_items < 0) return set_error(QB_ERR_INVALID, "n_items < 0");
-    if (!d->program || !d->items || !d->params || !d->item_out || !d->queue)
-        return set_error(QB_ERR_INVALID, "null device pointer in descriptor");
-    int32_t n_slots = 0;
-    int per_cu = 0;
-    int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, &n_slots, &per_cu);
-    if (rc) return rc;
-    const int groups = 1;
-    int grid = per_cu * g_num_cus;
-    const int need = (d->n_items + groups - 1) / groups;
-    if (need < grid) grid = need;
-    if (grid < 1) grid = 1;
-    if ((int64_t)grid * groups > d->n_slots && d->scratch_elems > 0)
-        return set_error(QB_ERR_INVALID, "scratch holds %d slots, launch needs %d", d->n_slots, grid * groups);
-    if (d->scratch_elems > 0 && !d->scratch) return set_error(QB_ERR_INVALID, "scratch is null");
-    if (d->scratch_elems < 0 || d->scratch_elems * (d->dtype == QB_C128 ? 16 : 8) > (int64_t)0xFFFFFFFF)
-        return set_error(QB_ERR_UNSUPPORTED, "scratch slot of %lld elements exceeds 4 GiB",
-                         (long long)d->scratch_elems);
-    if (d->program_words < 0 || d->program_words > (1 << 30))
-        return set_error(QB_ERR_INVALID, "program_words %d outside 0..2^30", d->program_words);
-    ExecArgs a;
-    a.program = d->program;
-    a.program_bytes = d->program_words ? 4u * (uint32_t)d->program_words : 0xFFFFFFFFu;
-    a.items = d->items;
-    a.n_items = d->n_items;
-    a.lds_elems = d->lds_elems;
-    a.params = d->params;
-    a.pool = d->const_pool;
-    a.scratch = d->scratch;
-    a.scratch_elems = d->scratch_elems;
-    a.item_out = d->item_out;
-    a.queue = d->queue;
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (grid < d->n_items) {
-        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, st, d->queue);
-        QB_HIP(hipGetLastError());
-    }
-    const int elem = d->dtype == QB_C128 ? 16 : 8;
-    const int lds_bytes = (d->lds_elems * elem + exec_aux_bytes(d->item_threads)) * groups;
-    return d->dtype == QB_C128 ? exec_launch_dispatch<double>(d->item_threads, a, grid, lds_bytes, st)
-                               : exec_launch_dispatch<float>(d->item_threads, a, grid, lds_bytes, st);
+    if (!d->queue) return set_error(QB_ERR_INVALID, "null device pointer in descriptor");
+    return qb_internal::execute_impl(d, d->scratch, d->n_slots, d->queue, stream);
 }
 
 int qb_wide_execute(const qb_wide_desc* d, void* stream) {
@@ -1512,7 +1641,10 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
         return set_error(QB_ERR_INVALID, "gate table outside the plan");
     for (int32_t g = 0; g < n_gates; ++g) {
         const int32_t* e = w + gates_off + (size_t)g * kGateWords;
-        if (e[1] < 0 || (int64_t)e[1] + e[2] > cap) return set_error(QB_ERR_INVALID, "gate %d outside the arena", g);
+        if (e[0] <= kConst || e[0] > kM)   // no constant pool on the wide path
+            return set_error(QB_ERR_INVALID, "gate %d: kind %d has no device builder", g, e[0]);
+        if (e[2] < 1 || e[2] > 8 || e[1] < 0 || (int64_t)e[1] + e[2] > cap)
+            return set_error(QB_ERR_INVALID, "gate %d outside the arena", g);
     }
     if (n_gates > 0) {
         const int blocks = (n_gates + 255) / 256;
@@ -1554,6 +1686,9 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
             const uint32_t loc = (uint32_t)in[0];
             const uint32_t top = (uint32_t)in[1], mask = (uint32_t)in[2];
             const int64_t n_elems = (1ll << __builtin_popcount(mask)) * (top ? 2 : 1);
+            // the kernel reads the summed variable's 1 at offset + top
+            if (top != 0u && !(sum && (int64_t)top == (1ll << __builtin_popcount(mask))))
+                return set_error(QB_ERR_INVALID, "op %d input %u: top %u is not 2^popcount(mask)", k, j, top);
             if (!(loc & kGlobal) || (int64_t)(loc & ~kGlobal) + n_elems > cap)
                 return set_error(QB_ERR_INVALID, "op %d input %u outside the arena", k, j);
             if (mask >> rank_out) return set_error(QB_ERR_INVALID, "op %d input %u mask beyond rank", k, j);
@@ -1647,6 +1782,41 @@ static void plan_free(qb_plan* p) {
     delete p;
 }
 
+int qb_check_program(const int32_t* program, int64_t program_words, const int32_t* items, int32_t n_items,
+                     int32_t n_params, int32_t item_threads, int32_t lds_elems, int64_t scratch_elems) {
+    if (!program || !items || n_items < 0 || program_words < 1 || n_params < 0 || lds_elems < 0 || scratch_elems < 0)
+        return set_error(QB_ERR_INVALID, "null or negative program, items or sizes");
+    if (program_words > (1 << 30))
+        return set_error(QB_ERR_INVALID, "program_words=%lld > 2^30", (long long)program_words);
+    if (item_threads != 64 && item_threads != 128 && item_threads != 256 && item_threads != 512 && item_threads != 1024)
+        return set_error(QB_ERR_INVALID, "item_threads %d not in {64,128,256,512,1024}", item_threads);
+    // output slots must be a permutation of 0..n_items-1; angle availability per plan
+    std::vector<uint8_t> slot_seen((size_t)n_items, 0);
+    std::map<int32_t, int64_t> plan_angles;   // plan offset -> fewest angles any of its items has
+    for (int32_t i = 0; i < n_items; ++i) {
+        const int32_t* it = items + (size_t)i * QB_ITEM_WORDS;
+        if (it[0] < 0 || (int64_t)it[0] + kPlanHdr > program_words)
+            return set_error(QB_ERR_INVALID, "item %d: plan offset %d outside the program", i, it[0]);
+        if (it[1] < 0 || it[1] > n_params) return set_error(QB_ERR_INVALID, "item %d: angle offset %d", i, it[1]);
+        if (it[3] < 0 || it[3] >= n_items || slot_seen[(size_t)it[3]])
+            return set_error(QB_ERR_INVALID, "item %d: output slot %d (slots must be a permutation)", i, it[3]);
+        slot_seen[(size_t)it[3]] = 1;
+        const int64_t avail = (int64_t)n_params - it[1];
+        auto f = plan_angles.find(it[0]);
+        if (f == plan_angles.end()) plan_angles.emplace(it[0], avail);
+        else if (avail < f->second) f->second = avail;
+    }
+    for (const auto& kv : plan_angles) {
+        const int rc = check_item_plan(program, program_words, kv.first, item_threads, lds_elems, scratch_elems,
+                                       kv.second, false);
+        if (rc) return rc;
+        const int32_t n_sliced = program[kv.first + 7];
+        if (n_sliced < 0 || n_sliced > 24)
+            return set_error(QB_ERR_INVALID, "plan at %d: %d sliced variables", kv.first, n_sliced);
+    }
+    return QB_OK;
+}
+
 int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
     if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
     if (!d || !out) return set_error(QB_ERR_INVALID, "null descriptor or output");
@@ -1665,13 +1835,9 @@ int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
         return set_error(QB_ERR_INVALID, "segments must cover items 0..n_items");
     bool unit_seg = true;
     for (int32_t t = 0; t < d->n_terms; ++t) unit_seg = unit_seg && d->seg[t + 1] - d->seg[t] == 1;
-    for (int32_t i = 0; i < d->n_items; ++i) {
-        const int32_t* it = d->items + (size_t)i * QB_ITEM_WORDS;
-        if (it[0] < 0 || (int64_t)it[0] + kPlanHdr > d->program_words)
-            return set_error(QB_ERR_INVALID, "item %d: plan offset %d outside the program", i, it[0]);
-        if (it[1] < 0 || it[1] > d->n_params) return set_error(QB_ERR_INVALID, "item %d: angle offset", i);
-        if (it[3] < 0 || it[3] >= d->n_items) return set_error(QB_ERR_INVALID, "item %d: output slot", i);
-    }
+    int rc0 = qb_check_program(d->program, d->program_words, d->items, d->n_items, d->n_params, d->item_threads,
+                               d->lds_elems, d->scratch_elems);
+    if (rc0) return rc0;
     int32_t n_slots = 0;
     int per_cu = 0;
     int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, &n_slots, &per_cu);

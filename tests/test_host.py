@@ -441,3 +441,128 @@ def test_order_items_serpentine_bands():
     snake = order_items(items, costs, band=4)          # runs [1 5 3 7] [0 8 4 6] [2 9], the second reversed
     assert list(snake[:, 0]) == [1, 5, 3, 7, 6, 4, 8, 0, 2, 9]
     assert list(snake[:, 3]) == list(snake[:, 0])
+
+
+def _checked_program(lanes=64, lim=64, cap=4096, p=2, seed=3):
+    """A small valid batched program (two plans, their items) for qb_check_program."""
+    G = nx.random_regular_graph(3, 12, seed=seed)
+    plans = [_edge_plan(G, e, p, lds_tensor_limit=lim, lds_capacity=cap, lanes=lanes) for e in list(G.edges())[:2]]
+    words, items, pos = [], [], 0
+    for k, pl in enumerate(plans):
+        words.append(pl.words)
+        items.append([pos, 0, 0, k])
+        pos += pl.words.size
+    words.append(np.zeros(320, dtype=np.int32))
+    prog = np.concatenate(words).astype(np.int32)
+    lds = max(pl.lds_need for pl in plans)
+    glb = max(pl.glb_need for pl in plans)
+    return plans, prog, np.asarray(items, dtype=np.int32), lds, glb
+
+
+def test_check_program_accepts_compiled_plans():
+    for lim in (64, 4):       # all-LDS plans, and plans with intermediates in scratch
+        plans, prog, items, lds, glb = _checked_program(lim=lim)
+        _lib.check_program(prog, items, 2 * 2, 64, lds, glb)
+
+
+def test_check_program_rejects_corruptions():
+    """Every corruption that would make qb_execute read or write out of bounds
+    is rejected on the host (no device work)."""
+    plans, prog, items, lds, glb = _checked_program()
+    ok = dict(n_params=4, item_threads=64, lds_elems=lds, scratch_elems=glb)
+
+    def rejects(prog_, items_, **kw):
+        args = dict(ok)
+        args.update(kw)
+        with pytest.raises(ValueError):
+            _lib.check_program(prog_, items_, args["n_params"], args["item_threads"], args["lds_elems"],
+                               args["scratch_elems"])
+
+    _lib.check_program(prog, items, **ok)
+    rejects(prog, items, lds_elems=lds - 1)                      # plan needs more LDS than reserved
+    bad = items.copy(); bad[1, 0] = prog.size                    # plan offset past the program
+    rejects(prog, bad)
+    bad = items.copy(); bad[1, 3] = 0                            # output slots not a permutation
+    rejects(prog, bad)
+    bad = items.copy(); bad[0, 1] = 5                            # angle offset past the angles
+    rejects(prog, bad)
+    rejects(prog, items, n_params=1)                             # gate angle index past the angles
+    p0 = plans[0].words
+    g0 = int(p0[2])
+    bad = prog.copy(); bad[g0] = 0                               # kConst gate: no constant pool
+    rejects(bad, items)
+    bad = prog.copy(); bad[g0 + 1] = lds                         # gate tensor outside the LDS data
+    rejects(bad, items)
+    # the first op record of plan 0: break its fields one at a time
+    op = int(p0[3])
+    w0 = int(p0[op])
+    if w0 & (1 << 15):                                           # gather op: lane table entry
+        tab = op + int(p0[op + 2])
+        bad = prog.copy(); bad[tab] = lds                        # lane writes past the LDS data
+        rejects(bad, items)
+        bad = prog.copy(); bad[tab + 1] = (lds << 16) | 4        # lane reads past the LDS data
+        rejects(bad, items)
+        bad = prog.copy(); bad[op + 1] = 65                      # more active lanes than item threads
+        rejects(bad, items)
+    bad = prog.copy(); bad[op] = (w0 & 0xFFFF) | (1 << 28)       # record longer than the program
+    rejects(bad, items)
+    bad = prog.copy(); bad[4] = lds                              # result outside the LDS data
+    rejects(bad, items)
+
+
+def test_check_program_rejects_step_mask_runs_and_top():
+    """Step ops: a runs word that disagrees with its mask, a top that is not
+    2^popcount(mask), or scratch locations without scratch are rejected."""
+    G = nx.random_regular_graph(3, 12, seed=3)
+    pl = _edge_plan(G, list(G.edges())[0], 2, lds_tensor_limit=4, lds_capacity=4096, gather=False)
+    prog = np.concatenate([pl.words, np.zeros(320, np.int32)]).astype(np.int32)
+    items = np.array([[0, 0, 0, 0]], dtype=np.int32)
+    _lib.check_program(prog, items, 4, 64, pl.lds_need, pl.glb_need)
+    pos, steps = int(pl.words[3]), []
+    for _ in range(int(pl.words[1])):
+        w0 = int(prog[pos])
+        if not w0 & ((1 << 15) | (1 << 13)):
+            steps.append(pos)
+        pos += w0 >> 16
+    s = next(q for q in steps if int(prog[q + 5]) != 0)          # a step whose input 0 has a mask
+    bad = prog.copy(); bad[s + 6] ^= 1 << 5                       # runs word no longer matches
+    with pytest.raises(ValueError):
+        _lib.check_program(bad, items, 4, 64, pl.lds_need, pl.glb_need)
+    bad = prog.copy(); bad[s + 4] = 3                             # top not 0 or 2^popcount
+    with pytest.raises(ValueError):
+        _lib.check_program(bad, items, 4, 64, pl.lds_need, pl.glb_need)
+    if pl.glb_need:
+        with pytest.raises(ValueError):
+            _lib.check_program(prog, items, 4, 64, pl.lds_need, 0)
+
+
+def test_oracle_transpose_and_slice_vs_loops():
+    """The oracle's own transpose + slice (no engine code) against explicit
+    index loops: ints drop an axis, slices keep it with the new size."""
+    from qtensor_amd.model import Var
+    rng = np.random.default_rng(0)
+    for trial in range(20):
+        r = int(rng.integers(1, 5))
+        ids = [int(x) for x in rng.choice(50, size=r, replace=False)]
+        vs = [Var(i) for i in ids]
+        data = rng.normal(size=(2,) * r) + 1j * rng.normal(size=(2,) * r)
+        sd = {}
+        for v in vs:
+            c = int(rng.integers(0, 3))
+            if c == 1:
+                sd[v] = int(rng.integers(0, 2))
+            elif c == 2:
+                b = int(rng.integers(0, 2))
+                sd[v] = slice(b, b + 1)
+        out_vars, out = npf.transpose_and_slice(tuple(vs), data, sd)
+        order = sorted(range(r), key=lambda a: ids[a])
+        kept = [a for a in order if not isinstance(sd.get(vs[a]), int)]
+        assert [int(v) for v in out_vars] == [ids[a] for a in kept]
+        ranges = [range(2) if vs[a] not in sd else
+                  ([sd[vs[a]]] if isinstance(sd[vs[a]], int) else range(2)[sd[vs[a]]]) for a in range(r)]
+        for v, a in zip(out_vars, kept):
+            assert v.size == len(ranges[a])
+        for full in np.ndindex(*(len(x) for x in ranges)):
+            idx = tuple(ranges[a][full[a]] for a in range(r))
+            out_idx = tuple(full[a] for a in kept)
+            assert out[out_idx] == data[idx]
