@@ -145,6 +145,24 @@ int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_
 int qb_execute(const qb_exec_desc* desc, void* stream);
 
 /*
+ * qb_execute without caller workspace.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * descriptor is qb_execute's with scratch = NULL, n_slots = 0 and queue =
+ * NULL: the work-queue ticket and the items' global scratch slots live in a
+ * library-owned static device array of QB_NOALLOC_SCRATCH_BYTES per device
+ * (256 bytes for the ticket, the rest for scratch), resolved by qb_init.  The
+ * call allocates nothing and rejects a launch whose resident items x
+ * scratch_elems elements do not fit (QB_ERR_UNSUPPORTED).  Every call on a
+ * device shares that array, so calls that overlap on one device race; the
+ * array keeps the last call's contents (the ticket is reset by a fill kernel
+ * whenever a launch takes tickets).  item_out, the program, items and angles
+ * stay the caller's, unchecked as for qb_execute.
+ */
+#define QB_NOALLOC_SCRATCH_BYTES (64u << 20)
+int qb_execute_noalloc(const qb_exec_desc* desc, void* stream);
+
+/*
  * Host check of a batched program and its items (no device work; usable
  * without qb_init): every item's plan offset, angle offset and output slot
  * (slots must be a permutation of 0..n_items-1), and for every plan its

@@ -24,7 +24,10 @@ QB_ITEM_WORDS = 4
 EXPORTED = ("qb_version", "qb_init", "qb_shutdown", "qb_last_error", "qb_step",
             "qb_exec_slots", "qb_execute", "qb_wide_execute", "qb_reduce_terms",
             "qb_greedy_order", "qb_elimination_widths", "qb_rgreedy_order", "qb_place_intervals",
-            "qb_plan_create", "qb_plan_execute", "qb_plan_destroy", "qb_check_program")
+            "qb_plan_create", "qb_plan_execute", "qb_plan_destroy", "qb_check_program",
+            "qb_execute_noalloc")
+# bytes of the static device workspace behind qb_execute_noalloc
+QB_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class StepDesc(ctypes.Structure):
@@ -87,6 +90,7 @@ def load():
         lib.qb_exec_slots.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_int32)]
         lib.qb_execute.argtypes = [ctypes.POINTER(ExecDesc), ctypes.c_void_p]
+        lib.qb_execute_noalloc.argtypes = [ctypes.POINTER(ExecDesc), ctypes.c_void_p]
         lib.qb_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(ctypes.c_void_p)]
         lib.qb_plan_execute.argtypes = [ctypes.c_void_p] * 5
         lib.qb_plan_destroy.argtypes = [ctypes.c_void_p]

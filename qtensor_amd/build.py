@@ -1,7 +1,8 @@
 """Build ``libqtensor_amd.so`` in-tree for gfx950 with hipcc.
 
 ``python -m qtensor_amd.build`` compiles ``csrc/qtensor_amd.hip`` (kernels and
-the C ABI) and ``csrc/planner.cpp`` (host-only ordering code) into
+the C ABI), ``csrc/noalloc.hip`` (the executor entry point that allocates
+nothing) and ``csrc/planner.cpp`` (host-only ordering code) into
 ``qtensor_amd/libqtensor_amd.so`` (always recompiles).
 """
 import os
@@ -10,6 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "qtensor_amd.hip")
+NOALLOC_SRC = os.path.join(HERE, "csrc", "noalloc.hip")   # qb_execute_noalloc and its static workspace
 HOST_SRC = os.path.join(HERE, "csrc", "planner.cpp")    # host-only code (elimination ordering)
 OUT = os.path.join(HERE, "libqtensor_amd.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
@@ -30,7 +32,7 @@ def hipcc():
 
 def build(verbose=False):
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", tmp, SRC, HOST_SRC]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", tmp, SRC, NOALLOC_SRC, HOST_SRC]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -16,6 +16,8 @@ int execute_impl(const qb_exec_desc* d, void* scratch, int32_t n_slots_avail, ui
 int resident_items(const qb_exec_desc* d, int32_t* n_slots);
 // Device selected by qb_init (-1 before).
 int current_device();
+// Resolve qb_execute_noalloc's static workspace on `device` (current), once.
+int noalloc_init(int device);
 }  // namespace qb_internal
 
 #endif

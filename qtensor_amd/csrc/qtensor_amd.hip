@@ -1552,6 +1552,9 @@ int qb_init(int device) {
     hipDeviceProp_t prop;
     QB_HIP(hipGetDeviceProperties(&prop, device));
     g_num_cus = prop.multiProcessorCount;
+    // this device's copy of qb_execute_noalloc's static workspace
+    const int rc = qb_internal::noalloc_init(device);
+    if (rc) return rc;
     g_device = device;
     return QB_OK;
 }
