@@ -11,8 +11,9 @@ per-slice FP64 results, summed with Neumaier compensation by
 ``forced_width_order`` reproduces a deliberately wide elimination order with
 the reference's randomized greedy rule (``qtensor/optimisation/RGreedy.py``):
 seeded samples at a given temperature until one reaches the target width.
-``slice_vars_for_order`` picks slice variables for a fixed order by how much
-of the wide elimination neighbourhoods they cover.
+``greedy_parvars_for_order`` picks slice variables for a fixed order by the
+reference's max-degree rule; ``slice_vars_for_order`` by how much of the wide
+elimination neighbourhoods they cover.
 """
 import ctypes
 from collections import Counter
@@ -68,6 +69,27 @@ def slice_vars_for_order(graph, order, n_vars):
                     cnt[v] += 1 << len(b)
         par.append(cnt.most_common(1)[0][0])
     return par
+
+
+def greedy_parvars_for_order(graph, order, n_vars):
+    """Slice variables by the reference's ``GreedyParvars`` rule
+    (``qtensor/optimisation/Greedy.py:78-92``, driven by ``SlicesOptimizer``,
+    ``qtensor/optimisation/Optimizer.py:109-129``): repeatedly take the
+    highest-degree variable of the line graph (first in graph order on ties)
+    and remove it.  The elimination order of the remaining variables stays
+    ``order``.  Returns ``(par, width)``: the variables and the order's width
+    after removing them."""
+    from .ordering import GreedyParvars
+    gp = GreedyParvars(graph.copy())
+    for _ in range(n_vars):
+        if gp.step():
+            break
+    par = list(gp.result)
+    g = graph.copy()
+    g.remove_nodes_from(par)
+    gone = set(par)
+    rest = [v for v in order if v not in gone]
+    return par, (max(elimination_widths(g, rest)) if rest else 0)
 
 
 class WidePlan:

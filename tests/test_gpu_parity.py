@@ -1,0 +1,160 @@
+"""Parity at the benchmark workloads' own sizes, against complex128 oracle
+values (``tests/golden/parity.*``, from ``oracle.qaoa_network``): every edge
+term of C2 (N=256, p=3) and C3 (N=1000, p=2) through the batched executor, C2's
+forced treewidth-30 edge on the wide path sliced by GreedyParvars, every C4
+edge (Erdos-Renyi N=64, p=3, two angle sets) in complex64 against complex128
+on the device and complex128 against the oracle where the oracle reaches, and
+a C5 serving trace whose results equal the same requests run alone and the
+oracle.  Tolerance: |delta| <= 1e-5 * max(1, |ref|) per term (complex64)."""
+import time
+
+import networkx as nx
+import numpy as np
+import pytest
+
+import parity_fixtures as pf
+from qtensor_amd.batched import LaunchShape, LightconeEngine
+from qtensor_amd.composer import DefaultQAOAComposer, QtreeQAOAComposer
+
+pytestmark = pytest.mark.gpu
+
+META, ARR = pf.load()
+
+
+def _check(vals, refs, tol_fn=pf.tol):
+    errs = [abs(v - r) for v, r in zip(vals, refs)]
+    bad = [(k, v, r) for k, (v, r) in enumerate(zip(vals, refs)) if abs(v - r) > tol_fn(r)]
+    assert not bad, f"{len(bad)} terms off, first {bad[:3]}, max err {max(errs):.3e}"
+    return max(errs)
+
+
+@pytest.mark.parametrize("config", [
+    ("tuned", LaunchShape(256, 4096, 2048), 9, "tree_trim", "rgreedy", QtreeQAOAComposer),
+    ("greedy", LaunchShape(256, 8192, 4096), None, "greedy", "greedy", QtreeQAOAComposer),
+    ("cli_composer", LaunchShape(256, 4096, 2048), 10, "tree_trim", "rgreedy", DefaultQAOAComposer),
+], ids=lambda c: c[0])
+def test_c2_all_terms_vs_oracle(gpu, config):
+    _, shape, max_tw, slicer, ordering, comp = config
+    m = META["c2"]
+    G = pf.graph(m)
+    eng = LightconeEngine(composer=comp, shape=shape, max_tw=max_tw, slicer=slicer, ordering=ordering)
+    C, zz = eng.energy(G, m["gamma"], m["beta"])
+    assert len(zz) == 384
+    err = _check(zz, ARR["c2"])
+    print(f"C2 {config[0]}: 384 terms, max |err| {err:.2e}")
+
+
+def test_c2_complex128_all_terms(gpu):
+    m = META["c2"]
+    eng = LightconeEngine(dtype="complex128", shape=LaunchShape(256, 4096, 2048), max_tw=9,
+                          slicer="tree_trim", ordering="rgreedy")
+    _, zz = eng.energy(pf.graph(m), m["gamma"], m["beta"])
+    _check(zz, ARR["c2"], lambda r: 1e-11)
+
+
+def test_c2_forced_edge_greedy_parvars(gpu):
+    """C2's extra edge: edge (0, 83) under a deliberately wide order (seeded
+    randomized greedy at temperature 8, first sample of width 30), sliced over
+    3 variables chosen by GreedyParvars, on the wide path."""
+    from qtensor_amd.tensornet import QtreeTensorNet
+    from qtensor_amd.wide import WideEngine, compile_wide, forced_width_order, greedy_parvars_for_order
+    m = META["c2"]
+    G = pf.graph(m)
+    e = tuple(m["edges"][0])
+    comp = QtreeQAOAComposer(G, gamma=m["gamma"], beta=m["beta"])
+    comp.energy_expectation_lightcone(e)
+    graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+    order = forced_width_order(graph, 30, temp=8.0, seed=0)
+    par, width = greedy_parvars_for_order(graph, order, 3)
+    assert width < 30
+    wp = compile_wide(comp.circuit, order=order, par_ids=par)
+    assert wp.n_slices == 8
+    for dtype, tol in (("complex64", pf.tol(ARR["c2"][0])), ("complex128", 1e-11)):
+        val = WideEngine(dtype=dtype).run(wp, list(m["gamma"]) + list(m["beta"]))
+        assert abs(val - ARR["c2"][0]) <= tol, (dtype, val, ARR["c2"][0])
+
+
+def test_c3_all_terms_vs_oracle(gpu):
+    m = META["c3"]
+    eng = LightconeEngine(shape=LaunchShape(64, 4096, 2048), max_tw=10, slicer="tree_trim", ordering="rgreedy")
+    _, zz = eng.energy(pf.graph(m), m["gamma"], m["beta"])
+    assert len(zz) == 1500
+    _check(zz, ARR["c3"])
+
+
+def test_c4_every_edge_two_sets(gpu):
+    """All 116 edges: complex64 against complex128 (same plans, FP64 gates)
+    on the device, and complex128 against the oracle on the edges it reaches."""
+    from qtensor_amd.ordering import greedy_peo
+    from qtensor_amd.tensornet import QtreeTensorNet
+    from qtensor_amd.wide import WideEngine, compile_wide, slice_vars_for_order
+    m = META["c4"]
+    G = pf.graph(m)
+    plans = []
+    for e in G.edges():
+        comp = DefaultQAOAComposer(G, gamma=[0.0] * 3, beta=[0.0] * 3)
+        comp.energy_expectation_lightcone(e)
+        graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+        order, degs = greedy_peo(graph)
+        s = max(0, max(degs) - 26)
+        plans.append(compile_wide(comp.circuit, order=order,
+                                  par_ids=slice_vars_for_order(graph, order, s) if s else []))
+    e64, e128 = WideEngine(dtype="complex64"), WideEngine(dtype="complex128")
+    for si, st in enumerate(m["sets"]):
+        prm = list(st["gamma"]) + list(st["beta"])
+        z64 = [e64.run(wp, prm) for wp in plans]
+        z128 = [e128.run(wp, prm) for wp in plans]
+        _check(z64, z128)
+        ref = ARR[f"c4_{st['index']}"]
+        _check([z128[k] for k in m["oracle_edges"]], ref, lambda r: 1e-10)
+
+
+def test_c5_served_trace_alone_vs_mix_and_oracle(gpu):
+    """The C5 pool and a Poisson trace (p in 1..4): every served result equals
+    the same request served alone (<= 1e-12) and the oracle (p <= 3); p = 4
+    on the N=64 pool graph is rejected as the reference's planner would
+    ("Estimated OOM")."""
+    from qtensor_amd.serving import EnergyServer
+    m = META["c5"]
+    pool = [nx.random_regular_graph(3, g["n"], seed=g["seed"]) for g in m["pool"]]
+    eng = LightconeEngine(shape=LaunchShape(64, 4096, 2048), max_tw=11, slicer="tree_trim")
+    eng4 = LightconeEngine(shape=LaunchShape(256, 4096, 2048), max_tw=16, slicer="tree_trim")
+    server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=16).start()
+    try:
+        # plan every pool graph the trace uses first, as a server with a warm plan store
+        for p in sorted({r["p"] for r in m["requests"]}):
+            for gi in sorted({r["graph"] for r in m["requests"] if r["p"] == p}):
+                try:
+                    server.submit(pool[gi], [0.1] * p, [0.2] * p, key=m["pool"][gi]["key"]).result()
+                except MemoryError:
+                    pass
+        subs = []
+        t_next = time.perf_counter()
+        for r in m["requests"]:
+            t_next += r["gap_s"]
+            time.sleep(max(0.0, t_next - time.perf_counter()))
+            subs.append(server.submit(pool[r["graph"]], r["gamma"], r["beta"], key=m["pool"][r["graph"]]["key"]))
+        mixed = []
+        for f in subs:
+            try:
+                mixed.append(f.result(timeout=120))
+            except MemoryError as exc:
+                mixed.append(exc)
+        for k, r in enumerate(m["requests"]):
+            f = server.submit(pool[r["graph"]], r["gamma"], r["beta"], key=m["pool"][r["graph"]]["key"])
+            try:
+                alone = f.result(timeout=120)
+            except MemoryError as exc:
+                alone = exc
+            G = pool[r["graph"]]
+            if isinstance(alone, MemoryError):
+                assert isinstance(mixed[k], MemoryError) and r["p"] == 4 and G.number_of_nodes() == 64
+                continue
+            (C_mix, zz_mix), (C_alone, zz_alone) = mixed[k], alone
+            assert len(zz_mix) == G.number_of_edges()
+            assert max(abs(a - b) for a, b in zip(zz_mix, zz_alone)) <= 1e-12
+            key = f"c5_{k}"
+            if r["p"] <= 3:
+                _check(zz_mix, ARR[key])
+    finally:
+        server.stop()

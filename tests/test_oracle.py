@@ -190,3 +190,83 @@ def test_oracle_greedy_known_answers():
     g.add_nodes_from([3, 1, 2])
     g.add_edges_from([(3, 1), (1, 2)])
     assert oo.greedy_peo(g)[0][0] == 3
+
+
+@pytest.mark.parametrize("d,n,p,seed", [(3, 10, 1, 1), (3, 12, 2, 2), (3, 14, 3, 3), (4, 10, 2, 4), (5, 10, 1, 5)])
+def test_causal_cone_network_vs_statevector(d, n, p, seed):
+    """The causal-cone oracle (no engine code) equals the full statevector on
+    every edge, and the p=1 closed form."""
+    from oracle import qaoa_network as qn
+    G = nx.random_regular_graph(d, n, seed=seed)
+    rng = np.random.default_rng(seed)
+    g, b = list(rng.uniform(0, np.pi, p)), list(rng.uniform(0, np.pi / 2, p))
+    sv = statevector.zz_expectations(G, g, b)
+    for e in G.edges():
+        z = qn.edge_zz(G, e, g, b)
+        assert abs(z - sv[e]) < 1e-12
+        if p == 1:
+            assert abs(z - closed_form.zz_p1(G, e, g[0], b[0])) < 1e-12
+
+
+def test_causal_cone_network_planted_mistakes():
+    """Dropping one cost edge of the cone, or one mixer, changes the value:
+    the statevector comparison above would catch either."""
+    from oracle import qaoa_network as qn
+    G = nx.random_regular_graph(3, 12, seed=7)
+    g, b = [0.31, 0.12], [0.27, 0.44]
+    e = list(G.edges())[0]
+    sv = statevector.zz_expectations(G, g, b)[e]
+    good = qn.contract(qn.edge_network(G, e, g, b))[0]
+    assert abs(good - sv) < 1e-12
+    orig = qn.causal_cone
+
+    def drop_edge(G_, edge, p):
+        S0, layers = orig(G_, edge, p)
+        (es, nodes), rest = layers[0], layers[1:]
+        return S0, [(es[1:], nodes)] + rest
+
+    def drop_mixer(G_, edge, p):
+        S0, layers = orig(G_, edge, p)
+        es, nodes = layers[-1]
+        return S0, layers[:-1] + [(es, nodes[1:])]
+
+    for planted in (drop_edge, drop_mixer):
+        qn.causal_cone = planted
+        try:
+            bad = qn.contract(qn.edge_network(G, e, g, b))[0]
+        finally:
+            qn.causal_cone = orig
+        assert abs(bad - sv) > 1e-6
+
+
+def test_parity_fixtures_agree_with_bucket_backend_pipeline():
+    """The full-size fixtures (causal-cone oracle) against the reference-shaped
+    pipeline (composer -> circ2buckets -> greedy -> numpy bucket backend) on a
+    sample of each workload's edges, and against the p = 1 closed form."""
+    import parity_fixtures as pf
+    from qtensor_amd.composer import DefaultQAOAComposer
+    from qtensor_amd.simulator import QAOAQtreeSimulator
+    meta, arr = pf.load()
+    for name, ks in (("c2", [0, 1, 200]), ("c3", [0, 7, 1499])):
+        m = meta[name]
+        G = pf.graph(m)
+        sim = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend())
+        for k in ks:
+            ref = complex(sim._get_edge_energy(G, m["gamma"], m["beta"], tuple(m["edges"][k]))[0])
+            assert abs(ref - arr[name][k]) < 1e-10
+    m = meta["c4"]
+    G = pf.graph(m)
+    sim = QAOAQtreeSimulator(DefaultQAOAComposer, bucket_backend=npf.NumpyBackend())
+    st = m["sets"][0]
+    for j in range(3):            # the three narrowest oracle edges
+        k = m["oracle_edges"][int(np.argsort(m["oracle_widths"])[j])]
+        pos = m["oracle_edges"].index(k)
+        ref = complex(sim._get_edge_energy(G, st["gamma"], st["beta"], tuple(m["edges"][k]))[0])
+        assert abs(ref - arr[f"c4_{st['index']}"][pos]) < 1e-10
+    c5 = meta["c5"]
+    for r, req in enumerate(c5["requests"]):
+        if req["p"] == 1:
+            g = c5["pool"][req["graph"]]
+            G = nx.random_regular_graph(3, g["n"], seed=g["seed"])
+            zz = [closed_form.zz_p1(G, e, req["gamma"][0], req["beta"][0]) for e in G.edges()]
+            assert np.max(np.abs(np.array(zz) - arr[f"c5_{r}"])) < 1e-12
