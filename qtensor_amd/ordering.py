@@ -13,7 +13,9 @@
 * ``TreeTrimSplitter``   -- ``Optimizer.py:178-220``: slice the nodes that weigh most
   in the large elimination neighbourhoods, a fifth of the width excess per round.
 * ``TamakiOptimizer``    -- ``Optimizer.py:161-176``: PACE-2017 heuristic via an
-  external ``tw-heuristic`` executable (Java), used when it is on ``PATH``.
+  external ``tw-heuristic`` executable (Java), used when it is on ``PATH``
+  (greedy with a warning otherwise); ``TamakiTrimSlicing`` (``Optimizer.py:227``)
+  adds tree-trimming slicing.
 * ``RGreedyOptimizer``   -- ``qtensor/optimisation/RGreedy.py:15-77``: pick the next
   node with probability proportional to exp(-(degree - min degree)/temp), keep the
   narrowest of ``repeats`` samples (seeded here).
@@ -255,8 +257,9 @@ class TreeTrimSplitter(SlicesOptimizer):
     here is the one of ``_trim_costs``), remove them and re-order."""
 
     def _split_graph(self, p_graph, max_tw):
-        peo_ints, path = self._get_ordering_ints(p_graph)
-        tw = max(path) if path else 0
+        # start from the order optimize() found (the solver's, for TamakiTrimSlicing)
+        peo_ints = list(getattr(self, "peo_ints", None) or self._get_ordering_ints(p_graph)[0])
+        tw = max(elimination_widths(p_graph, peo_ints)) if peo_ints else 0
         result = []
         self.slice_history = [(0, tw)]
         while tw > max_tw:
@@ -281,19 +284,38 @@ class RGreedyTreeTrimSplitter(RGreedyOrdering, TreeTrimSplitter):
 
 
 class TamakiOptimizer(OrderingOptimizer):
-    """Ordering from the PACE-2017 ``tw-heuristic`` solver run for ``wait_time`` s."""
+    """Ordering from the PACE-2017 ``tw-heuristic`` solver run for ``wait_time`` s
+    (``qtensor/optimisation/Optimizer.py:161-176``).  The solver reads the line
+    graph in PACE ``.gr`` format on stdin and prints tree decompositions; the
+    last one printed when it is stopped is turned into an elimination order.
+    Like the reference, only the first ordering of ``optimize`` comes from
+    the solver (``_get_ordering``); re-orderings inside slicers use the greedy
+    rule (``_get_ordering_ints``).  When ``tw-heuristic`` is not on ``PATH``,
+    or returns no decomposition, the greedy order is used with a warning."""
 
     def __init__(self, *args, wait_time=5, **kwargs):
+        super().__init__(*args, **kwargs)
         self.wait_time = wait_time
 
-    def _get_ordering_ints(self, graph):
+    def _get_ordering(self, graph):
+        try:
+            peo, path = self._tamaki_ordering_ints(graph)
+        except _SolverUnavailable as exc:
+            import warnings
+            warnings.warn(f"TamakiOptimizer: {exc}; using the greedy order", RuntimeWarning)
+            return super()._get_ordering(graph)
+        return _vars(graph, peo), path
+
+    def _tamaki_ordering_ints(self, graph):
         exe = shutil.which("tw-heuristic")
         if exe is None:
-            raise RuntimeError("TamakiOptimizer needs the 'tw-heuristic' executable on PATH")
+            raise _SolverUnavailable("the 'tw-heuristic' executable is not on PATH")
         nodes = list(graph.nodes())
+        if not nodes:
+            return [], []
         pos = {n: k + 1 for k, n in enumerate(nodes)}
-        lines = [f"p tw {len(nodes)} {graph.number_of_edges()}"]
-        lines += [f"{pos[a]} {pos[b]}" for a, b in graph.edges() if a != b]
+        edges = [(a, b) for a, b in graph.edges() if a != b]
+        lines = [f"p tw {len(nodes)} {len(edges)}"] + [f"{pos[a]} {pos[b]}" for a, b in edges]
         with tempfile.NamedTemporaryFile("w", suffix=".gr", delete=False) as f:
             f.write("\n".join(lines) + "\n")
             path = f.name
@@ -307,7 +329,34 @@ class TamakiOptimizer(OrderingOptimizer):
                     out, _ = proc.communicate()
         finally:
             os.unlink(path)
-        return _peo_from_td(out, nodes, graph)
+        try:
+            return _peo_from_td(_last_decomposition(out), nodes, graph)
+        except RuntimeError as exc:
+            raise _SolverUnavailable(str(exc)) from exc
+
+
+class _SolverUnavailable(RuntimeError):
+    pass
+
+
+def _last_decomposition(text):
+    """The last complete ``s td`` block of the solver's output (it may print
+    improving decompositions one after another)."""
+    blocks, cur = [], None
+    for line in text.splitlines():
+        if line.startswith("s td"):
+            cur = [line]
+            blocks.append(cur)
+        elif cur is not None:
+            cur.append(line)
+    if not blocks:
+        return text
+    return "\n".join(blocks[-1])
+
+
+class TamakiTrimSlicing(TamakiOptimizer, TreeTrimSplitter):
+    """Tamaki ordering, then tree-trimming slicing with greedy re-ordering
+    (``qtensor/optimisation/Optimizer.py:227``)."""
 
 
 def _peo_from_td(text, nodes, graph):

@@ -566,3 +566,110 @@ def test_oracle_transpose_and_slice_vs_loops():
             idx = tuple(ranges[a][full[a]] for a in range(r))
             out_idx = tuple(full[a] for a in kept)
             assert out[out_idx] == data[idx]
+
+
+def _td(bags, tree_edges, n, width):
+    lines = [f"s td {len(bags)} {width + 1} {n}"]
+    lines += [f"b {k + 1} " + " ".join(map(str, b)) for k, b in enumerate(bags)]
+    lines += [f"{a} {b}" for a, b in tree_edges]
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("name", ["path", "cycle", "grid"])
+def test_peo_from_td_hand_written_decompositions(name):
+    """PACE .td -> elimination order (leaves of the decomposition first): for
+    these hand-written decompositions the order's width equals the largest
+    bag size minus one, and every node is eliminated once."""
+    from qtensor_amd.ordering import _peo_from_td
+    if name == "path":
+        g = nx.path_graph(6)
+        bags = [[k, k + 1] for k in range(1, 6)]
+    elif name == "cycle":
+        g = nx.cycle_graph(6)
+        bags = [[1, 2, 6], [2, 3, 6], [3, 4, 6], [4, 5, 6]]
+    else:
+        g = nx.grid_2d_graph(3, 3)
+        g = nx.convert_node_labels_to_integers(g, ordering="sorted")     # row-major 0..8
+        bags = [[k, k + 1, k + 2, k + 3] for k in range(1, 7)]
+    tree = [(k, k + 1) for k in range(1, len(bags))]
+    nodes = list(g.nodes())
+    width = max(len(b) for b in bags) - 1
+    peo, widths = _peo_from_td(_td(bags, tree, len(nodes), width), nodes, g)
+    assert sorted(peo) == sorted(nodes)
+    assert max(widths) == width
+    assert widths == elimination_widths(g, peo)
+
+
+def _stub_solver(tmp_path):
+    """A stand-in ``tw-heuristic`` on PATH: reads a .gr on stdin and prints a
+    tree decomposition built from a min-degree elimination (a valid .td)."""
+    exe = tmp_path / "tw-heuristic"
+    exe.write_text(
+        "#!/usr/bin/env python3\n"
+        "import sys\n"
+        "adj = {}\n"
+        "for line in sys.stdin:\n"
+        "    p = line.split()\n"
+        "    if not p or p[0] in ('c',):\n"
+        "        continue\n"
+        "    if p[0] == 'p':\n"
+        "        adj = {v: set() for v in range(1, int(p[2]) + 1)}\n"
+        "        continue\n"
+        "    a, b = int(p[0]), int(p[1])\n"
+        "    adj[a].add(b); adj[b].add(a)\n"
+        "bags, owner = [], {}\n"
+        "live = {v: set(s) for v, s in adj.items()}\n"
+        "while live:\n"
+        "    v = min(live, key=lambda x: (len(live[x]), x))\n"
+        "    nb = live.pop(v)\n"
+        "    for u in nb:\n"
+        "        live[u] |= nb - {u}; live[u].discard(v)\n"
+        "    bags.append(sorted(nb | {v})); owner[v] = len(bags)\n"
+        "edges = []\n"
+        "for k, b in enumerate(bags, start=1):\n"
+        "    later = [owner[u] for u in b if owner[u] > k]\n"
+        "    if later: edges.append((k, min(later)))\n"
+        "print('s td', len(bags), max(len(b) for b in bags), len(adj))\n"
+        "for k, b in enumerate(bags, start=1): print('b', k, *b)\n"
+        "for a, b in edges: print(a, b)\n"
+        "sys.stdout.flush()\n")
+    exe.chmod(0o755)
+    return str(tmp_path)
+
+
+def test_tamaki_falls_back_to_greedy_without_solver(monkeypatch, tmp_path):
+    from qtensor_amd.ordering import TamakiOptimizer
+    monkeypatch.setenv("PATH", str(tmp_path))           # no tw-heuristic anywhere
+    G = nx.random_regular_graph(3, 12, seed=2)
+    comp = QtreeQAOAComposer(G, gamma=[0.2], beta=[0.3])
+    comp.energy_expectation_lightcone(list(G.edges())[0])
+    from qtensor_amd.tensornet import QtreeTensorNet
+    tn = QtreeTensorNet.from_qtree_gates(comp.circuit)
+    with pytest.warns(RuntimeWarning, match="tw-heuristic"):
+        peo, _ = TamakiOptimizer(wait_time=1).optimize(tn)
+    ref, _ = OrderingOptimizer().optimize(QtreeTensorNet.from_qtree_gates(comp.circuit))
+    assert [int(v) for v in peo] == [int(v) for v in ref]
+
+
+def test_tamaki_and_tamaki_trim_slicing_with_stub_solver(monkeypatch, tmp_path):
+    """The solver's decomposition drives the order (TamakiOptimizer) and the
+    first order of TamakiTrimSlicing, whose slices sum to the statevector term
+    (the sliced == unsliced invariant, test_qaoa_energy.py:106-121)."""
+    from qtensor_amd.ordering import TamakiOptimizer, TamakiTrimSlicing
+    from qtensor_amd.simulator import FeynmanSimulator
+    monkeypatch.setenv("PATH", _stub_solver(tmp_path) + os.pathsep + os.environ.get("PATH", ""))
+    G = nx.random_regular_graph(3, 14, seed=3)
+    g, b = [0.21, 0.43], [0.37, 0.12]
+    sv = statevector.zz_expectations(G, g, b)
+    opt = TamakiOptimizer(wait_time=20)
+    sim = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=npf.NumpyBackend(), optimizer=opt)
+    e = list(G.edges())[1]
+    assert abs(sim._get_edge_energy(G, g, b, e)[0] - sv[e]) < 1e-10
+    assert opt.treewidth == max(elimination_widths(opt.graph, opt.peo_ints))
+    tts = TamakiTrimSlicing(max_tw=opt.treewidth - 2, tw_bias=0, wait_time=20)
+    comp = QtreeQAOAComposer(G, gamma=g, beta=b)
+    comp.energy_expectation_lightcone(e)
+    fs = FeynmanSimulator(bucket_backend=npf.NumpyBackend(), optimizer=tts)
+    val = fs.simulate(comp.circuit, tw_bias=0)
+    assert len(fs.parallel_vars) >= 1 and tts.treewidth <= opt.treewidth - 2
+    assert abs(complex(val[0]) - sv[e]) < 1e-10
