@@ -9,12 +9,17 @@ buffers resident on the device: the executor launch and the FP64 term
 reductions issued on one stream (the same launches replayed from a captured
 HIP graph are reported beside the value; they are slower on this stack).
 
-Reported beside the value: the plain per-bucket GPU backend on a sample of
-edges, the error of every term against a complex128 run of the same
-kernels and of a sample against the numpy oracle, the executor kernel's
-achieved step-model bandwidth against the HBM peak, and the oracle timed on
-the host (the CPU baseline: a process pool over edges, with single-core
-complex128 / complex64 rates and the planning / contraction split beside it).
+Reported beside the value: the error of every term against the committed
+complex128 oracle values (the value is withheld if any term misses the 1e-5
+bar), a cold evaluation (empty plan cache: planning included), the forced
+treewidth-30 edge on the wide path and the whole C2 job (384 batched terms +
+that edge), the same terms with the reference CLI's composer, the plain
+per-bucket GPU backend on a sample of edges, the executor kernel's achieved
+step-model bandwidth against the HBM peak and what limits it, the oracle timed
+on the host (the CPU baseline: a process pool over edges, with single-core
+rates and the planning / contraction split beside it), and a bounded serving
+leg (Poisson arrivals against a cold then a warm plan store: p50 / p99 and
+saturation terms/s).
 """
 import argparse
 import json
@@ -172,34 +177,86 @@ def plain_form(G, gamma, beta, n_edges):
     return vals, time.perf_counter() - t0
 
 
-def forced_edge(G, gamma, beta, executor_value=None, n_slice_vars=3, target_tw=30):
+def forced_edge(G, gamma, beta, reference=None, n_slice_vars=3, target_tw=30, steps=5):
     """C2's extra edge: the first edge under a deliberately wide order (seeded
     randomized greedy, temperature 8, first sample of width 30), sliced over
-    ``n_slice_vars`` variables, contracted with one grid-wide step kernel per
-    step (the plain per-bucket regime at large rank).  Checked against
-    ``executor_value`` (the batched executor's term for the same edge) or, if
-    None, against the greedy-ordered wide plan of the same edge."""
+    ``n_slice_vars`` variables chosen by the reference's GreedyParvars rule
+    (max degree first, qtensor/optimisation/Greedy.py:78-92) with the order kept
+    for the remaining variables, on the wide path (grid-wide fused step kernels,
+    slices on 4 streams).  ``reference``: the oracle's value of that edge term
+    (None: the greedy-ordered wide plan of the same edge)."""
+    import numpy as np
     from qtensor_amd.composer import QtreeQAOAComposer
     from qtensor_amd.tensornet import QtreeTensorNet
-    from qtensor_amd.wide import WideEngine, compile_wide, forced_width_order, slice_vars_for_order
+    from qtensor_amd.wide import WideEngine, compile_wide, forced_width_order, greedy_parvars_for_order
     e = list(G.edges())[0]
+    t0 = time.perf_counter()
     comp = QtreeQAOAComposer(G, gamma=gamma, beta=beta)
     comp.energy_expectation_lightcone(e)
     graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
     order = forced_width_order(graph, target_tw, temp=8.0, seed=0)
-    wp = compile_wide(comp.circuit, order=order, par_ids=slice_vars_for_order(graph, order, n_slice_vars))
+    par, width = greedy_parvars_for_order(graph, order, n_slice_vars)
+    wp = compile_wide(comp.circuit, order=order, par_ids=par)
+    plan_s = time.perf_counter() - t0
     eng = WideEngine()
-    if executor_value is None:
-        executor_value = eng.run(compile_wide(comp.circuit), list(gamma) + list(beta))
-    eng.run(wp, list(gamma) + list(beta))                       # warm
-    val, secs = eng.run(wp, list(gamma) + list(beta), timing=True)
+    prm = list(gamma) + list(beta)
+    ref_kind = "oracle"
+    if reference is None:
+        reference = eng.run(compile_wide(comp.circuit), prm)
+        ref_kind = "greedy-ordered wide plan"
+    eng.run(wp, prm)                                            # warm
+    times, val = [], None
+    for _ in range(steps):
+        val, secs = eng.run(wp, prm, timing=True)
+        times.append(secs)
+    secs = float(np.median(times))
     nbytes = wp.plan.bytes_step_model * wp.n_slices
     gbs = nbytes / secs / 1e9
     return {"edge": list(e), "order": "randomized greedy T=8 seed 0, first width-30 sample",
-            "treewidth_unsliced": target_tw, "slices": wp.n_slices, "treewidth_per_slice": wp.treewidth,
-            "arena_GB": round(wp.plan.glb_need * 8 / 1e9, 2), "step_model_GB": round(nbytes / 1e9, 2),
-            "seconds": round(secs, 5), "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-            "abs_err_vs_greedy_plan": abs(val - executor_value)}
+            "slicer": "GreedyParvars (max degree), order kept", "slice_vars": len(par),
+            "treewidth_unsliced": target_tw, "slices": wp.n_slices, "treewidth_per_slice": width,
+            "arena_GB": round(wp.plan.glb_need * 8 / 1e9, 3), "step_model_GB": round(nbytes / 1e9, 3),
+            "plan_seconds": round(plan_s, 3), "seconds": round(secs, 6), "achieved_GBs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "abs_err": abs(val - reference), "reference": ref_kind}
+
+
+def cold_evaluation(G, p, gamma, beta, shape, max_tw, slicer, ordering, composer=None):
+    """One energy evaluation from nothing: empty plan cache, host planning,
+    upload, one executor launch and the term reductions, terms on the host."""
+    import torch
+    from qtensor_amd.batched import LightconeEngine, clear_plan_cache
+    from qtensor_amd.composer import QtreeQAOAComposer
+    clear_plan_cache()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng = LightconeEngine(composer=composer or QtreeQAOAComposer, shape=shape, max_tw=max_tw, slicer=slicer,
+                          ordering=ordering)
+    wl = eng.workload(G, p)
+    t1 = time.perf_counter()
+    b = eng.batch([(wl, gamma, beta)])
+    b.run()
+    terms = b.terms()[0]
+    t2 = time.perf_counter()
+    return {"seconds": round(t2 - t0, 4), "plan_seconds": round(t1 - t0, 4), "upload_and_eval_seconds":
+            round(t2 - t1, 4), "terms_per_s": round(len(terms) / (t2 - t0), 1)}
+
+
+def oracle_fixture(name, G):
+    """Committed complex128 oracle values of a workload's edge terms
+    (tests/golden/parity.*, written on the CPU by oracle.qaoa_network), or None
+    when the fixture's edges are not this graph's."""
+    import numpy as np
+    base = os.path.join(ROOT, "tests", "golden")
+    try:
+        with open(os.path.join(base, "parity.json")) as f:
+            meta = json.load(f)
+        with np.load(os.path.join(base, "parity.npz"), allow_pickle=False) as z:
+            vals = z[name]
+    except (OSError, KeyError):
+        return None
+    if [list(e) for e in G.edges()] != meta[name]["edges"]:
+        return None
+    return vals
 
 
 def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
@@ -273,35 +330,66 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
             "per_set": per_set}
 
 
-def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4),
-                slicer="tree_trim", shape4=None, max_tw4=16, slicer4="tree_trim", ordering="greedy",
-                ordering4="greedy"):
-    """Poisson stream of energy requests (C5) through the continuous-batching server.
-    Depths 1-3 use ``shape`` (default 64-thread items) sliced below treewidth
-    ``max_tw``; depth 4 uses ``shape4`` (default 256-thread items) sliced below
-    ``max_tw4``, with larger tensors in global scratch; ``slicer`` and
-    ``slicer4`` pick their slice variables (tree trimming needs about half
-    the work items of max-degree slices on these graphs; with max-degree
-    slices at p = 4 the server saturates at about a third of the rate).
-    ``ordering`` and ``ordering4`` are the lightcones' elimination orders at
-    depths 1-3 and at depth 4."""
+def serve_pool(pool_size=8, seed=5):
+    """The serving pool: random 3-regular graphs, N = 2 * U{10..1000} from
+    default_rng(seed), graph seeds 100, 101, ..."""
     import networkx as nx
     import numpy as np
-    from qtensor_amd.batched import LaunchShape, LightconeEngine
-    from qtensor_amd.serving import EnergyServer, poisson_trace, trace_failures
     rng = np.random.default_rng(seed)
     pool = []
     for k in range(pool_size):
         n = int(rng.integers(10, 1001)) * 2
         pool.append((f"rr3_n{n}_s{k}", nx.random_regular_graph(3, n, seed=100 + k)))
+    return pool, rng
+
+
+def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4),
+                slicer="tree_trim", shape4=None, max_tw4=16, slicer4="tree_trim", ordering="greedy",
+                ordering4="greedy", cold_rate=None, cold_requests=0):
+    """Poisson streams of energy requests (C5) through the continuous-batching server.
+    Depths 1-3 use ``shape`` (default 64-thread items) sliced below treewidth
+    ``max_tw``; depth 4 uses ``shape4`` (default 256-thread items) sliced below
+    ``max_tw4``, with larger tensors in global scratch; ``slicer`` and
+    ``slicer4`` pick their slice variables.  ``ordering`` and ``ordering4`` are
+    the lightcones' elimination orders at depths 1-3 and at depth 4.
+
+    With ``cold_requests`` > 0 a first trace at ``cold_rate`` runs against an
+    empty plan store: new graphs are planned while the server runs, and their
+    latency (and that of requests queued behind them) counts.  The pool is then
+    planned completely outside the timing and one trace runs per rate in
+    ``rates``."""
+    import numpy as np
+    from qtensor_amd.batched import LaunchShape, LightconeEngine, clear_plan_cache
+    from qtensor_amd.serving import EnergyServer, poisson_trace, trace_failures
+    pool, rng = serve_pool(pool_size, seed)
+    clear_plan_cache()
     eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw, slicer=slicer,
                           ordering=ordering)
     eng4 = LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=max_tw4, slicer=slicer4,
                            ordering=ordering4)
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
     server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=workers).start()
+    out = {"pool": [k for k, _ in pool], "p_choices": list(p_choices), "planning_workers": workers, "rates": []}
+
+    def trace(rate, n, label):
+        server.finished.clear()
+        server.rejected = 0
+        subs = poisson_trace(server, pool, rate, n, rng, p_choices=p_choices)
+        st = server.stats()
+        st["rate_per_s"] = rate
+        st["trace"] = label
+        fails = trace_failures(subs)
+        st["failed"] = len(fails)
+        st["failure_kinds"] = sorted({m.split(":")[0] + ": " + m.split(":")[1][:40] for m in fails})
+        print(f"[serve] {label} rate {rate}/s: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms "
+              f"{st['terms_per_s']:.0f} terms/s, rejected {st['rejected']}, failed {st['failed']}",
+              file=sys.stderr, flush=True)
+        return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
+
+    if cold_requests:
+        out["cold_trace"] = trace(cold_rate, cold_requests, "cold plan store")
     cold = {}
-    for p in p_choices:                  # cold planning, outside the timed traces
+    for p in p_choices:                  # planning of the rest of the pool, outside the timed traces
         t0 = time.perf_counter()
         for key, G in pool:
             f = server.submit(G, [0.1] * p, [0.2] * p, key=key)
@@ -309,24 +397,11 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
                 f.result()
             except MemoryError:
                 pass
-        cold[p] = round(time.perf_counter() - t0, 1)
+        cold[p] = round(time.perf_counter() - t0, 2)
         print(f"[serve] pool planned for p={p} in {cold[p]} s", file=sys.stderr, flush=True)
-    rejected = server.rejected
-    out = {"pool": [k for k, _ in pool], "p_choices": list(p_choices), "cold_planning_s": cold,
-           "planning_workers": workers, "rejected_at_planning": rejected, "rates": []}
+    out["pool_planning_s"] = cold
     for rate in rates:
-        server.finished.clear()
-        server.rejected = 0
-        subs = poisson_trace(server, pool, rate, n_requests, rng, p_choices=p_choices)
-        st = server.stats()
-        st["rate_per_s"] = rate
-        fails = trace_failures(subs)
-        st["failed"] = len(fails)
-        st["failure_kinds"] = sorted({m.split(":")[0] + ": " + m.split(":")[1][:40] for m in fails})
-        out["rates"].append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
-        print(f"[serve] rate {rate}/s: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms "
-              f"{st['terms_per_s']:.0f} terms/s, rejected {st['rejected']}, failed {st['failed']}",
-              file=sys.stderr, flush=True)
+        out["rates"].append(trace(rate, n_requests, "warm plan store"))
     server.stop()
     return out
 
@@ -340,13 +415,14 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--plain-edges", type=int, default=24)
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
     ap.add_argument("--shape", default=None,
                     help="fixed launch shape item_threads,lds_capacity,lds_tensor_limit (skips the autotuner)")
     ap.add_argument("--precision", action="store_true",
                     help="run only the precision study (Erdos-Renyi N=64, p=3) and print its JSON line")
     ap.add_argument("--precision-sets", type=int, default=16)
     ap.add_argument("--serve", action="store_true",
-                    help="run only the serving trace (Poisson arrivals) and print its JSON line")
+                    help="run only the serving traces (Poisson arrivals) and print their JSON line")
     ap.add_argument("--serve-rates", default="10,40,160,640")
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--serve-pool", type=int, default=8)
@@ -386,7 +462,7 @@ def main():
         rates = [float(x) for x in args.serve_rates.split(",")]
         ordering = args.ordering or "greedy"
         res = serve_bench(rates, args.serve_requests, args.serve_pool, ordering=ordering,
-                          ordering4=args.serve_ordering4)
+                          ordering4=args.serve_ordering4, cold_rate=20.0, cold_requests=40)
         best = max(res["rates"], key=lambda r: r["terms_per_s"])
         print(json.dumps({"metric": "served edge-energy terms/sec and p99 request latency",
                           "value": best["terms_per_s"], "unit": "terms/s", "n_gpus": 1,
@@ -397,8 +473,9 @@ def main():
         return
 
     G, p, gamma, beta, desc = workload(args.workload)
+    oracle_vals = oracle_fixture(args.workload, G)
     if args.forced_edge:
-        fe = forced_edge(G, gamma, beta)
+        fe = forced_edge(G, gamma, beta, reference=None if oracle_vals is None else complex(oracle_vals[0]))
         print(json.dumps({"metric": "forced treewidth-30 edge: step-model GB/s", "value": fe["achieved_GBs"],
                           "unit": "GB/s", "n_gpus": 1, "higher_is_better": True, "dtype": "complex64",
                           "data": "synthetic", "config": {"workload": desc + "_forced_tw30_edge"},
@@ -422,17 +499,16 @@ def main():
     print(f"[bench] tuned: {shape.key()} max_tw={max_tw} {slicer} {ordering} ({tune_s:.1f} s)",
           file=sys.stderr, flush=True)
 
-    from qtensor_amd.batched import clear_plan_cache
-    clear_plan_cache()                  # the autotuner compiled these plans already
+    # cold: one evaluation from an empty plan cache (planning, upload, launch, terms)
+    cold = cold_evaluation(G, p, gamma, beta, shape, max_tw, slicer, ordering)
+    print(f"[bench] cold evaluation: {cold['seconds']} s ({cold['plan_seconds']} s planning), "
+          f"{cold['terms_per_s']} terms/s", file=sys.stderr, flush=True)
     eng = LightconeEngine(shape=shape, max_tw=max_tw, slicer=slicer, ordering=ordering)
     t0 = time.perf_counter()
     wl = eng.workload(G, p)
-    plan_cold = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    eng.workload(G, p)
     plan_warm = time.perf_counter() - t0
     # algorithmic bytes: the unsliced greedy plans of the same networks
-    wl_alg = LightconeEngine(shape=shape).workload(G, p)
+    wl_alg = LightconeEngine(shape=LaunchShape(256, 8192, 4096)).workload(G, p)
     alg_bytes = wl_alg.step_model_bytes()
     batch = eng.batch([(wl, gamma, beta)])
     # a step is the executor launch and the FP64 reductions issued on the
@@ -513,17 +589,34 @@ def main():
                           "terms_per_s": round(value, 1), "calibration_copy_bytes": 2 << 30}), flush=True)
         return
 
-    # error against FP64 (complex128 run of the same plans)
+    # error: every term against the committed complex128 oracle values and
+    # against a complex128 run of the same kernels
     eng128 = LightconeEngine(dtype="complex128", shape=shape)
     b128 = eng128.batch([(eng128.workload(G, p), gamma, beta)])
     b128.run()
     ref128 = np.array(b128.terms()[0])
     err_fp64 = float(np.max(np.abs(terms64 - ref128) / np.maximum(1.0, np.abs(ref128))))
+    err_oracle = None
+    if oracle_vals is not None:
+        err_oracle = float(np.max(np.abs(terms64 - oracle_vals) / np.maximum(1.0, np.abs(oracle_vals))))
+    worst = err_oracle if err_oracle is not None else err_fp64
+    meets = worst <= 1e-5
     energy = (n_terms - math.fsum(terms64.real)) / 2
 
+    traffic = EXEC_TRAFFIC_BYTES if (args.workload == "c2" and batch.scratch_elems == 0 and
+                                     (shape.item_threads, max_tw, slicer, ordering, batch.n_items)
+                                     == EXEC_TRAFFIC_CONFIG) else None
+    # which resource limits the executor: HBM if its measured traffic moves at
+    # a large fraction of the peak, otherwise instruction issue and the latency
+    # of its dependent op chains (all tensor data stay in LDS)
+    hbm_rate = None if traffic is None else traffic / (kern_ms * 1e-3) / 1e9
+    limiter = ("hbm" if hbm_rate is not None and hbm_rate >= 0.5 * HBM_PEAK_GBS else
+               "issue and dependent-op latency (LDS-resident data; measured HBM traffic "
+               + ("unknown for this configuration)" if hbm_rate is None else
+                  f"{hbm_rate:.0f} GB/s = {hbm_rate / HBM_PEAK_GBS:.0%} of peak)"))
     out = {
         "metric": "edge-energy terms/sec at <=1e-5 err vs FP64 (batched form)",
-        "value": round(value, 1), "unit": "terms/s", "n_gpus": 1, "steps": args.steps,
+        "value": round(value, 1) if meets else None, "unit": "terms/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "complex64",
         "data": "synthetic (seeded networkx graph and angles)",
@@ -534,27 +627,55 @@ def main():
                    "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw, "slicer": slicer,
                    "work_items": batch.n_items},
         "autotune": {"seconds": round(tune_s, 1), "candidates": len(table)},
-        "max_rel_err_vs_fp64": err_fp64,
+        "max_rel_err_vs_oracle": err_oracle, "max_rel_err_vs_fp64_same_kernels": err_fp64,
+        "meets_1e-5": meets,
         "energy": energy,
-        "plan_seconds": {"cold": round(plan_cold, 3), "warm": round(plan_warm, 3)},
+        "cold": cold,
+        "plan_seconds": {"cold": cold["plan_seconds"], "warm": round(plan_warm, 3)},
         "graph_replay_ms_per_step": None if graph_ms is None else round(graph_ms, 4),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": EXEC_TRAFFIC_BYTES if (args.workload == "c2" and batch.scratch_elems == 0 and
-                                                       (shape.item_threads, max_tw, slicer, ordering,
-                                                        batch.n_items)
-                                                       == EXEC_TRAFFIC_CONFIG) else None,
+        "roofline": {"bound": "hbm", "limiter": limiter, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
                      "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,
                      "executed_step_bytes_per_launch": batch.bytes_step_model,
                      "measured_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1)},
     }
-    if args.workload == "c2":            # the extra wide edge belongs to the N=256, p=3 workload
-        fe = forced_edge(G, gamma, beta, terms64[0])
+    if not meets:
+        out["error"] = f"a term misses the 1e-5 bar (max rel err {worst:.3e}): value withheld"
+    if args.workload == "c2":
+        # the extra wide edge belongs to the N=256, p=3 workload: the whole C2 job
+        # is the 384 batched terms plus that edge on the wide path
+        fe = forced_edge(G, gamma, beta, reference=None if oracle_vals is None else complex(oracle_vals[0]))
         out["forced_edge"] = fe
-        print(f"[bench] forced edge: tw {fe['treewidth_unsliced']} in {fe['slices']} slices, "
-              f"{fe['step_model_GB']} GB in {fe['seconds']*1e3:.2f} ms = {fe['achieved_GBs']} GB/s",
-              file=sys.stderr, flush=True)
+        whole_s = ms_per_step * 1e-3 + fe["seconds"]
+        out["c2_whole_job"] = {"terms": n_terms + 1, "seconds": round(whole_s, 6),
+                               "terms_per_s": round((n_terms + 1) / whole_s, 1),
+                               "note": "384 batched edge terms (one step) + the forced treewidth-30 edge"}
+        print(f"[bench] forced edge: tw {fe['treewidth_unsliced']} -> {fe['treewidth_per_slice']} in "
+              f"{fe['slices']} slices, {fe['step_model_GB']} GB in {fe['seconds']*1e3:.2f} ms; whole C2 job "
+              f"{out['c2_whole_job']['terms_per_s']:.0f} terms/s", file=sys.stderr, flush=True)
+        # the reference energy CLI's composer (ZZ gates + simplification,
+        # qtensor/cli.py:301): same terms, smaller networks
+        from qtensor_amd.composer import DefaultQAOAComposer
+        eng_cli = LightconeEngine(composer=DefaultQAOAComposer, shape=LaunchShape(256, 4096, 2048), max_tw=10,
+                                  slicer="tree_trim", ordering=ordering)
+        b_cli = eng_cli.batch([(eng_cli.workload(G, p), gamma, beta)])
+        for _ in range(args.warmup):
+            b_cli.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b_cli.run(sync=False)
+        torch.cuda.synchronize()
+        cli_s = (time.perf_counter() - t0) / args.steps
+        t_cli = np.array(b_cli.terms()[0])
+        wl_cli_alg = LightconeEngine(composer=DefaultQAOAComposer, shape=LaunchShape(256, 8192, 4096)).workload(G, p)
+        out["cli_composer"] = {"composer": "DefaultQAOAComposer (ZZ + simplify)", "terms_per_s": round(n_terms / cli_s, 1),
+                               "ms_per_step": round(cli_s * 1e3, 4), "work_items": b_cli.n_items,
+                               "step_model_bytes_per_launch": wl_cli_alg.step_model_bytes(),
+                               "max_rel_err_vs_oracle": None if oracle_vals is None else float(
+                                   np.max(np.abs(t_cli - oracle_vals) / np.maximum(1.0, np.abs(oracle_vals))))}
+        print(f"[bench] CLI composer: {n_terms / cli_s:.1f} terms/s", file=sys.stderr, flush=True)
     if not args.no_baselines:
         n_plain = min(args.plain_edges, n_terms)
         pv, pdt = plain_form(G, gamma, beta, n_plain)
@@ -576,6 +697,14 @@ def main():
             "cpu_model": cb["cpu_model"], "numpy_blas": cb["blas"],
             "pool_vs_single_max_abs_diff": cb["pool_max_abs_diff"]}
         out["max_abs_err_vs_oracle_sample"] = float(np.max(np.abs(terms64[:len(cv)] - cv)))
+    if args.workload == "c2" and not args.no_serve:
+        # C5: a bounded serving leg (cold plan store, then a warm trace at saturation)
+        res = serve_bench([2000.0], 120, 8, cold_rate=20.0, cold_requests=40)
+        warm = res["rates"][0]
+        out["served"] = {"workload": "poisson_energy_requests_3reg_n20_2000_p1_4",
+                         "cold_trace": res["cold_trace"], "saturation_trace": warm,
+                         "pool_planning_s": res["pool_planning_s"],
+                         "p50_ms": warm["p50_ms"], "p99_ms": warm["p99_ms"], "terms_per_s": warm["terms_per_s"]}
     print(json.dumps(out), flush=True)
 
 
