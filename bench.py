@@ -177,26 +177,40 @@ def plain_form(G, gamma, beta, n_edges):
     return vals, time.perf_counter() - t0
 
 
-def forced_edge(G, gamma, beta, reference=None, n_slice_vars=3, target_tw=30, steps=5):
+def forced_edge(G, gamma, beta, reference=None, target_tw=30, steps=5):
     """C2's extra edge: the first edge under a deliberately wide order (seeded
-    randomized greedy, temperature 8, first sample of width 30), sliced over
-    ``n_slice_vars`` variables chosen by the reference's GreedyParvars rule
-    (max degree first, qtensor/optimisation/Greedy.py:78-92) with the order kept
-    for the remaining variables, on the wide path (grid-wide fused step kernels,
-    slices on 4 streams).  ``reference``: the oracle's value of that edge term
-    (None: the greedy-ordered wide plan of the same edge)."""
+    randomized greedy, temperature 8, first sample of width 30), sliced over 2-4
+    variables with the order kept for the rest, on the wide path (grid-wide
+    fused step kernels, slices on 4 streams).  Slice variables: for 2, 3 and 4
+    variables both the reference's GreedyParvars rule (max degree first,
+    qtensor/optimisation/Greedy.py:78-92) and the order-aware neighbourhood rule
+    (wide.slice_vars_for_order) are evaluated, and the choice with the smallest
+    slices x sum of 2^width runs (GreedyParvars on ties).  ``reference``: the
+    oracle's value of that edge term (None: the greedy-ordered wide plan of the
+    same edge)."""
     import numpy as np
     from qtensor_amd.composer import QtreeQAOAComposer
+    from qtensor_amd.ordering import elimination_widths
     from qtensor_amd.tensornet import QtreeTensorNet
-    from qtensor_amd.wide import WideEngine, compile_wide, forced_width_order, greedy_parvars_for_order
+    from qtensor_amd.wide import (WideEngine, compile_wide, forced_width_order, greedy_parvars_for_order,
+                                  slice_vars_for_order)
     e = list(G.edges())[0]
     t0 = time.perf_counter()
     comp = QtreeQAOAComposer(G, gamma=gamma, beta=beta)
     comp.energy_expectation_lightcone(e)
     graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
     order = forced_width_order(graph, target_tw, temp=8.0, seed=0)
-    par, width = greedy_parvars_for_order(graph, order, n_slice_vars)
-    wp = compile_wide(comp.circuit, order=order, par_ids=par)
+    choices = []
+    for n_vars in (2, 3, 4):
+        for rule in ("GreedyParvars", "neighbourhood"):
+            par = (greedy_parvars_for_order(graph, order, n_vars)[0] if rule == "GreedyParvars"
+                   else slice_vars_for_order(graph, order, n_vars))
+            rest = [v for v in order if v not in set(par)]
+            ws = elimination_widths(graph.subgraph(rest), rest)
+            cost = (1 << len(par)) * sum(2.0 ** w for w in ws)
+            choices.append({"rule": rule, "slice_vars": len(par), "width": max(ws), "cost": cost, "par": par})
+    best = min(choices, key=lambda c: (c["cost"], c["rule"] != "GreedyParvars"))
+    wp = compile_wide(comp.circuit, order=order, par_ids=best["par"])
     plan_s = time.perf_counter() - t0
     eng = WideEngine()
     prm = list(gamma) + list(beta)
@@ -213,8 +227,9 @@ def forced_edge(G, gamma, beta, reference=None, n_slice_vars=3, target_tw=30, st
     nbytes = wp.plan.bytes_step_model * wp.n_slices
     gbs = nbytes / secs / 1e9
     return {"edge": list(e), "order": "randomized greedy T=8 seed 0, first width-30 sample",
-            "slicer": "GreedyParvars (max degree), order kept", "slice_vars": len(par),
-            "treewidth_unsliced": target_tw, "slices": wp.n_slices, "treewidth_per_slice": width,
+            "slicer": best["rule"] + " (order kept)", "slice_vars": best["slice_vars"],
+            "candidates": [{k: c[k] for k in ("rule", "slice_vars", "width")} for c in choices],
+            "treewidth_unsliced": target_tw, "slices": wp.n_slices, "treewidth_per_slice": wp.treewidth,
             "arena_GB": round(wp.plan.glb_need * 8 / 1e9, 3), "step_model_GB": round(nbytes / 1e9, 3),
             "plan_seconds": round(plan_s, 3), "seconds": round(secs, 6), "achieved_GBs": round(gbs, 1),
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "abs_err": abs(val - reference), "reference": ref_kind}

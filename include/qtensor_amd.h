@@ -36,6 +36,9 @@
  *                        (RGreedyOptimizer, qtensor/optimisation/RGreedy.py:15-77)
  *   qb_place_intervals -> LDS / scratch offsets of a plan's buffers (the plan compiler's
  *                        own layout step; the reference allocates with numpy)
+ *   qb_lightcone_classes / qb_compile_lightcone -> per-edge lightcone, circuit,
+ *                        network, order, slices and plan (QAOASimulator.py:16-24,
+ *                        Simulate.py:47-94, CircuitComposer.py:82-169), natively
  *   qb_init / qb_shutdown / qb_last_error -> backend construction and the Python
  *                        exceptions of the reference (AssertionError, ValueError)
  */
@@ -280,6 +283,68 @@ int qb_rgreedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj
  */
 int qb_place_intervals(int32_t m, const int64_t* sizes, const int64_t* starts, const int64_t* ends,
                        const int64_t* order, int64_t base, int64_t* offsets, int64_t* peak);
+
+/*
+ * Native lightcone planner (host only; usable without qb_init).
+ *
+ * qb_lightcone_classes replaces the per-edge lightcone extraction of
+ * QtreeQAOAComposer.energy_expectation_lightcone (qtensor/CircuitComposer.py:82-96,
+ * qtensor/utils.py:176-189) for a whole graph: every edge's radius-p ball is
+ * relabelled canonically (colour refinement seeded with endpoint and distance,
+ * then individualisation), and edges whose relabelled lightcones are equal share
+ * a class.  The graph (nodes 0..n_nodes-1) is CSR as for qb_greedy_order; edges
+ * holds n_edges (u, v) pairs.  edge_class[e] receives the class of edge e;
+ * class_words receives, per class in order of first appearance,
+ * [n, n_edges_c, i, j, a0, b0, a1, b1, ...] (canonical labels, a < b, sorted;
+ * (i, j) the edge's own endpoints).  If capacity is too small the call fails with
+ * QB_ERR_UNSUPPORTED and *used tells the size needed.
+ */
+int qb_lightcone_classes(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_idx, int32_t n_edges,
+                         const int32_t* edges, int32_t p, int32_t* edge_class, int32_t* n_classes,
+                         int32_t* class_words, int64_t capacity, int64_t* used);
+
+/*
+ * qb_compile_lightcone replaces, for one lightcone class, the chain that
+ * QAOASimulator._get_edge_energy and QtreeSimulator.simulate_batch run per edge
+ * (qtensor/QAOASimulator.py:16-24, qtensor/Simulate.py:47-94): the QAOA circuit
+ * U Z_i Z_j U^dagger with cost gates as cX.ZPhase.cX (composer 0,
+ * QtreeQAOAComposer), ZZ (1, ZZQtreeQAOAComposer) or ZZ with commutation-based
+ * cancellation (2, DefaultQAOAComposer; qtensor/__init__.py:55-63), the bucket
+ * network, the elimination order (0: min-degree greedy, qtensor/utils.py:12-44;
+ * 1: best of rg_repeats randomized-greedy samples and the greedy order,
+ * RGreedy.py:15-77), slices (0: none; 1: GreedyParvars, Optimizer.py:90-159;
+ * 2: tree trimming, Optimizer.py:178-228) until the width is below
+ * max_tw - tw_bias, and the executor plan of that network (qb_execute's program
+ * format) for work items of `lanes` threads, intermediates of at most
+ * lds_tensor_limit elements in LDS within lds_capacity elements (-1: no limit).
+ * `lightcone` is one class record of qb_lightcone_classes.  Errors: an LDS
+ * capacity that the plan exceeds (QB_ERR_INVALID, "LDS capacity"), a width the
+ * slicer cannot reach (QB_ERR_UNSUPPORTED, "Estimated OOM", as the reference's
+ * Optimizer.py:124).
+ */
+typedef struct qb_lightcone_plan_opts {
+    int32_t composer;          /* 0, 1 or 2 (above) */
+    int32_t ordering;          /* 0 greedy, 1 randomized greedy */
+    int32_t slicer;            /* 0 none, 1 GreedyParvars, 2 tree trimming */
+    int32_t max_tw;
+    int32_t tw_bias;
+    int32_t lanes;             /* 64, 128, 256, 512 or 1024 */
+    int64_t lds_tensor_limit;
+    int64_t lds_capacity;      /* -1: unbounded */
+    double rg_temp;            /* randomized greedy: temperature, samples, seed */
+    int32_t rg_repeats;
+    uint64_t rg_seed;
+} qb_lightcone_plan_opts;
+
+typedef struct qb_lightcone_plan_info {
+    int32_t n_steps, n_ops, n_gates, treewidth, max_rank, n_sliced;
+    int64_t lds_need, glb_need;
+    int64_t bytes_step_model, flops;   /* complex64 step model of the bucket steps */
+    int32_t n_barriers, n_gather, n_reduce, steps_words;
+} qb_lightcone_plan_info;
+
+int qb_compile_lightcone(const int32_t* lightcone, int32_t p, const qb_lightcone_plan_opts* opts, int32_t* words,
+                         int64_t capacity, int64_t* used, qb_lightcone_plan_info* info);
 
 #ifdef __cplusplus
 }

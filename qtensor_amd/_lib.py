@@ -25,7 +25,7 @@ EXPORTED = ("qb_version", "qb_init", "qb_shutdown", "qb_last_error", "qb_step",
             "qb_exec_slots", "qb_execute", "qb_wide_execute", "qb_reduce_terms",
             "qb_greedy_order", "qb_elimination_widths", "qb_rgreedy_order", "qb_place_intervals",
             "qb_plan_create", "qb_plan_execute", "qb_plan_destroy", "qb_check_program",
-            "qb_execute_noalloc")
+            "qb_execute_noalloc", "qb_lightcone_classes", "qb_compile_lightcone")
 # bytes of the static device workspace behind qb_execute_noalloc
 QB_NOALLOC_SCRATCH_BYTES = 64 << 20
 

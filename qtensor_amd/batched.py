@@ -147,11 +147,17 @@ class LightconeEngine:
     """
 
     def __init__(self, composer=QtreeQAOAComposer, device=0, dtype="complex64", shape=None,
-                 max_tw=None, tw_bias=0, slicer="greedy", ordering="greedy"):
+                 max_tw=None, tw_bias=0, slicer="greedy", ordering="greedy", planner="native"):
         if slicer not in ("greedy", "tree_trim"):
             raise ValueError(f"unknown slicer {slicer!r}")
         if ordering not in ("greedy", "rgreedy"):
             raise ValueError(f"unknown ordering {ordering!r}")
+        if planner not in ("native", "python"):
+            raise ValueError(f"unknown planner {planner!r}")
+        # "native": lightcone classes and plans from csrc/lightcone_planner.cpp
+        # (same program words as the Python compiler, which "python" runs and
+        # which other composers always use)
+        self.planner = planner
         self.ordering = ordering
         self.composer = composer
         self.device_index = device
@@ -220,12 +226,48 @@ class LightconeEngine:
             self._plans[ck] = plan
         return plan
 
+    def _native(self):
+        from . import native_planner
+        return self.planner == "native" and native_planner.supports(self.composer)
+
+    def plan_native(self, key, record, p, shape):
+        """``plan`` through the native planner (same plan, same cache)."""
+        from . import native_planner
+        ck = self._cache_key(key, p, shape)
+        with self._lock:
+            hit = self._plans.get(ck)
+        if hit is not None:
+            self._check_fits(hit, shape)
+            return hit
+        lim = shape.lds_tensor_limit
+        while True:
+            try:
+                plan = native_planner.compile_lightcone(
+                    record, p, self.composer, ordering=self.ordering,
+                    slicer=self.slicer if self.max_tw is not None else None, max_tw=self.max_tw,
+                    tw_bias=self.tw_bias, lanes=shape.item_threads, lds_tensor_limit=lim,
+                    lds_capacity=shape.lds_capacity)
+                break
+            except ValueError as exc:
+                if "LDS capacity" not in str(exc) or lim <= shape.item_threads:
+                    raise
+                lim //= 2
+        self._check_fits(plan, shape)
+        with self._lock:
+            self._plans[ck] = plan
+        return plan
+
     def workload(self, G, p, edges=None, workers=None):
-        """Plans of every edge's lightcone.  With ``workers`` > 1 the canonical
-        lightcones and the distinct plans not yet cached are computed in a
-        process pool of that size."""
+        """Plans of every edge's lightcone.  The native planner computes the
+        canonical lightcone classes of all edges in one call and compiles each
+        class not yet cached (on ``workers`` threads, default the host's cores
+        up to 16: the planner runs without the interpreter lock).  The Python planner, with
+        ``workers`` > 1, computes canonical lightcones and plans in a process
+        pool of that size."""
         shape = self.shape_for(p)
         edges = list(G.edges()) if edges is None else list(edges)
+        if self._native():
+            return self._workload_native(G, p, edges, shape, workers)
         if workers and workers > 1 and len(edges) > 1:
             self._plan_parallel(G, p, edges, shape, workers)
         keyed = self._keyed.pop((id(G), p), None) if hasattr(self, "_keyed") else None
@@ -237,6 +279,31 @@ class LightconeEngine:
                 key, n, ce, i, j = canonical_lightcone(G, e, p)
             plan = self.plan(key, n, ce, i, j, p, shape)
             pk = (key, p)
+            plans[pk] = plan
+            works.append(EdgeWork(e, pk, 1 << plan.n_sliced))
+        return GraphWorkload(G, p, edges, works, plans)
+
+    def _workload_native(self, G, p, edges, shape, workers):
+        from . import native_planner
+        edge_class, classes = native_planner.lightcone_classes(G, edges, p)
+
+        def one(c):
+            key, n, ce, i, j, rec = c
+            return self.plan_native(key, rec, p, shape)
+
+        if workers is None:       # default: the host's cores, up to 16
+            import os
+            workers = min(16, len(os.sched_getaffinity(0)))
+        if workers > 1 and len(classes) >= 8:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(min(workers, len(classes))) as ex:
+                plans_c = list(ex.map(one, classes))
+        else:
+            plans_c = [one(c) for c in classes]
+        works, plans = [], {}
+        for e, c in zip(edges, edge_class):
+            pk = (classes[c][0], p)
+            plan = plans_c[c]
             plans[pk] = plan
             works.append(EdgeWork(e, pk, 1 << plan.n_sliced))
         return GraphWorkload(G, p, edges, works, plans)

@@ -39,9 +39,31 @@ def nodes_at_distance(G, nodes, dist):
     return found
 
 
+def _node_key(x):
+    return (str(type(x)), x)
+
+
 def get_edge_subgraph(G, edge, dist):
-    """Radius-``dist`` subgraph around ``edge`` (``qtensor/utils.py:187-189``)."""
-    return G.subgraph(nodes_at_distance(G, edge, dist))
+    """Radius-``dist`` subgraph around ``edge`` (``qtensor/utils.py:187-189``).
+
+    Returned as a new graph whose nodes iterate in ascending order and whose
+    edges iterate as ascending (a, b) pairs of node positions: a networkx
+    subgraph view iterates a small subgraph in the order of its node *set*,
+    which would make the gate order of the composed circuit depend on hashing.
+    The gates a different order moves commute (cost gates with each other,
+    mixers on distinct qubits), so the circuit's value is the same."""
+    keep = nodes_at_distance(G, edge, dist)
+    nodes = sorted(keep, key=_node_key)
+    pos = {v: k for k, v in enumerate(nodes)}
+    pairs = set()
+    for a in nodes:
+        for b in G[a]:
+            if b in keep and b != a:
+                pairs.add((a, b) if pos[a] < pos[b] else (b, a))
+    sub = nx.Graph()
+    sub.add_nodes_from(nodes)
+    sub.add_edges_from(sorted(pairs, key=lambda e: (pos[e[0]], pos[e[1]])))
+    return sub
 
 
 def _dagger(gate):
@@ -239,9 +261,10 @@ def simplify_circuit(circuit):
     """Cancel or merge same-type, same-site gate pairs that commute with
     every gate between them, nearest pairs first.
 
-    Parametric pairs whose angles sum to zero (symbolically, or numerically
-    below 1e-5) are removed; other parametric pairs merge into one gate with
-    the summed angle at the later position.  ``Z`` and ``cX`` pairs cancel;
+    Parametric pairs whose angles sum to zero are removed (symbolically when
+    the gates carry angle terms, else numerically below 1e-5); other
+    parametric pairs merge into one gate with the summed angle at the later
+    position.  ``Z`` and ``cX`` pairs cancel;
     ``H`` pairs are left alone (H*H is the identity, not H).
     """
     gates = list(circuit)
@@ -269,10 +292,13 @@ def simplify_circuit(circuit):
         if not gi.parametric:          # Z*Z = cX*cX = identity
             alive[i] = alive[j] = False
             continue
-        terms = _merge_terms(getattr(gi, "param_terms", ()), getattr(gj, "param_terms", ()))
+        ti, tj = getattr(gi, "param_terms", ()), getattr(gj, "param_terms", ())
+        terms = _merge_terms(ti, tj)
         alpha = gi.alpha + gj.alpha
-        if (terms == () and (getattr(gi, "param_terms", ()) or getattr(gj, "param_terms", ()))) \
-                or abs(alpha) < 1e-5:
+        # symbolic angles decide exactly (a compiled plan must stay valid for
+        # every angle); gates with numeric angles only cancel as the
+        # reference's do, below 1e-5
+        if (terms == () if (ti or tj) else abs(alpha) < 1e-5):
             alive[i] = alive[j] = False
             continue
         merged = type(gi)(*gi.qubits, alpha=alpha)

@@ -10,6 +10,7 @@
 // widths of that order.  Plain host code: no device work.
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -54,6 +55,12 @@ struct Graph {
     // Remove v: its neighbours lose v and become a clique.  Returns its width.
     int32_t eliminate(int32_t v) {
         std::vector<int32_t> nv;
+        return eliminate(v, nv);
+    }
+
+    // The same, leaving v's former neighbours in `nv`.
+    int32_t eliminate(int32_t v, std::vector<int32_t>& nv) {
+        nv.clear();
         nv.swap(nb[v]);
         for (int32_t u : nv) {
             std::vector<int32_t>& s = nb[u];
@@ -76,6 +83,45 @@ struct Graph {
     }
 };
 
+// Live nodes in node order (a doubly linked list) with their neighbour counts,
+// for the elimination loops: each step scans only the nodes still present.
+struct LiveList {
+    std::vector<int32_t> next, prev, deg;
+    int32_t head = -1;
+
+    void reset(const Graph& g) {
+        const int32_t n = (int32_t)g.nb.size();
+        next.resize(n);
+        prev.resize(n);
+        deg.resize(n);
+        for (int32_t v = 0; v < n; ++v) {
+            next[v] = v + 1 < n ? v + 1 : -1;
+            prev[v] = v - 1;
+            deg[v] = (int32_t)g.nb[v].size();
+        }
+        head = n > 0 ? 0 : -1;
+    }
+    void remove(int32_t v) {
+        if (prev[v] >= 0) next[prev[v]] = next[v]; else head = next[v];
+        if (next[v] >= 0) prev[next[v]] = prev[v];
+    }
+    // first live node with the fewest neighbours
+    int32_t first_min(int32_t& lo) const {
+        int32_t best = -1;
+        lo = INT32_MAX;
+        for (int32_t v = head; v >= 0; v = next[v])
+            if (deg[v] < lo) { lo = deg[v]; best = v; }
+        return best;
+    }
+    // eliminate v from g and the list; returns its width
+    int32_t eliminate(Graph& g, int32_t v, std::vector<int32_t>& nv) {
+        remove(v);
+        const int32_t w = g.eliminate(v, nv);
+        for (int32_t u : nv) deg[u] = (int32_t)g.nb[u].size();
+        return w;
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -87,20 +133,14 @@ int qb_greedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj_
     Graph g;
     if (!g.load(n_nodes, adj_ptr, adj_idx))
         return qb_internal::fail(QB_ERR_INVALID, "qb_greedy_order: malformed adjacency");
-    std::vector<char> alive(n_nodes, 1);
+    LiveList live;
+    live.reset(g);
+    std::vector<int32_t> nv;
     for (int32_t step = 0; step < n_nodes; ++step) {
-        int32_t best = -1;
-        size_t best_deg = 0;
-        for (int32_t v = 0; v < n_nodes; ++v) {
-            if (!alive[v]) continue;
-            if (best < 0 || g.nb[v].size() < best_deg) {
-                best = v;
-                best_deg = g.nb[v].size();
-            }
-        }
-        alive[best] = 0;
+        int32_t lo;
+        const int32_t best = live.first_min(lo);
         order[step] = best;
-        widths[step] = g.eliminate(best);
+        widths[step] = live.eliminate(g, best, nv);
     }
     return QB_OK;
 }
@@ -129,38 +169,39 @@ int qb_rgreedy_order(int32_t n_nodes, const int32_t* adj_ptr, const int32_t* adj
         return (double)(z >> 11) * (1.0 / 9007199254740992.0);
     };
     std::vector<int32_t> cur_order(n_nodes), cur_width(n_nodes);
-    std::vector<double> weight(n_nodes);
+    // exp(-d / temp) for every neighbour-count gap d (a node never has more
+    // than n_nodes - 1 neighbours): the same values, computed once
+    std::vector<double> boltzmann((size_t)n_nodes + 1);
+    for (int32_t d = 0; d <= n_nodes; ++d) boltzmann[d] = std::exp(-(double)d / temp);
     int32_t best_max = -1;
     double best_sum = 0.0;
+    Graph g = g0;
+    LiveList live;
+    std::vector<int32_t> nv;
     for (int32_t sample = 0; sample <= repeats; ++sample) {
-        Graph g = g0;
-        std::vector<char> alive(n_nodes, 1);
+        if (sample > 0)
+            for (int32_t v = 0; v < n_nodes; ++v) g.nb[v].assign(g0.nb[v].begin(), g0.nb[v].end());
+        live.reset(g);
         int32_t w_max = 0;
         double w_sum = 0.0;
         for (int32_t step = 0; step < n_nodes; ++step) {
-            size_t lo = SIZE_MAX;
-            int32_t first = -1;
-            for (int32_t v = 0; v < n_nodes; ++v)
-                if (alive[v] && g.nb[v].size() < lo) { lo = g.nb[v].size(); first = v; }
-            int32_t pick = first;
+            int32_t lo;
+            int32_t pick = live.first_min(lo);
             if (sample > 0) {
+                // the Boltzmann weights of the live nodes, in node order (removed
+                // nodes weigh 0 and change no partial sum)
                 double total = 0.0;
-                for (int32_t v = 0; v < n_nodes; ++v) {
-                    weight[v] = alive[v] ? std::exp(-(double)(g.nb[v].size() - lo) / temp) : 0.0;
-                    total += weight[v];
-                }
+                for (int32_t v = live.head; v >= 0; v = live.next[v]) total += boltzmann[live.deg[v] - lo];
                 const double r = uniform() * total;
                 double acc = 0.0;
-                for (int32_t v = 0; v < n_nodes; ++v) {
-                    if (!alive[v]) continue;
-                    acc += weight[v];
+                for (int32_t v = live.head; v >= 0; v = live.next[v]) {
+                    acc += boltzmann[live.deg[v] - lo];
                     pick = v;
                     if (acc > r) break;
                 }
             }
-            alive[pick] = 0;
             cur_order[step] = pick;
-            cur_width[step] = g.eliminate(pick);
+            cur_width[step] = live.eliminate(g, pick, nv);
             w_max = std::max(w_max, cur_width[step]);
             w_sum += std::ldexp(1.0, cur_width[step]);
         }

@@ -152,6 +152,7 @@ class EnergyServer:
         self._pending = queue.Queue()
         self._done_q = queue.Queue()
         self._workloads = {}
+        self._planning = {}           # (key, p) -> Future of a workload being planned
         self._stop = threading.Event()
         self._progress = threading.Event()
         self._lock = threading.Lock()
@@ -173,6 +174,10 @@ class EnergyServer:
         self.store = PlanStore(self.device)
         self.slots = [_Slot(stream=torch.cuda.Stream(device=self.device)) for _ in range(self.n_streams)]
         self._slot_free = threading.Semaphore(self.n_streams)
+        from concurrent.futures import ThreadPoolExecutor
+        # one planner thread: each workload's classes compile on the native
+        # planner's own threads
+        self._planner = ThreadPoolExecutor(1)
         self._threads = [threading.Thread(target=self._schedule_loop, daemon=True),
                          threading.Thread(target=self._complete_loop, daemon=True)]
         for t in self._threads:
@@ -191,6 +196,8 @@ class EnergyServer:
         self._pending.put(None)
         for t in self._threads:
             t.join(timeout=60)
+        if getattr(self, "_planner", None) is not None:
+            self._planner.shutdown(wait=True)
 
     def stats(self):
         with self._lock:
@@ -208,20 +215,34 @@ class EnergyServer:
     def engine_for(self, p):
         return self.engines.get(p, self.engine)
 
+    def _plan(self, G, p, k):
+        """Plan one (graph, depth) on the planner thread."""
+        try:
+            wl = self.engine_for(p).workload(G, p, workers=self.planning_workers)
+        except ValueError as exc:
+            # e.g. more slice variables than a work item can carry: the
+            # reference's planner stops the same way ("Estimated OOM")
+            raise MemoryError(f"Estimated OOM: {exc}") from exc
+        wl.shape_key = self.engine_for(p).shape_for(p).key()
+        self._workloads[k] = wl
+        self._progress.set()
+        return wl
+
     def _workload(self, req):
+        """The request's workload if planned; otherwise start planning it
+        (once per graph and depth) on the planner thread and return None."""
         p = len(req.gamma)
         k = (req.key, p)
         wl = self._workloads.get(k)
-        if wl is None:
-            try:
-                wl = self.engine_for(p).workload(req.G, p, workers=self.planning_workers)
-            except ValueError as exc:
-                # e.g. more slice variables than a work item can carry: the
-                # reference's planner stops the same way ("Estimated OOM")
-                raise MemoryError(f"Estimated OOM: {exc}") from exc
-            wl.shape_key = self.engine_for(p).shape_for(p).key()
-            self._workloads[k] = wl
-        return wl
+        if wl is not None:
+            return wl
+        fut = self._planning.get(k)
+        if fut is None:
+            fut = self._planner.submit(self._plan, req.G, p, k)
+            self._planning[k] = fut
+        if not fut.done():
+            return None
+        return fut.result()          # raises the planning error, if any
 
     def _schedule_loop(self):
         import torch
@@ -247,32 +268,40 @@ class EnergyServer:
                 break
             if not waiting:
                 continue
-            # host planning for new graphs (overlaps with batches in flight)
-            ready = []
+            # new graphs are planned on the planner thread while planned
+            # requests go on to the device; a request waits only for its own plan
+            ready, planning = [], []
             for req in waiting:
                 if req.workload is None:
                     try:
-                        req.workload = self._workload(req)
-                        req.nbytes = predicted_bytes(req.workload, self.elem)
+                        wl = self._workload(req)
                     except Exception as exc:      # planning error goes to the caller
                         self.rejected += 1
                         req.future.set_exception(exc)
                         continue
+                    if wl is None:
+                        planning.append(req)
+                        continue
+                    req.workload = wl
+                    req.nbytes = predicted_bytes(wl, self.elem)
                     if not self.admission.fits_alone(req.nbytes):
                         self.rejected += 1
                         req.future.set_exception(MemoryError(
                             f"Estimated OOM: request needs {req.nbytes} B, budget {self.admission.budget} B"))
                         continue
                 ready.append(req)
-            waiting = [r for r in ready]
-            if not waiting:
+            waiting = ready + planning
+            if not ready:
+                if planning:          # nothing to launch until a plan is done
+                    self._progress.clear()
+                    self._progress.wait(timeout=0.005)
                 continue
             if not self._slot_free.acquire(timeout=0.05):
                 continue
             self._progress.clear()
             batch, n_items = [], 0
-            shape_key = waiting[0].workload.shape_key
-            for req in list(waiting):
+            shape_key = ready[0].workload.shape_key
+            for req in list(ready):
                 if req.workload.shape_key != shape_key:
                     continue              # another launch shape: a later batch
                 items = sum(w.n_slices for w in req.workload.works)

@@ -673,3 +673,65 @@ def test_tamaki_and_tamaki_trim_slicing_with_stub_solver(monkeypatch, tmp_path):
     val = fs.simulate(comp.circuit, tw_bias=0)
     assert len(fs.parallel_vars) >= 1 and tts.treewidth <= opt.treewidth - 2
     assert abs(complex(val[0]) - sv[e]) < 1e-10
+
+
+@pytest.mark.parametrize("p,lanes,lim", [(1, 64, 2048), (2, 128, 64), (3, 256, 2048)])
+def test_native_planner_equals_python_compiler(p, lanes, lim):
+    """The native lightcone planner gives the Python planner's canonical
+    lightcone keys and, for every composer, order and slicer, the same program
+    words and plan figures (the Python compiler is its checker)."""
+    from qtensor_amd import native_planner as npl
+    from qtensor_amd.ordering import (RGreedyOrderingOptimizer, RGreedySlicesOptimizer,
+                                      RGreedyTreeTrimSplitter, TreeTrimSplitter)
+    G = nx.random_regular_graph(3, 50, seed=p)
+    edges = list(G.edges())[:16]
+    edge_class, classes = npl.lightcone_classes(G, edges, p)
+    assert [classes[c][0] for c in edge_class] == [canonical_lightcone(G, e, p)[0] for e in edges]
+    configs = [("greedy", None, None), ("rgreedy", None, None), ("greedy", "greedy", 6),
+               ("rgreedy", "tree_trim", 6), ("greedy", "tree_trim", 5)]
+    fields = ("n_steps", "n_ops", "n_gates", "treewidth", "max_rank", "lds_need", "glb_need", "n_sliced",
+              "bytes_step_model", "flops", "n_barriers", "n_gather", "n_reduce", "steps_words")
+    n_cmp = 0
+    for Comp in (QtreeQAOAComposer, ZZQtreeQAOAComposer, SimpZZQtreeComposer):
+        for ordering, slicer, mtw in configs:
+            for key, n, ce, i, j, rec in classes[:3]:
+                comp = Comp(canonical_graph(n, ce), gamma=[0.0] * p, beta=[0.0] * p)
+                comp.energy_expectation(i, j)
+                rg = ordering == "rgreedy"
+                if mtw is None:
+                    opt = RGreedyOrderingOptimizer() if rg else OrderingOptimizer()
+                elif slicer == "tree_trim":
+                    opt = (RGreedyTreeTrimSplitter if rg else TreeTrimSplitter)(max_tw=mtw, tw_bias=0)
+                else:
+                    opt = (RGreedySlicesOptimizer if rg else SlicesOptimizer)(max_tw=mtw, tw_bias=0)
+                res = []
+                for build in (lambda: PlanCompiler(lds_tensor_limit=lim, lds_capacity=4096, lanes=lanes)
+                              .compile_circuit(comp.circuit, opt),
+                              lambda: npl.compile_lightcone(rec, p, Comp, ordering=ordering, slicer=slicer,
+                                                            max_tw=mtw, lanes=lanes, lds_tensor_limit=lim,
+                                                            lds_capacity=4096)):
+                    try:
+                        res.append(build())
+                    except (ValueError, MemoryError) as exc:
+                        res.append(type(exc))
+                py, nat = res
+                if isinstance(py, type) or isinstance(nat, type):
+                    assert py == nat
+                    continue
+                assert np.array_equal(py.words, nat.words)
+                for f in fields:
+                    assert getattr(py, f) == getattr(nat, f), f
+                n_cmp += 1
+    assert n_cmp >= 30
+
+
+def test_native_planner_rejects_bad_input():
+    from qtensor_amd import native_planner as npl
+    with pytest.raises(ValueError):
+        npl.compile_lightcone(np.array([3, 1, 0, 5, 0, 1], np.int32), 1, QtreeQAOAComposer)   # j outside
+    with pytest.raises(ValueError):
+        npl.compile_lightcone(npl.lightcone_record(2, [(0, 1)], 0, 1), 1, QtreeQAOAComposer, lanes=96)
+    # a width no slicing can reach below: "Estimated OOM", as the reference's slicer
+    with pytest.raises(MemoryError):
+        npl.compile_lightcone(npl.lightcone_record(4, [(0, 1), (1, 2), (2, 3), (0, 3)], 0, 1), 2,
+                              QtreeQAOAComposer, slicer="greedy", max_tw=0, lds_capacity=None)
