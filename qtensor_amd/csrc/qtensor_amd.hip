@@ -304,41 +304,79 @@ constexpr int kFusedHdr = 8;
 constexpr uint32_t kOpFused = 1u << 14;
 constexpr uint32_t kOpReduce = 1u << 13;
 
+constexpr int kFusedHiRuns = 6;
+
 struct FusedArgs {
     const char* in[kFusedMaxIn];  // input base addresses in the arena
     uint32_t mask[kFusedMaxIn];   // input variables as bits of the union offset
-    const int32_t* lo_tab;        // (1 << lo_bits) x n_in offsets of the low output bits
-    const int32_t* x_tab;         // (1 << k) x n_in offsets of the summed variables
+    // Offset of input j contributed by the high output bits hi = o >> lo_bits:
+    // hi_n[j] <= kFusedHiRuns runs, run r copying bits [src, src + w) of hi to
+    // bits [dst, dst + w) of the offset, packed src | w << 5 | dst << 10 in
+    // hi_run[j][r]; more runs: pext(pdep(hi << lo_bits, omask), mask[j]).
+    uint32_t hi_run[kFusedMaxIn][kFusedHiRuns];
+    uint32_t hi_n[kFusedMaxIn];
     char* out;
     uint32_t omask;               // output variables as bits of the union offset
     uint32_t rank_out, k, lo_bits;
 };
 
-// out[o] = sum_x prod_j in_j[off_j(o) + x_tab[x][j]], off_j(o) = lo_tab[o_lo][j]
-// + pext(pdep(o_hi << lo_bits, omask), mask_j).  One thread per output element,
-// grid-stride; blocks of 256 threads share o_hi, so the high part is uniform.
+template <int NI>
+__device__ __forceinline__ uint32_t fused_hi_offset(const FusedArgs& a, int j, uint32_t hi) {
+    const uint32_t n = a.hi_n[j];
+    if (n > (uint32_t)kFusedHiRuns) return pext_loop(pdep_loop(hi << a.lo_bits, a.omask), a.mask[j]);
+    uint32_t off = 0;
+#pragma unroll
+    for (int r = 0; r < kFusedHiRuns; ++r) {
+        if ((uint32_t)r >= n) break;
+        const uint32_t f = a.hi_run[j][r];
+        off |= bfe(hi, f & 31u, (f >> 5) & 31u) << (f >> 10);
+    }
+    return off;
+}
+
+// out[o] = sum_x prod_j in_j[off_j(o) + x_tab[x][j]], o = hi << lo_bits | lo.
+// Thread t of a block owns lo = t for every hi the block visits: its low
+// offsets lo_tab[t][j] are read once, and the high part of every offset is
+// uniform over the block (scalar arithmetic on the hi runs).  Blocks have
+// 256 threads and lo_bits = min(8, rank_out); with rank_out < 8 only the
+// first 2^rank_out threads of the one block work.  The tables come as
+// separate restrict arguments so that their uniform reads are scalar loads;
+// two summed assignments are in flight per iteration when k >= 1.
 template <typename R, int NI>
-__global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a) {
+__global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a, const int32_t* __restrict__ lo_tab,
+                                                    const int32_t* __restrict__ x_tab) {
     using V = typename Cx<R>::V;
-    const uint64_t n_out = 1ull << a.rank_out;
-    const uint32_t lo_mask = (1u << a.lo_bits) - 1u;
+    const uint32_t tid = threadIdx.x;
+    if (tid >= (1u << a.lo_bits)) return;
+    uint32_t lo_off[NI];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) lo_off[j] = (uint32_t)lo_tab[tid * NI + j];
     const uint32_t n_x = 1u << a.k;
-    for (uint64_t o = (uint64_t)blockIdx.x * 256u + threadIdx.x; o < n_out; o += (uint64_t)gridDim.x * 256u) {
-        const uint32_t lo = (uint32_t)o & lo_mask;
-        const uint32_t uh = pdep_loop((uint32_t)(o & ~(uint64_t)lo_mask), a.omask);
-        uint32_t base[NI];
+    const uint32_t n_hi = 1u << (a.rank_out - a.lo_bits);
+    for (uint32_t hi = blockIdx.x; hi < n_hi; hi += gridDim.x) {
+        const V* in[NI];
 #pragma unroll
-        for (int j = 0; j < NI; ++j) base[j] = (uint32_t)a.lo_tab[lo * NI + j] + pext_loop(uh, a.mask[j]);
+        for (int j = 0; j < NI; ++j)
+            in[j] = reinterpret_cast<const V*>(a.in[j]) + lo_off[j] + fused_hi_offset<NI>(a, j, hi);
         V acc = {0, 0};
-        for (uint32_t x = 0; x < n_x; ++x) {
-            const int32_t* xt = a.x_tab + x * NI;
-            V p = reinterpret_cast<const V*>(a.in[0])[base[0] + (uint32_t)xt[0]];
+        if (n_x == 1u) {
+            V p = in[0][x_tab[0]];
 #pragma unroll
-            for (int j = 1; j < NI; ++j)
-                p = cmul(p, reinterpret_cast<const V*>(a.in[j])[base[j] + (uint32_t)xt[j]]);
-            acc = cadd(acc, p);
+            for (int j = 1; j < NI; ++j) p = cmul(p, in[j][x_tab[j]]);
+            acc = p;
+        } else {
+            for (uint32_t x = 0; x < n_x; x += 2) {
+                const int32_t* xt = x_tab + x * NI;
+                V p = in[0][xt[0]], q = in[0][xt[NI]];
+#pragma unroll
+                for (int j = 1; j < NI; ++j) {
+                    p = cmul(p, in[j][xt[j]]);
+                    q = cmul(q, in[j][xt[NI + j]]);
+                }
+                acc = cadd(acc, cadd(p, q));
+            }
         }
-        reinterpret_cast<V*>(a.out)[o] = acc;
+        reinterpret_cast<V*>(a.out)[((uint64_t)hi << a.lo_bits) | tid] = acc;
     }
 }
 
@@ -1335,22 +1373,65 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
             if ((uint32_t)w[tab + ((int64_t)n_in << lo_bits) + x * n_in + j] != pext_host(pdep_host(x, smask), mask))
                 return set_error(QB_ERR_INVALID, "fused op %d: summed-variable table", k_op);
     }
-    a.lo_tab = plan_dev + tab;
-    a.x_tab = plan_dev + tab + ((int64_t)n_in << lo_bits);
+    // high-bit runs: hi bit b is output bit lo_bits + b, i.e. union bit u_b (the
+    // (lo_bits + b)-th set bit of omask); it lands on bit popcount(mask & (2^u_b - 1))
+    // of input j's offset when the input carries u_b
+    {
+        uint32_t ubit[32];
+        uint32_t n_ob = 0;
+        for (uint32_t u = 0; u < 32; ++u)
+            if ((omask >> u) & 1u) ubit[n_ob++] = u;
+        for (uint32_t j = 0; j < n_in; ++j) {
+            const uint32_t mask = a.mask[j];
+            uint32_t n = 0, prev_b = 0, prev_d = 0;
+            for (uint32_t b = 0; b + lo_bits < n_ob; ++b) {
+                const uint32_t u = ubit[b + lo_bits];
+                if (!((mask >> u) & 1u)) continue;
+                const uint32_t d = (uint32_t)__builtin_popcount(mask & ((1u << u) - 1u));
+                if (n > 0 && n <= (uint32_t)kFusedHiRuns && b == prev_b + 1 && d == prev_d + 1) {
+                    a.hi_run[j][n - 1] += 1u << 5;        // extend the run by one bit
+                } else {
+                    if (n < (uint32_t)kFusedHiRuns) a.hi_run[j][n] = b | (1u << 5) | (d << 10);
+                    ++n;
+                }
+                prev_b = b;
+                prev_d = d;
+            }
+            a.hi_n[j] = n;
+            // the kernel's evaluation must equal the pext of the deposit for every
+            // high bit alone and for all of them (each bit lies in one run)
+            const uint32_t hi_bits = rank_out - lo_bits;
+            for (uint32_t b = 0; b <= hi_bits; ++b) {
+                const uint32_t hi = b < hi_bits ? (1u << b) : ((1u << hi_bits) - 1u);
+                uint32_t got = pext_host(pdep_host(hi << lo_bits, omask), mask);
+                if (n <= (uint32_t)kFusedHiRuns) {
+                    got = 0;
+                    for (uint32_t r = 0; r < n; ++r) {
+                        const uint32_t f = a.hi_run[j][r], src = f & 31u, wd = (f >> 5) & 31u;
+                        got |= ((hi >> src) & ((1u << wd) - 1u)) << (f >> 10);
+                    }
+                }
+                if (got != pext_host(pdep_host(hi << lo_bits, omask), mask))
+                    return set_error(QB_ERR_UNSUPPORTED, "fused op %d input %u: high-bit runs", k_op, j);
+            }
+        }
+    }
+    const int32_t* lo_tab = plan_dev + tab;                        // (1 << lo_bits) x n_in
+    const int32_t* x_tab = plan_dev + tab + ((int64_t)n_in << lo_bits);  // (1 << k) x n_in
     a.out = arena + (int64_t)(out_loc & ~kGlobal) * elem;
     a.omask = omask;
     a.rank_out = rank_out;
     a.k = k;
     a.lo_bits = lo_bits;
-    const uint64_t want = ((1ull << rank_out) + 255) / 256;
+    const uint64_t want = 1ull << (rank_out - lo_bits);     // one block per value of the high bits
     const uint64_t gcap = (uint64_t)g_num_cus * 16;
     const int grid = (int)(want < gcap ? want : gcap);
 #define QB_FUSED_CASE(N)                                                                      \
     case N:                                                                                   \
         if (dtype == QB_C128)                                                                 \
-            hipLaunchKernelGGL((fused_kernel<double, N>), dim3(grid), dim3(256), 0, st, a);   \
+            hipLaunchKernelGGL((fused_kernel<double, N>), dim3(grid), dim3(256), 0, st, a, lo_tab, x_tab); \
         else                                                                                  \
-            hipLaunchKernelGGL((fused_kernel<float, N>), dim3(grid), dim3(256), 0, st, a);    \
+            hipLaunchKernelGGL((fused_kernel<float, N>), dim3(grid), dim3(256), 0, st, a, lo_tab, x_tab);  \
         break;
     switch (n_in) {
         QB_FUSED_CASE(1)
