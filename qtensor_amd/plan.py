@@ -284,13 +284,17 @@ def _pext_table(mask, n):
     return tuple(_pext(o, mask) for o in range(n))
 
 
+def _step_work(n_in, union_rank):
+    """Arithmetic of a step including its loads: per union element, n_in loads,
+    n_in - 1 complex products and one complex sum."""
+    return (6 * max(n_in - 1, 0) + 2 + 4 * n_in) * (1 << union_rank)
+
+
 def _step_time(in_ranks, out_rank, union_rank, bw=4e12, rate=40e12, launch=4e-6):
     """Time model of one step or fused step on the device (memory, arithmetic
     including its loads, launch)."""
-    n = len(in_ranks)
     nbytes = 8 * (sum(1 << r for r in in_ranks) + (1 << out_rank))
-    work = (6 * max(n - 1, 0) + 2 + 4 * n) * (1 << union_rank)
-    return max(nbytes / bw, work / rate) + launch
+    return max(nbytes / bw, _step_work(len(in_ranks), union_rank) / rate) + launch
 
 
 def fuse_steps(steps):
@@ -345,6 +349,58 @@ def fuse_steps(steps):
     return out
 
 
+def split_small_inputs(steps):
+    """Within a step or fused step of three or more inputs, multiply all inputs
+    but the largest into one tensor first (summing the summed variables only
+    they carry), so that the wide contraction reads two inputs per summed
+    assignment instead of all of them, when the time model says the two passes
+    beat one.  The product of a bucket's tensors may be formed in any order
+    (``qtensor/ProcessingFrameworks.py:106-146`` multiplies them pairwise)."""
+    def ranks(nodes):
+        return [len(x.dev) for x in nodes]
+
+    out = []
+    for st in steps:
+        if isinstance(st, ReduceStep) or len(st.inputs) < 3:
+            out.append(st)
+            continue
+        if getattr(st, "fused", False):
+            union = set(st.union)
+            members = st.members
+        else:
+            union = set()
+            for nd in st.inputs:
+                union |= set(nd.dev)
+            members = None
+        out_set = set(st.out.dev)
+        big = max(st.inputs, key=lambda nd: len(nd.dev))
+        rest = [nd for nd in st.inputs if nd is not big]
+        u_rest = set()
+        for nd in rest:
+            u_rest |= set(nd.dev)
+        private = {v for v in u_rest if v not in out_set and v not in big.dev}
+        p_dev = tuple(sorted(u_rest - private))
+        main_union = union - private
+        if len(u_rest) > 30 or len(private) > FUSED_MAX_SUM or len(p_dev) > len(st.out.dev):
+            out.append(st)
+            continue
+        # the model's memory term hides the per-element work of a many-input
+        # contraction, so a product at least 64x smaller than the contraction
+        # is taken whenever it cuts the arithmetic
+        t_one = _step_time(ranks(st.inputs), len(out_set), len(union))
+        t_two = _step_time(ranks(rest), len(p_dev), len(u_rest)) + \
+            _step_time([len(big.dev), len(p_dev)], len(out_set), len(main_union))
+        small = len(u_rest) + 6 <= len(main_union) and \
+            _step_work(2, len(main_union)) < 0.9 * _step_work(len(st.inputs), len(union))
+        if not (t_two < 0.9 * t_one or small):
+            out.append(st)
+            continue
+        prod = Node((), p_dev, buf=Buf(1 << len(p_dev)))
+        out.append(FusedStep(rest, prod, u_rest, []))
+        out.append(FusedStep([big, prod], st.out, main_union, members if members is not None else []))
+    return out
+
+
 @dataclass
 class Plan:
     words: np.ndarray
@@ -365,6 +421,7 @@ class Plan:
     step_sizes: list = field(default_factory=list)
     steps_words: int = 0
     dag: list = field(default_factory=list)   # per step: (producer steps of inputs, n_out, n_in, sum)
+    bytes_ops: int = 0        # bytes (complex64) the ops read and write: inputs and output of each op
 
     def lds_elems(self, elem_bytes=8):
         """LDS elements one item needs (its data; op records stay in global memory)."""
@@ -617,7 +674,7 @@ class PlanCompiler:
         exec_steps = steps
         if wide:
             if self.fuse:
-                exec_steps = fuse_steps(steps)
+                exec_steps = split_small_inputs(fuse_steps(steps))
             ops, op_kind = [[i] for i in range(len(exec_steps))], [False] * len(exec_steps)
         else:
             if self.reduce:
@@ -695,6 +752,7 @@ class PlanCompiler:
             else:
                 op_words.append(self._encode_step(exec_steps[members[0]], sync_glb))
 
+        bytes_ops = sum(8 * (sum(nd.n for nd in st.inputs) + st.out.n) for st in exec_steps)
         n_ops = len(op_words)
         gates_off = PLAN_HDR
         steps_off = gates_off + GATE_WORDS * len(gates)
@@ -712,7 +770,7 @@ class PlanCompiler:
                     bytes_step_model=total_bytes, flops=total_flops, n_barriers=barriers,
                     n_gather=n_gather, n_fused=n_fused, n_reduce=n_reduce, step_sizes=step_sizes,
                     steps_words=steps_words,
-                    dag=dag)
+                    dag=dag, bytes_ops=bytes_ops)
 
     def _gather_ok(self, st, lanes, lim):
         return self.gather and st.out.n <= lim and st.gatherable(lanes, lim)
