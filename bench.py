@@ -177,25 +177,14 @@ def plain_form(G, gamma, beta, n_edges):
     return vals, time.perf_counter() - t0
 
 
-def forced_edge(G, gamma, beta, reference=None, target_tw=30, steps=5):
-    """C2's extra edge: the first edge under a deliberately wide order (seeded
-    randomized greedy, temperature 8, first sample of width 30), sliced over 2-4
-    variables with the order kept for the rest, on the wide path (grid-wide
-    fused step kernels, slices on 4 streams).  Slice variables: for 2, 3 and 4
-    variables both the reference's GreedyParvars rule (max degree first,
-    qtensor/optimisation/Greedy.py:78-92) and the order-aware neighbourhood rule
-    (wide.slice_vars_for_order) are evaluated, and the choice with the smallest
-    slices x sum of 2^width runs (GreedyParvars on ties).  ``reference``: the
-    oracle's value of that edge term (None: the greedy-ordered wide plan of the
-    same edge)."""
-    import numpy as np
+def forced_edge_plan(G, gamma, beta, target_tw=30):
+    """The forced edge's wide plan and how its slice variables were chosen
+    (see ``forced_edge``)."""
     from qtensor_amd.composer import QtreeQAOAComposer
     from qtensor_amd.ordering import elimination_widths
     from qtensor_amd.tensornet import QtreeTensorNet
-    from qtensor_amd.wide import (WideEngine, compile_wide, forced_width_order, greedy_parvars_for_order,
-                                  slice_vars_for_order)
+    from qtensor_amd.wide import compile_wide, forced_width_order, greedy_parvars_for_order, slice_vars_for_order
     e = list(G.edges())[0]
-    t0 = time.perf_counter()
     comp = QtreeQAOAComposer(G, gamma=gamma, beta=beta)
     comp.energy_expectation_lightcone(e)
     graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
@@ -210,7 +199,99 @@ def forced_edge(G, gamma, beta, reference=None, target_tw=30, steps=5):
             cost = (1 << len(par)) * sum(2.0 ** w for w in ws)
             choices.append({"rule": rule, "slice_vars": len(par), "width": max(ws), "cost": cost, "par": par})
     best = min(choices, key=lambda c: (c["cost"], c["rule"] != "GreedyParvars"))
-    wp = compile_wide(comp.circuit, order=order, par_ids=best["par"])
+    return e, comp, compile_wide(comp.circuit, order=order, par_ids=best["par"]), best, choices
+
+
+def _op_contractions(words):
+    """Per op of a plan: (input ranks, summed-variable count, shared summed
+    variables of every pair of inputs) from the op records."""
+    from qtensor_amd import plan as P
+    w = [int(x) & 0xFFFFFFFF for x in words]
+    pos, out = w[3], []
+    for _ in range(w[1]):
+        w0 = w[pos]
+        if w0 & P.OP_FUSED:
+            n_in, meta, smask = w0 & 15, w[pos + 1], w[pos + 5]
+            masks = [w[pos + P.FUSED_HDR + 4 * j + 1] for j in range(n_in)]
+            ranks = [bin(m).count("1") for m in masks]
+            pairs = [(ranks[a], ranks[b], bin(masks[a] & masks[b] & smask).count("1"))
+                     for a in range(n_in) for b in range(a + 1, n_in)]
+            out.append((ranks, (meta >> 16) & 255, pairs))
+        elif not w0 & (P.OP_GATHER | P.OP_REDUCE):
+            n_in, rank_out, s = w0 & 15, (w0 >> 4) & 31, (w0 >> 9) & 1
+            ranks = [bin(w[pos + P.STEP_HDR + P.IN_WORDS * j + 2]).count("1") + (1 if w[pos + P.STEP_HDR +
+                     P.IN_WORDS * j + 1] else 0) for j in range(n_in)]
+            out.append((ranks, s, [(ranks[a], ranks[b], s) for a in range(n_in) for b in range(a + 1, n_in)]))
+        pos += w0 >> 16
+    return out
+
+
+def mfma_census():
+    """Which contractions of the benchmark plans could use the matrix cores:
+    an op qualifies when two of its inputs hold at least 2^10 elements each and
+    share at least 5 summed variables (K >= 32 summed entries per output).
+    Plans: every edge of the Erdos-Renyi N=64 p=3 workload (wide plans as the
+    precision study builds them) and the forced treewidth-30 edge of the
+    3-regular N=256 p=3 workload.  The batched executor's plans sum one
+    variable per step (K = 2) by construction.  Host only."""
+    import networkx as nx
+    import numpy as np
+    from qtensor_amd.composer import DefaultQAOAComposer
+    from qtensor_amd.ordering import greedy_peo
+    from qtensor_amd.tensornet import QtreeTensorNet
+    from qtensor_amd.wide import compile_wide, slice_vars_for_order
+
+    def tally(plans):
+        n_ops, k_hist, qual, best = 0, {}, 0, None
+        for wp in plans:
+            for ranks, k, pairs in _op_contractions(wp.plan.words):
+                n_ops += 1
+                k_hist[k] = k_hist.get(k, 0) + 1
+                for ra, rb, shared in pairs:
+                    if ra >= 10 and rb >= 10:
+                        if best is None or shared > best[2]:
+                            best = (ra, rb, shared)
+                        if shared >= 5:
+                            qual += 1
+        return {"ops": n_ops, "summed_variables_histogram": {str(k): v for k, v in sorted(k_hist.items())},
+                "qualifying_pairs": qual,
+                "largest_shared_sum_of_large_pairs": None if best is None else
+                {"input_ranks": best[:2], "shared_summed_variables": best[2], "K": 1 << best[2]}}
+
+    t0 = time.perf_counter()
+    G = nx.erdos_renyi_graph(64, 4 / 63, seed=4)
+    c4 = []
+    for e in G.edges():
+        comp = DefaultQAOAComposer(G, gamma=[0.0] * 3, beta=[0.0] * 3)
+        comp.energy_expectation_lightcone(e)
+        graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+        order, degs = greedy_peo(graph)
+        s = max(0, max(degs) - 26)
+        c4.append(compile_wide(comp.circuit, order=order,
+                               par_ids=slice_vars_for_order(graph, order, s) if s else []))
+    G2, _, gamma, beta, _ = workload("c2")
+    _, _, fwp, _, _ = forced_edge_plan(G2, gamma, beta)
+    res = {"er64_p3_edges": len(c4), "er64_p3": tally(c4), "forced_tw30_edge": tally([fwp]),
+           "seconds": round(time.perf_counter() - t0, 1)}
+    res["qualifying_pairs"] = res["er64_p3"]["qualifying_pairs"] + res["forced_tw30_edge"]["qualifying_pairs"]
+    return res
+
+
+def forced_edge(G, gamma, beta, reference=None, target_tw=30, steps=5):
+    """C2's extra edge: the first edge under a deliberately wide order (seeded
+    randomized greedy, temperature 8, first sample of width 30), sliced over 2-4
+    variables with the order kept for the rest, on the wide path (grid-wide
+    fused step kernels, slices on 4 streams).  Slice variables: for 2, 3 and 4
+    variables both the reference's GreedyParvars rule (max degree first,
+    qtensor/optimisation/Greedy.py:78-92) and the order-aware neighbourhood rule
+    (wide.slice_vars_for_order) are evaluated, and the choice with the smallest
+    slices x sum of 2^width runs (GreedyParvars on ties).  ``reference``: the
+    oracle's value of that edge term (None: the greedy-ordered wide plan of the
+    same edge)."""
+    import numpy as np
+    from qtensor_amd.wide import WideEngine, compile_wide
+    t0 = time.perf_counter()
+    e, comp, wp, best, choices = forced_edge_plan(G, gamma, beta, target_tw)
     plan_s = time.perf_counter() - t0
     eng = WideEngine()
     prm = list(gamma) + list(beta)
@@ -462,6 +543,9 @@ def main():
     ap.add_argument("--serve-ordering4", default="greedy", choices=("greedy", "rgreedy"),
                     help="with --serve: elimination order of the depth-4 requests (--ordering sets "
                          "depths 1-3)")
+    ap.add_argument("--mfma-census", action="store_true",
+                    help="host only: count the contractions of the Erdos-Renyi and forced-edge plans that "
+                         "could use the matrix cores, and print their JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -492,6 +576,13 @@ def main():
                           "serving": res}), flush=True)
         return
 
+    if args.mfma_census:
+        res = mfma_census()
+        print(json.dumps({"metric": "contractions qualifying for the matrix cores", "value": res["qualifying_pairs"],
+                          "unit": "input pairs", "n_gpus": 0, "higher_is_better": False, "data": "synthetic",
+                          "config": {"workload": "er64_p3_all_edges+qaoa_p3_3reg_n256_forced_tw30_edge"},
+                          "census": res}), flush=True)
+        return
     G, p, gamma, beta, desc = workload(args.workload)
     oracle_vals = oracle_fixture(args.workload, G)
     if args.forced_edge:

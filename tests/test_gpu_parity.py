@@ -55,9 +55,11 @@ def test_c2_complex128_all_terms(gpu):
 def test_c2_forced_edge_greedy_parvars(gpu):
     """C2's extra edge: edge (0, 83) under a deliberately wide order (seeded
     randomized greedy at temperature 8, first sample of width 30), sliced over
-    3 variables chosen by GreedyParvars, on the wide path."""
+    3 variables chosen by GreedyParvars, and over 2 chosen by the order-aware
+    rule (the bench's pick), on the wide path."""
     from qtensor_amd.tensornet import QtreeTensorNet
-    from qtensor_amd.wide import WideEngine, compile_wide, forced_width_order, greedy_parvars_for_order
+    from qtensor_amd.wide import (WideEngine, compile_wide, forced_width_order, greedy_parvars_for_order,
+                                  slice_vars_for_order)
     m = META["c2"]
     G = pf.graph(m)
     e = tuple(m["edges"][0])
@@ -72,6 +74,10 @@ def test_c2_forced_edge_greedy_parvars(gpu):
     for dtype, tol in (("complex64", pf.tol(ARR["c2"][0])), ("complex128", 1e-11)):
         val = WideEngine(dtype=dtype).run(wp, list(m["gamma"]) + list(m["beta"]))
         assert abs(val - ARR["c2"][0]) <= tol, (dtype, val, ARR["c2"][0])
+    wn = compile_wide(comp.circuit, order=order, par_ids=slice_vars_for_order(graph, order, 2))
+    assert wn.n_slices == 4 and wn.treewidth == 28
+    val = WideEngine(dtype="complex64").run(wn, list(m["gamma"]) + list(m["beta"]))
+    assert abs(val - ARR["c2"][0]) <= pf.tol(ARR["c2"][0]), (val, ARR["c2"][0])
 
 
 def test_c3_all_terms_vs_oracle(gpu):
