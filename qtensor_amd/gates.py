@@ -62,9 +62,19 @@ class Gate:
         raise NotImplementedError
 
     def tensor(self):
-        t = self.gen_tensor()
-        if self._dagger and not self.parametric:
-            t = _adjoint(t, self)
+        """The gate's tensor.  Equal gates (same class, adjoint flag and
+        parameters) share one read-only array."""
+        key = (self.__class__, self._dagger, tuple(sorted(self._parameters.items())))
+        t = _TENSORS.get(key)
+        if t is None:
+            t = self.gen_tensor()
+            if self._dagger and not self.parametric:
+                t = _adjoint(t, self)
+            t = np.array(t)
+            t.setflags(write=False)
+            if len(_TENSORS) >= 1 << 14:
+                _TENSORS.clear()
+            _TENSORS[key] = t
         return t
 
     def dagger_me(self):
@@ -87,6 +97,9 @@ class Gate:
         if self.parametric:
             args += f",a={self._parameters['alpha']:.4g}"
         return f"{self.label}({args})"
+
+
+_TENSORS = {}
 
 
 def _adjoint(t, gate):

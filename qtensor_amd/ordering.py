@@ -117,7 +117,57 @@ class Optimizer:
         raise NotImplementedError
 
 
+def _line_csr(buckets, ignore_ids):
+    """The CSR adjacency of the line graph ``buckets2graph`` builds (nodes in
+    order of first appearance, a clique per tensor, ignored variables left
+    out) without building the graph: ``(vars, ptr, idx)`` with ``vars[i]``
+    the first ``Var`` seen for node i."""
+    index, first, adj = {}, [], []
+    for bucket in buckets:
+        for t in bucket:
+            ids = []
+            for v in t.indices:
+                k = v.identity
+                if k in ignore_ids:
+                    continue
+                i = index.get(k)
+                if i is None:
+                    i = index[k] = len(first)
+                    first.append(v)
+                    adj.append(set())
+                ids.append(i)
+            if len(ids) > 1:
+                for a in ids:
+                    adj[a].update(ids)
+    ptr = np.zeros(len(first) + 1, dtype=np.int32)
+    flat = []
+    for i, nb in enumerate(adj):
+        nb.discard(i)
+        flat += nb
+        ptr[i + 1] = ptr[i] + len(nb)
+    return first, ptr, np.asarray(flat, dtype=np.int32)
+
+
 class OrderingOptimizer(Optimizer):
+    _graph = None
+    _graph_source = None
+
+    @property
+    def graph(self):
+        """The line graph the order was computed on (built on first use when
+        the order came from the bucket lists directly)."""
+        if self._graph is None and self._graph_source is not None:
+            from .circuit import buckets2graph
+            buckets, ignored = self._graph_source
+            self._graph = buckets2graph(buckets, ignore_variables=ignored)
+            self._graph_source = None
+        return self._graph
+
+    @graph.setter
+    def graph(self, g):
+        self._graph = g
+        self._graph_source = None
+
     def _get_ordering_ints(self, graph):
         return greedy_peo(graph)
 
@@ -125,14 +175,34 @@ class OrderingOptimizer(Optimizer):
         peo, path = self._get_ordering_ints(graph)
         return _vars(graph, peo), path
 
+    def _greedy_from_buckets(self, tensor_net, ignored):
+        """Greedy order straight from the bucket lists (same nodes, order and
+        adjacency as the line graph, so the same order as ``greedy_peo``)."""
+        first, ptr, idx = _line_csr(tensor_net.buckets, {int(v) for v in ignored})
+        n = len(first)
+        order = np.zeros(max(n, 1), dtype=np.int32)
+        widths = np.zeros(max(n, 1), dtype=np.int32)
+        lib = _lib.load()
+        _lib.check(lib.qb_greedy_order(n, _i32p(ptr), _i32p(idx if idx.size else np.zeros(1, np.int32)),
+                                       _i32p(order), _i32p(widths)))
+        peo = [Var(first[k].identity, size=first[k].size, name=first[k].name) for k in order[:n]]
+        # the bucket lists of this network are not modified later (reordering
+        # and slicing build new ones), so the graph can be built from them on demand
+        self._graph, self._graph_source = None, (tensor_net.buckets, list(ignored))
+        return peo, [int(w) for w in widths[:n]]
+
     def optimize(self, tensor_net):
-        graph = tensor_net.get_line_graph()
         free_vars = list(tensor_net.free_vars)
         ignored = list(tensor_net.ket_vars) + list(tensor_net.bra_vars)
-        if free_vars:
-            graph = _with_clique(graph, free_vars)
-        self.graph = graph
-        peo, path = self._get_ordering(graph)
+        if not free_vars and type(self)._get_ordering_ints is OrderingOptimizer._get_ordering_ints \
+                and type(self)._get_ordering is OrderingOptimizer._get_ordering:
+            peo, path = self._greedy_from_buckets(tensor_net, ignored)
+        else:
+            graph = tensor_net.get_line_graph()
+            if free_vars:
+                graph = _with_clique(graph, free_vars)
+            self.graph = graph
+            peo, path = self._get_ordering(graph)
         self.treewidth = max(path) if path else 0
         if free_vars:
             free_ids = set(int(v) for v in free_vars)

@@ -137,3 +137,53 @@ def test_step_large_rank_register_tiling(gpu, dtype, tol, seed):
         ref = np.asarray(npf.process_bucket(bucket, no_sum=no_sum).data)
         assert got.shape == ref.shape
         assert np.max(np.abs(got - ref)) <= tol * max(1.0, np.max(np.abs(ref)))
+
+
+def test_feynman_thread_pool_shares_one_backend(gpu):
+    """FeynmanSimulator(pool_type='thread') (``qtensor/FeynmanSimulator.py:36-41``):
+    four threads run the 2^s slices through one shared HipBackend at once; the
+    sum equals the serial numpy oracle's, slice by slice and in total."""
+    from qtensor_amd.ordering import SlicesOptimizer
+    from qtensor_amd.simulator import FeynmanSimulator
+    G = nx.random_regular_graph(3, 16, seed=7)
+    comp = QtreeQAOAComposer(G, gamma=[0.3, 0.7], beta=[0.2, 0.4])
+    comp.energy_expectation_lightcone(list(G.edges())[0])
+    be = HipBackend()
+    sim = FeynmanSimulator(bucket_backend=be, optimizer=SlicesOptimizer(tw_bias=0, max_tw=6),
+                           pool_type="thread", n_processes=4)
+    got = sim.simulate(comp.circuit)
+    assert len(sim.parallel_vars) >= 3
+    ref_sim = FeynmanSimulator(bucket_backend=npf.NumpyBackend(), optimizer=SlicesOptimizer(tw_bias=0, max_tw=6))
+    ref = ref_sim.simulate(comp.circuit)
+    assert [int(v) for v in ref_sim.parallel_vars] == [int(v) for v in sim.parallel_vars]
+    whole = QtreeSimulator(bucket_backend=npf.NumpyBackend()).simulate(comp.circuit)
+    assert abs(got[0] - ref[0]) <= 1e-5 and abs(got[0] - whole[0]) <= 1e-5
+    parts = [sim._parallel_unit(k)[0] for k in range(2 ** len(sim.parallel_vars))]
+    ref_parts = [ref_sim._parallel_unit(k)[0] for k in range(2 ** len(sim.parallel_vars))]
+    assert np.allclose(parts, ref_parts, atol=1e-5)
+
+
+def test_sliced_buckets_are_device_spans(gpu):
+    """get_sliced_buckets puts every tensor in one device buffer; a span reads
+    back as the transposed and sliced gate, and repeated gates reuse the host
+    slicing."""
+    from qtensor_amd.backend import DeviceSpan
+    from qtensor_amd.circuit import slice_tensor_data
+    from qtensor_amd.tensornet import QtreeTensorNet
+    G = nx.random_regular_graph(3, 10, seed=2)
+    comp = QtreeQAOAComposer(G, gamma=[0.5], beta=[0.25])
+    comp.energy_expectation_lightcone(list(G.edges())[0])
+    tn = QtreeTensorNet.from_qtree_gates(comp.circuit)
+    be = HipBackend(dtype="complex128")
+    sd = {v: slice(0, 1) for v in tn.ket_vars + tn.bra_vars}
+    out = be.get_sliced_buckets(tn.buckets, tn.data_dict, sd)
+    n = 0
+    for bucket, ob in zip(tn.buckets, out):
+        for t, s in zip(bucket, ob):
+            ix, ref = slice_tensor_data(t, tn.data_dict[t.data_key], sd)
+            assert [int(v) for v in s.indices] == [int(v) for v in ix]
+            assert [v.size for v in s.indices] == [v.size for v in ix]
+            assert type(s.data) is DeviceSpan
+            assert np.array_equal(np.asarray(s.data), ref)
+            n += 1
+    assert n > 0 and len(be._slice_cache) < n

@@ -735,3 +735,28 @@ def test_native_planner_rejects_bad_input():
     with pytest.raises(MemoryError):
         npl.compile_lightcone(npl.lightcone_record(4, [(0, 1), (1, 2), (2, 3), (0, 3)], 0, 1), 2,
                               QtreeQAOAComposer, slicer="greedy", max_tw=0, lds_capacity=None)
+
+
+def test_greedy_from_buckets_equals_line_graph_greedy():
+    """The optimizer's bucket-list path (no networkx graph) gives the order,
+    treewidth and (lazily built) line graph of the graph path, on lightcones of
+    three composers."""
+    import networkx as nx
+    from qtensor_amd.composer import DefaultQAOAComposer, QtreeQAOAComposer, ZZQtreeQAOAComposer
+    from qtensor_amd.ordering import OrderingOptimizer, greedy_peo
+    from qtensor_amd.tensornet import QtreeTensorNet
+    for seed in range(6):
+        G = nx.random_regular_graph(3, 24 + 2 * seed, seed=seed)
+        for Comp in (QtreeQAOAComposer, ZZQtreeQAOAComposer, DefaultQAOAComposer):
+            for e in list(G.edges())[:2]:
+                comp = Comp(G, gamma=[0.1, 0.2], beta=[0.3, 0.4])
+                comp.energy_expectation_lightcone(e)
+                tn = QtreeTensorNet.from_qtree_gates(comp.circuit)
+                opt = OrderingOptimizer()
+                peo, _ = opt.optimize(tn)
+                g = tn.get_line_graph()
+                ref, path = greedy_peo(g)
+                assert [int(v) for v in peo][len(tn.ket_vars) + len(tn.bra_vars):] == ref
+                assert opt.treewidth == max(path)
+                assert list(opt.graph.nodes) == list(g.nodes)
+                assert {frozenset(x) for x in opt.graph.edges} == {frozenset(x) for x in g.edges}
