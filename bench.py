@@ -169,7 +169,8 @@ def plain_form(G, gamma, beta, n_edges):
     from qtensor_amd.backend import HipBackend
     sim = QAOAQtreeSimulator(QtreeQAOAComposer, bucket_backend=HipBackend())
     edges = list(G.edges())[:n_edges]
-    sim._get_edge_energy(G, gamma, beta, edges[0])       # warm
+    for e in edges[:4]:                                   # warm: gate and slicing memos, result slabs
+        sim._get_edge_energy(G, gamma, beta, e)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     vals = [complex(sim._get_edge_energy(G, gamma, beta, e)[0]) for e in edges]

@@ -380,6 +380,70 @@ __global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a, const int
     }
 }
 
+// Two-input fused op with register blocking: out[o] = sum_x A[a(o, x)] B[b(o, x)].
+// Up to two high output bits that A carries and B does not (regA) and up to two
+// that B carries and A does not (regB) are handled in registers: per summed
+// assignment a thread loads 2^RA values of A and 2^RB of B and forms their
+// 2^(RA+RB) products, instead of two loads per output.  Blocks visit the
+// remaining high bits hk; hi = deposit(hk) over the non-register high bits
+// (at most kFusedDepRuns runs, src | w << 5 | dst << 10 each).
+constexpr int kFusedDepRuns = 5;
+
+struct Fused2Args {
+    uint32_t roff_a[4], roff_b[4];   // offset in A (B) of register pattern p of regA (regB)
+    uint32_t rdep_a[4], rdep_b[4];   // high-bit pattern of register pattern p
+    uint32_t dep_run[kFusedDepRuns];
+    uint32_t n_dep, hk_bits;
+};
+
+template <typename R, int RA, int RB>
+__global__ __launch_bounds__(256) void fused2_kernel(const FusedArgs a, const int32_t* __restrict__ lo_tab,
+                                                     const int32_t* __restrict__ x_tab, const Fused2Args e) {
+    using V = typename Cx<R>::V;
+    constexpr int NA = 1 << RA, NB = 1 << RB;
+    const uint32_t tid = threadIdx.x;
+    if (tid >= (1u << a.lo_bits)) return;
+    const uint32_t lo_a = (uint32_t)lo_tab[tid * 2], lo_b = (uint32_t)lo_tab[tid * 2 + 1];
+    const V* __restrict__ in_a = reinterpret_cast<const V*>(a.in[0]);
+    const V* __restrict__ in_b = reinterpret_cast<const V*>(a.in[1]);
+    const uint32_t n_x = 1u << a.k;
+    const uint32_t n_hk = 1u << e.hk_bits;
+    for (uint32_t hk = blockIdx.x; hk < n_hk; hk += gridDim.x) {
+        uint32_t hi = 0;
+#pragma unroll
+        for (int r = 0; r < kFusedDepRuns; ++r) {
+            if ((uint32_t)r >= e.n_dep) break;
+            const uint32_t f = e.dep_run[r];
+            hi |= bfe(hk, f & 31u, (f >> 5) & 31u) << (f >> 10);
+        }
+        const V* pa = in_a + lo_a + fused_hi_offset<2>(a, 0, hi);
+        const V* pb = in_b + lo_b + fused_hi_offset<2>(a, 1, hi);
+        V acc[NA][NB];
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) acc[i][j] = V{0, 0};
+        for (uint32_t x = 0; x < n_x; ++x) {
+            const uint32_t xa = (uint32_t)x_tab[2 * x], xb = (uint32_t)x_tab[2 * x + 1];
+            V va[NA], vb[NB];
+#pragma unroll
+            for (int i = 0; i < NA; ++i) va[i] = pa[xa + e.roff_a[i]];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) vb[j] = pb[xb + e.roff_b[j]];
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) acc[i][j] = cadd(acc[i][j], cmul(va[i], vb[j]));
+        }
+        V* out = reinterpret_cast<V*>(a.out);
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                out[((uint64_t)(hi | e.rdep_a[i] | e.rdep_b[j]) << a.lo_bits) | tid] = acc[i][j];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched executor
 // ---------------------------------------------------------------------------
@@ -1423,8 +1487,80 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
     a.rank_out = rank_out;
     a.k = k;
     a.lo_bits = lo_bits;
-    const uint64_t want = 1ull << (rank_out - lo_bits);     // one block per value of the high bits
     const uint64_t gcap = (uint64_t)g_num_cus * 16;
+    if (n_in == 2 && rank_out >= lo_bits + 1) {
+        // register bits: high output bits carried by one input only
+        const uint32_t hi_bits = rank_out - lo_bits;
+        uint32_t ubit[32], n_ob = 0;
+        for (uint32_t u = 0; u < 32; ++u)
+            if ((omask >> u) & 1u) ubit[n_ob++] = u;
+        uint32_t reg_a[2], reg_b[2], ra = 0, rb = 0;
+        for (uint32_t b = hi_bits; b-- > 0;) {          // the highest bits first
+            const uint32_t u = ubit[b + lo_bits];
+            const bool in_a = (a.mask[0] >> u) & 1u, in_b = (a.mask[1] >> u) & 1u;
+            if (in_a && !in_b && ra < 2) reg_a[ra++] = b;
+            else if (in_b && !in_a && rb < 2) reg_b[rb++] = b;
+        }
+        if (ra + rb > 0) {
+            Fused2Args e;
+            memset(&e, 0, sizeof(e));
+            uint32_t regmask = 0;
+            for (uint32_t i = 0; i < ra; ++i) regmask |= 1u << reg_a[i];
+            for (uint32_t i = 0; i < rb; ++i) regmask |= 1u << reg_b[i];
+            const uint32_t keep = ((1u << hi_bits) - 1u) & ~regmask;
+            for (uint32_t p = 0; p < (1u << ra); ++p) {
+                uint32_t hp = 0;
+                for (uint32_t i = 0; i < ra; ++i) if ((p >> i) & 1u) hp |= 1u << reg_a[i];
+                e.rdep_a[p] = hp;
+                e.roff_a[p] = pext_host(pdep_host(hp << lo_bits, omask), a.mask[0]);
+            }
+            for (uint32_t p = 0; p < (1u << rb); ++p) {
+                uint32_t hp = 0;
+                for (uint32_t i = 0; i < rb; ++i) if ((p >> i) & 1u) hp |= 1u << reg_b[i];
+                e.rdep_b[p] = hp;
+                e.roff_b[p] = pext_host(pdep_host(hp << lo_bits, omask), a.mask[1]);
+            }
+            // deposit runs of hk into keep; checked against pdep for every bit and all
+            uint32_t n = 0, src = 0;
+            for (uint32_t b = 0; b < hi_bits;) {
+                if (!((keep >> b) & 1u)) { ++b; continue; }
+                uint32_t b0 = b;
+                while (b < hi_bits && ((keep >> b) & 1u)) ++b;
+                if (n >= (uint32_t)kFusedDepRuns) return set_error(QB_ERR_UNSUPPORTED, "fused op %d: deposit runs", k_op);
+                e.dep_run[n++] = src | ((b - b0) << 5) | (b0 << 10);
+                src += b - b0;
+            }
+            e.n_dep = n;
+            e.hk_bits = src;
+            for (uint32_t b = 0; b <= e.hk_bits; ++b) {
+                const uint32_t hk = b < e.hk_bits ? (1u << b) : ((1u << e.hk_bits) - 1u);
+                uint32_t hi = 0;
+                for (uint32_t r = 0; r < n; ++r) {
+                    const uint32_t f = e.dep_run[r], wd = (f >> 5) & 31u;
+                    hi |= ((hk >> (f & 31u)) & ((1u << wd) - 1u)) << (f >> 10);
+                }
+                if (hi != pdep_host(hk, keep))
+                    return set_error(QB_ERR_UNSUPPORTED, "fused op %d: deposit runs", k_op);
+            }
+            const uint64_t want2 = 1ull << e.hk_bits;
+            const int grid2 = (int)(want2 < gcap ? want2 : gcap);
+#define QB_FUSED2_CASE(RA_, RB_)                                                                         \
+            if (ra == RA_ && rb == RB_) {                                                                \
+                if (dtype == QB_C128)                                                                    \
+                    hipLaunchKernelGGL((fused2_kernel<double, RA_, RB_>), dim3(grid2), dim3(256), 0, st, a, lo_tab, \
+                                       x_tab, e);                                                        \
+                else                                                                                     \
+                    hipLaunchKernelGGL((fused2_kernel<float, RA_, RB_>), dim3(grid2), dim3(256), 0, st, a, lo_tab,  \
+                                       x_tab, e);                                                        \
+            }
+            QB_FUSED2_CASE(0, 1) QB_FUSED2_CASE(0, 2) QB_FUSED2_CASE(1, 0) QB_FUSED2_CASE(1, 1)
+            QB_FUSED2_CASE(1, 2) QB_FUSED2_CASE(2, 0) QB_FUSED2_CASE(2, 1) QB_FUSED2_CASE(2, 2)
+#undef QB_FUSED2_CASE
+            QB_HIP(hipGetLastError());
+            return QB_OK;
+        }
+    }
+    const uint64_t want = 1ull << (rank_out - lo_bits);     // one block per value of the high bits
     const int grid = (int)(want < gcap ? want : gcap);
 #define QB_FUSED_CASE(N)                                                                      \
     case N:                                                                                   \
