@@ -186,9 +186,12 @@ int qb_check_program(const int32_t* program, int64_t program_words, const int32_
  * live in `arena` (global memory), for networks too large for the executor's
  * LDS tiers.  Builds the gate tensors from `params` for the slice assignment
  * `slice_bits`, launches one grid-wide step kernel per step op in order on
- * `stream`, and writes the scalar result as complex128 to out[0..1].  The host
- * copy of the plan is parsed and every tensor range is checked against
- * `arena_elems` before any launch.
+ * `stream`, and writes the result as complex128: the scalar to out[0..1], or,
+ * for a plan with b open (batch) variables (plan word 11; the amplitudes of
+ * `QtreeSimulator.simulate_batch(batch_vars=b)`, qtensor/Simulate.py:66-94),
+ * element e of the 2^b result to out[2 e out_stride .. + 1] (first open
+ * variable = most significant bit of e).  The host copy of the plan is parsed
+ * and every tensor range is checked against `arena_elems` before any launch.
  */
 typedef struct qb_wide_desc {
     const int32_t* plan_host;    /* host: plan words */
@@ -199,7 +202,8 @@ typedef struct qb_wide_desc {
     int64_t arena_elems;
     uint32_t slice_bits;
     int32_t dtype;
-    double* out;                 /* device: 2 doubles */
+    double* out;                 /* device: 2 doubles per result element */
+    int64_t out_stride;          /* complex128 values between result elements (0 or 1: adjacent) */
 } qb_wide_desc;
 
 int qb_wide_execute(const qb_wide_desc* desc, void* stream);

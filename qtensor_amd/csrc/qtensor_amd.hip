@@ -1284,12 +1284,17 @@ __global__ void build_gates_global(const int32_t* __restrict__ gates, int32_t n_
                       reinterpret_cast<V*>(arena), (const V*)nullptr);
 }
 
+// Result of a wide plan: n elements from arena[loc], element e as complex128
+// to out[2 e stride].
 template <typename R>
-__global__ void store_result(const void* __restrict__ arena, uint32_t loc, double* __restrict__ out) {
+__global__ void store_result(const void* __restrict__ arena, uint32_t loc, uint32_t n, int64_t stride,
+                             double* __restrict__ out) {
     using V = typename Cx<R>::V;
-    const V r = reinterpret_cast<const V*>(arena)[loc];
-    out[0] = (double)r.x;
-    out[1] = (double)r.y;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const V r = reinterpret_cast<const V*>(arena)[loc + e];
+        out[2 * (int64_t)e * stride] = (double)r.x;
+        out[2 * (int64_t)e * stride + 1] = (double)r.y;
+    }
 }
 
 // Host: pext runs word of a mask (n_runs << 30 | run0 | run1 << 15); 3 << 30 when
@@ -1607,6 +1612,9 @@ static int check_item_plan(const int32_t* prog, int64_t words, int64_t off, int3
     const int64_t glb_need = (int64_t)(uint32_t)p[6] | ((int64_t)(uint32_t)p[9] << 32);
     if (n_gates < 0 || n_ops < 0 || gates_off < kPlanHdr || ops_off < kPlanHdr)
         return set_error(QB_ERR_INVALID, "plan at %lld: bad header", (long long)off);
+    if (p[11] != 0)
+        return set_error(QB_ERR_INVALID, "plan at %lld has %d open variables; the executor returns scalars",
+                         (long long)off, p[11]);
     if (lds_need > lds_elems || lds_need > 65536)
         return set_error(QB_ERR_INVALID, "plan at %lld needs %lld LDS elements, the launch reserves %d",
                          (long long)off, (long long)lds_need, lds_elems);
@@ -1931,12 +1939,19 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
         if (rc) return rc;
         pos += w0 >> 16;
     }
-    if (!(result_loc & kGlobal) || (int64_t)(result_loc & ~kGlobal) >= cap)
+    const int32_t rank = w[11];
+    if (rank < 0 || rank > 24) return set_error(QB_ERR_INVALID, "result rank %d", rank);
+    const uint32_t n_res = 1u << rank;
+    if (!(result_loc & kGlobal) || (int64_t)(result_loc & ~kGlobal) + n_res > cap)
         return set_error(QB_ERR_INVALID, "result outside the arena");
+    const int64_t stride = d->out_stride > 0 ? d->out_stride : 1;
+    const int rblocks = (int)((n_res + 255) / 256 < 1024 ? (n_res + 255) / 256 : 1024);
     if (d->dtype == QB_C128)
-        hipLaunchKernelGGL(store_result<double>, dim3(1), dim3(1), 0, st, d->arena, result_loc & ~kGlobal, d->out);
+        hipLaunchKernelGGL(store_result<double>, dim3(rblocks), dim3(256), 0, st, d->arena, result_loc & ~kGlobal,
+                           n_res, stride, d->out);
     else
-        hipLaunchKernelGGL(store_result<float>, dim3(1), dim3(1), 0, st, d->arena, result_loc & ~kGlobal, d->out);
+        hipLaunchKernelGGL(store_result<float>, dim3(rblocks), dim3(256), 0, st, d->arena, result_loc & ~kGlobal,
+                           n_res, stride, d->out);
     QB_HIP(hipGetLastError());
     return QB_OK;
 }

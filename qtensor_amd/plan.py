@@ -516,8 +516,12 @@ class PlanCompiler:
     """
 
     def __init__(self, lds_tensor_limit=1 << 12, lds_capacity=None, elem_bytes=8, lanes=64,
-                 gather=True, window=None, gate_space="lds", schedule="list", fuse=False, reduce=True):
+                 gather=True, window=None, gate_space="lds", schedule="list", fuse=False, reduce=True,
+                 bake_angles=False):
         self.lds_tensor_limit = lds_tensor_limit
+        # gates keep the angle they were composed with instead of rebuilding it
+        # from the parameter vector (circuits that carry no parameter terms)
+        self.bake_angles = bake_angles
         self.lds_capacity = lds_capacity      # elements available per item, or None
         self.elem_bytes = elem_bytes
         self.lanes = lanes
@@ -556,7 +560,15 @@ class PlanCompiler:
 
     # -- back end
     def compile(self, buckets, data_dict, ket_vars, bra_vars, par_vars=(), treewidth=-1,
-                ket_state=0, bra_state=0):
+                ket_state=0, bra_state=0, free_vars=()):
+        """``free_vars`` (wide plans only): output variables kept open, whose
+        buckets come last (after the sliced ones); the result is then the
+        tensor over them (variables in ascending id, the first one the most
+        significant bit of the element index), and header word 11 holds their
+        count."""
+        free_ids = [int(v) for v in free_vars]
+        if free_ids and self.gate_space != "global":
+            raise ValueError("open (batch) variables need a wide plan")
         fixed = {}
         for vs, state in ((ket_vars, ket_state), (bra_vars, bra_state)):
             vs_sorted = sorted(vs, key=str)
@@ -630,7 +642,7 @@ class PlanCompiler:
                 return Node(nd.indices[1:], nd.dev, buf=nd.buf, step=nd.step)
             return emit(nodes, no_sum)
 
-        n_nosum = len(par_index)
+        n_nosum = len(par_index) + len(free_ids)
         n_contract = len(sym) - n_nosum
         scalars = []
         for b in range(n_contract):
@@ -644,14 +656,17 @@ class PlanCompiler:
         rest = [nd for b in range(n_contract, len(sym)) for nd in sym[b]]
         if rest:
             res = process(rest, no_sum=True)
-            if res.indices and any(v.size == 2 for v in res.indices):
-                raise ValueError("plans with free (batch) variables are not supported")
+            if any(v.size == 2 and int(v) not in free_ids for v in res.indices):
+                raise ValueError("plans with free (batch) variables need them in free_vars")
             scalars.append(res)
         if not scalars:
             raise ValueError("network has no scalar result")
         result = scalars[0] if len(scalars) == 1 else process(scalars, no_sum=True)
-        if result.dev:
-            raise ValueError("network result is not a scalar")
+        if sorted(result.dev) != sorted(free_ids):
+            raise ValueError("network result is not a tensor over exactly the free variables")
+        if free_ids and result.buf.leaf:
+            # an open result that is a gate tensor as is: copy it into an intermediate
+            result = emit([result], no_sum=True)
 
         # step-model figures of the bucket steps (before any fusion renumbers them)
         total_bytes, total_flops, max_rank = 0, 0, 0
@@ -759,7 +774,7 @@ class PlanCompiler:
         steps_words = sum(len(w) for w in op_words)
         words = [len(gates), n_ops, gates_off, steps_off, _i32(result.buf.loc),
                  lds.peak, _i32(glb.peak & 0xFFFFFFFF), len(par_index), steps_words,
-                 _i32(glb.peak >> 32), 0, 0]
+                 _i32(glb.peak >> 32), 0, len(free_ids)]
         for g in gates:
             words += g
         for w in op_words:
@@ -1025,6 +1040,8 @@ class PlanCompiler:
         alpha, terms = 0.0, ()
         if info is not None:
             kind, alpha, terms = info
+            if self.bake_angles:
+                terms = ()
         name = t.name.rstrip("+")
         if kind == KIND_CONST:
             kind = _KIND_BY_NAME.get(name, KIND_CONST)

@@ -131,6 +131,28 @@ class QAOAQtreeSimulator(QAOASimulator, QtreeSimulator):
     pass
 
 
+class WideQtreeSimulator(QtreeSimulator):
+    """``QtreeSimulator`` whose elimination runs on the device as one compiled
+    wide plan (gates built in FP64 on the device, one grid kernel per op)
+    instead of one ``process_bucket`` launch per bucket: same order rule
+    (greedy, open variables last), same ``simulate_batch`` result (the 2^b
+    amplitudes of the first b output qubits, ``qtensor/Simulate.py:66-94``).
+    ``max_tw`` guards the order's width as the reference does."""
+
+    def __init__(self, dtype="complex64", max_tw=None, device=0):
+        self.dtype = dtype
+        self.max_tw = max_tw
+        self.device = device
+
+    def simulate_batch(self, qc, batch_vars=0, peo=None):
+        from .wide import WideEngine, compile_wide
+        wp = compile_wide(qc, order=peo, batch_vars=batch_vars, bake_angles=True)
+        if self.max_tw and wp.treewidth > self.max_tw:
+            raise ValueError(f"Treewidth {wp.treewidth} is larger than max_tw={self.max_tw}.")
+        val = WideEngine(device=self.device, dtype=self.dtype).run(wp, [])
+        return np.atleast_1d(np.asarray(val, dtype=np.complex128)).flatten()
+
+
 def int_slice(value, vars_to_slice):
     """``{var: int}`` for the multi-index ``value`` over ``vars_to_slice``
     (``qtensor/FeynmanSimulator.py:18-25``)."""

@@ -99,25 +99,43 @@ class WidePlan:
         self.n_slices = 1 << n_slice_vars
 
 
-def compile_wide(circuit, order=None, par_ids=(), lanes=256, fuse=True):
+def compile_wide(circuit, order=None, par_ids=(), lanes=256, fuse=True, batch_vars=0, bake_angles=False):
     """Compile ``circuit`` with elimination ``order`` (variable ids of the line
     graph; greedy when None) and slice variables ``par_ids`` into a wide plan.
     With ``fuse``, chains of steps are contracted in one pass where the time
-    model favours it (``plan.fuse_steps``)."""
+    model favours it (``plan.fuse_steps``).  ``batch_vars`` = b keeps the
+    output wires of qubits 0..b-1 open, as ``QtreeSimulator.simulate_batch``
+    does (``qtensor/Simulate.py:66-94``): the plan's result is the 2^b
+    amplitudes, qubit 0 the most significant bit.  Their variables join the
+    line graph as a clique and go last in the order, as in
+    ``qtensor/optimisation/Optimizer.py:60-80``.  ``bake_angles``: gates keep
+    the angles they were composed with (no parameter vector needed)."""
     tn = QtreeTensorNet.from_qtree_gates(circuit)
-    graph = tn.get_line_graph()
+    free = list(tn.bra_vars[:batch_vars])
+    bra = list(tn.bra_vars[batch_vars:])
+    ignore = list(tn.ket_vars) + bra
+    from .circuit import buckets2graph
+    graph = buckets2graph(tn.buckets, ignore_variables=ignore)
+    free_ids = [int(v) for v in free]
+    for a in range(len(free_ids)):
+        for b in range(a + 1, len(free_ids)):
+            graph.add_edge(free_ids[a], free_ids[b])
     if order is None:
         order, _ = greedy_peo(graph)
+    order = [int(v) for v in order if int(v) in graph]     # line-graph variables only
     par_ids = list(par_ids)
-    rest = [v for v in order if v not in set(par_ids)]
+    drop = set(par_ids) | set(free_ids)
+    rest = [v for v in order if v not in drop]
     g = graph.copy()
     g.remove_nodes_from(par_ids)
-    tw = max(elimination_widths(g, rest)) if rest else 0
-    peo = list(tn.ket_vars) + list(tn.bra_vars) + _vars(graph, rest) + _vars(graph, par_ids)
+    tw = max(elimination_widths(g, rest + free_ids)) if rest else 0
+    peo = list(tn.ket_vars) + bra + _vars(graph, rest) + _vars(graph, par_ids) + free
     buckets, perm = reorder_buckets(tn.buckets, peo)
-    pc = PlanCompiler(lds_tensor_limit=0, gather=False, gate_space="global", lanes=lanes, fuse=fuse)
+    pc = PlanCompiler(lds_tensor_limit=0, gather=False, gate_space="global", lanes=lanes, fuse=fuse,
+                      bake_angles=bake_angles)
     plan = pc.compile(buckets, tn.data_dict, [perm[v] for v in tn.ket_vars],
-                      [perm[v] for v in tn.bra_vars], [perm[v] for v in _vars(graph, par_ids)], tw)
+                      [perm[v] for v in bra], [perm[v] for v in _vars(graph, par_ids)], tw,
+                      free_vars=[perm[v] for v in free])
     return WidePlan(plan, tw, len(par_ids))
 
 
@@ -129,7 +147,9 @@ class WideEngine:
         self.dtype = dtype
 
     def run(self, wplan, params, stream=None, timing=False, n_streams=4):
-        """Sum of the plan over all its slices (complex, FP64 accumulation).
+        """Sum of the plan over all its slices (complex, FP64 accumulation): a
+        complex scalar, or for a plan with b open variables a complex128 numpy
+        array of the 2^b amplitudes.
 
         Slices are independent: they are spread over ``n_streams`` streams, each
         with a private arena, so the latency-bound small steps of one slice
@@ -146,10 +166,18 @@ class WideEngine:
         words = np.ascontiguousarray(plan.words, dtype=np.int32)
         plan_dev = torch.from_numpy(words).to(dev)
         arenas = [torch.empty(max(plan.glb_need, 1), dtype=tdtype, device=dev) for _ in range(n_streams)]
-        prm = torch.tensor([float(x) for x in params], dtype=torch.float64, device=dev)
-        out = torch.zeros(2 * n, dtype=torch.float64, device=dev)
-        seg = torch.tensor([0, n], dtype=torch.int32, device=dev)
-        term = torch.zeros(2, dtype=torch.float64, device=dev)
+        rank = int(words[11])
+        if rank > 20:
+            raise ValueError(f"{rank} open variables: at most 20")
+        n_res = 1 << rank
+        # angles, padded (the gate builder may read a few words past the last angle)
+        prm = torch.zeros(len(params) + 8, dtype=torch.float64, device=dev)
+        if len(params):
+            prm[:len(params)] = torch.tensor([float(x) for x in params], dtype=torch.float64)
+        # element e of slice s at item e * n + s: the slices of each element are one segment
+        out = torch.zeros(2 * n * n_res, dtype=torch.float64, device=dev)
+        seg = torch.arange(0, n * n_res + 1, n, dtype=torch.int32, device=dev)
+        term = torch.zeros(2 * n_res, dtype=torch.float64, device=dev)
         total = torch.zeros(2, dtype=torch.float64, device=dev)
         main = stream if stream is not None else torch.cuda.current_stream(dev)
         streams = [main] + [torch.cuda.Stream(device=dev) for _ in range(n_streams - 1)]
@@ -167,6 +195,7 @@ class WideEngine:
             d.arena = arenas[k].data_ptr()
             d.arena_elems = arenas[k].numel()
             d.dtype = qb_dtype
+            d.out_stride = n
             descs.append(d)
         for s in range(n):
             k = s % n_streams
@@ -179,12 +208,16 @@ class WideEngine:
             ev.record(st)
             main.wait_event(ev)
         h = ctypes.c_void_p(main.cuda_stream)
-        _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(seg.data_ptr()), 1,
+        _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(seg.data_ptr()), n_res,
                                        ctypes.c_void_p(term.data_ptr()), ctypes.c_void_p(total.data_ptr()), h))
         ev1.record(main)
         ev1.synchronize()
-        r = total.cpu().numpy()
-        val = complex(r[0], r[1])
+        if rank:
+            t = term.cpu().numpy()
+            val = t[0::2] + 1j * t[1::2]
+        else:
+            r = total.cpu().numpy()
+            val = complex(r[0], r[1])
         if timing:
             return val, ev0.elapsed_time(ev1) * 1e-3
         return val

@@ -200,4 +200,7 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
             buf[(out_loc & ~P.GLOBAL) + o] = acc
         pos += length
     buf = glb if result_loc & P.GLOBAL else lds
-    return complex(buf[(result_loc & 0xFFFFFFFF) & ~P.GLOBAL])
+    loc = (result_loc & 0xFFFFFFFF) & ~P.GLOBAL
+    if w[11]:            # open variables: the 2^rank result tensor
+        return np.array(buf[loc: loc + (1 << w[11])], dtype=np.complex128)
+    return complex(buf[loc])

@@ -103,3 +103,30 @@ def test_wide_rejects_arena_overflow(gpu):
     d.slice_bits, d.dtype, d.out = 0, _lib.QB_C64, out.data_ptr()
     rc = lib.qb_wide_execute(ctypes.byref(d), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     assert rc == _lib.QB_ERR_INVALID and b"arena" in lib.qb_last_error()
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex64", 1e-5), ("complex128", 1e-11)])
+def test_wide_simulator_amplitude_batches(gpu, dtype, tol):
+    """WideQtreeSimulator.simulate_batch(batch_vars=b) (qtensor/Simulate.py:66-94)
+    on the device: the 2^b amplitudes of the statevector, and a sliced plan with
+    open variables (4 slices on 4 streams, element-wise compensated sums)."""
+    from oracle import statevector
+    from qtensor_amd.simulator import WideQtreeSimulator
+    from qtensor_amd.ordering import greedy_peo
+    from qtensor_amd.tensornet import QtreeTensorNet
+    from qtensor_amd.wide import WideEngine, compile_wide, slice_vars_for_order
+    for seed, b in ((3, 0), (4, 2), (5, 4)):
+        G = nx.random_regular_graph(3, 12, seed=seed)
+        gamma, beta = [0.3, 0.5], [0.2, 0.7]
+        comp = QtreeQAOAComposer(G, gamma=gamma, beta=beta)
+        comp.ansatz_state()
+        nq = G.number_of_nodes()
+        sv = statevector.qaoa_state(nq, [tuple(e) for e in G.edges()], gamma, beta)
+        ref = np.array([sv[k << (nq - b)] for k in range(1 << b)])
+        got = WideQtreeSimulator(dtype=dtype).simulate_batch(comp.circuit, batch_vars=b)
+        assert got.shape == ref.shape and np.max(np.abs(got - ref)) <= tol
+        graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+        order, _ = greedy_peo(graph)
+        ws = compile_wide(comp.circuit, order=order, par_ids=slice_vars_for_order(graph, order, 2), batch_vars=b)
+        val = np.atleast_1d(WideEngine(dtype=dtype).run(ws, gamma + beta))
+        assert np.max(np.abs(val - ref)) <= tol

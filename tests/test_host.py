@@ -760,3 +760,36 @@ def test_greedy_from_buckets_equals_line_graph_greedy():
                 assert opt.treewidth == max(path)
                 assert list(opt.graph.nodes) == list(g.nodes)
                 assert {frozenset(x) for x in opt.graph.edges} == {frozenset(x) for x in g.edges}
+
+
+@pytest.mark.parametrize("fuse", [False, True], ids=["steps", "fused"])
+def test_wide_plan_open_variables_vs_statevector(fuse):
+    """Wide plans with b open output variables (simulate_batch(batch_vars=b),
+    qtensor/Simulate.py:66-94): the interpreted program gives the 2^b
+    amplitudes of the statevector (qubits 0..b-1 most significant, the rest
+    0), with angles from the parameter vector or baked in, and sliced plans
+    sum to the same amplitudes."""
+    from plan_interp import run_plan
+    from qtensor_amd.tensornet import QtreeTensorNet
+    from qtensor_amd.wide import compile_wide, slice_vars_for_order
+    from qtensor_amd.ordering import greedy_peo
+    for seed, b in ((3, 1), (4, 2), (5, 3)):
+        G = nx.random_regular_graph(3, 10, seed=seed)
+        gamma, beta = [0.3, 0.5], [0.2, 0.7]
+        comp = QtreeQAOAComposer(G, gamma=gamma, beta=beta)
+        comp.ansatz_state()
+        nq = G.number_of_nodes()
+        sv = statevector.qaoa_state(nq, [tuple(e) for e in G.edges()], gamma, beta)
+        ref = np.array([sv[k << (nq - b)] for k in range(1 << b)])
+        prm = np.array(gamma + beta + [0.0] * 8)
+        for bake in (False, True):
+            wp = compile_wide(comp.circuit, batch_vars=b, fuse=fuse, bake_angles=bake)
+            assert wp.plan.words[11] == b
+            got = run_plan(wp.plan.words, np.zeros(8) if bake else prm)
+            assert np.allclose(got, ref, atol=1e-12)
+        graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+        order, _ = greedy_peo(graph)
+        par = slice_vars_for_order(graph, order, 2)
+        ws = compile_wide(comp.circuit, order=order, par_ids=par, batch_vars=b, fuse=fuse)
+        got = sum(run_plan(ws.plan.words, prm, slice_bits=s) for s in range(ws.n_slices))
+        assert ws.n_slices == 4 and np.allclose(got, ref, atol=1e-12)
