@@ -810,13 +810,15 @@ def main():
             "pool_vs_single_max_abs_diff": cb["pool_max_abs_diff"]}
         out["max_abs_err_vs_oracle_sample"] = float(np.max(np.abs(terms64[:len(cv)] - cv)))
     if args.workload == "c2" and not args.no_serve:
-        # C5: a bounded serving leg (cold plan store, then a warm trace at saturation)
-        res = serve_bench([2000.0], 120, 8, cold_rate=20.0, cold_requests=40)
-        warm = res["rates"][0]
+        # C5: a bounded serving leg (cold plan store; then warm traces at 40 requests/s,
+        # where latency is the server's own, and at saturation, where throughput is)
+        res = serve_bench([40.0, 2000.0], 80, 8, cold_rate=20.0, cold_requests=40)
+        warm, sat = res["rates"]
         out["served"] = {"workload": "poisson_energy_requests_3reg_n20_2000_p1_4",
-                         "cold_trace": res["cold_trace"], "saturation_trace": warm,
+                         "cold_trace": res["cold_trace"], "warm_trace": warm, "saturation_trace": sat,
                          "pool_planning_s": res["pool_planning_s"],
-                         "p50_ms": warm["p50_ms"], "p99_ms": warm["p99_ms"], "terms_per_s": warm["terms_per_s"]}
+                         "p50_ms": warm["p50_ms"], "p99_ms": warm["p99_ms"], "latency_at_rate_per_s": 40.0,
+                         "terms_per_s": sat["terms_per_s"]}
     print(json.dumps(out), flush=True)
 
 
