@@ -81,6 +81,11 @@ def load():
             raise ImportError(
                 f"{LIB_PATH} is missing; build it with `python -m qtensor_amd.build` "
                 "(the engine has no CPU fallback)")
+        # torch carries its own HIP and HSA runtimes; loading it first makes the
+        # library bind to those same runtimes (one runtime per process).  Opened
+        # first, the library would bring a second pair, and whichever of the two
+        # initialises the device second finds none.
+        import torch  # noqa: F401
         lib = ctypes.CDLL(LIB_PATH)
         lib.qb_version.restype = ctypes.c_int
         lib.qb_init.argtypes = [ctypes.c_int]

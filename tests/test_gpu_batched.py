@@ -218,3 +218,22 @@ def test_autotuner_pick_matches_statevector(gpu):
     sv = statevector.zz_expectations(G, g, b)
     for e, z in zip(G.edges(), zz):
         assert abs(z - sv[e]) <= _tol(sv[e])
+
+
+def test_library_loaded_before_torch_touches_the_device(gpu):
+    """A fresh process that plans (loading the library) before torch has used
+    the GPU still finds the device: the library binds to torch's HIP runtime."""
+    import os
+    import subprocess
+    import sys
+    code = ("import networkx as nx\n"
+            "from qtensor_amd.batched import LightconeEngine\n"
+            "eng = LightconeEngine()\n"
+            "G = nx.random_regular_graph(3, 10, seed=42)\n"
+            "wl = eng.workload(G, 1)\n"
+            "C, zz = eng.energy(G, [0.3], [0.2])\n"
+            "print(len(zz))\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=100)
+    assert res.returncode == 0, res.stderr[-2000:]
+    assert res.stdout.strip().splitlines()[-1] == "15"
