@@ -750,7 +750,10 @@ def main():
                      "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
                      "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,
                      "executed_step_bytes_per_launch": batch.bytes_step_model,
-                     "measured_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1)},
+                     "measured_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1),
+                     # every intermediate of these plans lives in LDS: the steps' bytes as executed
+                     # (sliced plans) over the kernel time are the LDS-side rate
+                     "lds_side_GBs": round(batch.bytes_step_model / (kern_ms * 1e-3) / 1e9, 1)},
     }
     if not meets:
         out["error"] = f"a term misses the 1e-5 bar (max rel err {worst:.3e}): value withheld"
