@@ -145,6 +145,16 @@ class WideEngine:
     def __init__(self, device=0, dtype="complex64"):
         self.device_index = device
         self.dtype = dtype
+        self._threads = None
+
+    def _pool(self, n):
+        """Host threads that enqueue the streams' slices (kept for later runs)."""
+        from concurrent.futures import ThreadPoolExecutor
+        if self._threads is None or self._threads[0] < n:
+            if self._threads is not None:
+                self._threads[1].shutdown(wait=False)
+            self._threads = (n, ThreadPoolExecutor(max_workers=n, thread_name_prefix="qb-wide"))
+        return self._threads[1]
 
     def run(self, wplan, params, stream=None, timing=False, n_streams=4):
         """Sum of the plan over all its slices (complex, FP64 accumulation): a
@@ -197,12 +207,21 @@ class WideEngine:
             d.dtype = qb_dtype
             d.out_stride = n
             descs.append(d)
-        for s in range(n):
-            k = s % n_streams
+        def issue(k):
+            # the slices of stream k, in order (the library releases the GIL, so
+            # the streams' host-side parsing and launches proceed in parallel)
             d = descs[k]
-            d.slice_bits = s
-            d.out = out.data_ptr() + 16 * s
-            _lib.check(lib.qb_wide_execute(ctypes.byref(d), ctypes.c_void_p(streams[k].cuda_stream)))
+            h = ctypes.c_void_p(streams[k].cuda_stream)
+            for s in range(k, n, n_streams):
+                d.slice_bits = s
+                d.out = out.data_ptr() + 16 * s
+                _lib.check(lib.qb_wide_execute(ctypes.byref(d), h))
+
+        if n_streams == 1:
+            issue(0)
+        else:
+            for f in [self._pool(n_streams).submit(issue, k) for k in range(n_streams)]:
+                f.result()
         for st in streams[1:]:
             ev = torch.cuda.Event()
             ev.record(st)
