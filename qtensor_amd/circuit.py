@@ -57,27 +57,33 @@ def circ2buckets(qubit_count, circuit):
     buckets = [[] for _ in range(qubit_count)]
     next_id = qubit_count
 
+    gate_info = data_dict.gate_info
+
     def place(gate, closing_names=None):
         nonlocal next_id
-        changed = set(gate._changes_qubits)
+        changed = gate._changes_qubits
         indices = []
+        low = None
         for pos, q in enumerate(gate.qubits):
+            cur = layer_vars[q]
+            if low is None or cur.identity < low:
+                low = cur.identity
             if pos in changed:
                 name = closing_names[q] if closing_names else None
                 new = Var(next_id, name=name)
                 next_id += 1
                 buckets.append([])
-                indices += [layer_vars[q], new]
+                indices += [cur, new]
                 layer_vars[q] = new
             else:
-                indices.append(layer_vars[q])
-        key = (gate.label, len(data_dict))
+                indices.append(cur)
+        label = gate.label
+        key = (label, len(data_dict))
         data_dict[key] = gate.tensor()
-        terms = tuple(getattr(gate, "param_terms", ()))
-        alpha = gate.parameters.get("alpha", 0.0)
-        data_dict.gate_info[key] = (gate.kind, alpha, terms)
-        t = Tensor(gate.label, indices, data_key=key)
-        buckets[min(int(v) for v in indices)].append(t)
+        gate_info[key] = (gate.kind, gate.parameters.get("alpha", 0.0), tuple(getattr(gate, "param_terms", ())))
+        # a new variable's id exceeds every current one, so the smallest index
+        # is the smallest of the wires' current variables
+        buckets[low].append(Tensor(label, indices, data_key=key))
 
     for q in range(qubit_count):
         place(M(q))
@@ -100,14 +106,17 @@ def reorder_buckets(old_buckets, permutation):
     Returns ``(perm_buckets, perm_dict)``.
     """
     perm_dict = {}
+    by_id = {}
     for n, var in enumerate(permutation):
-        perm_dict[var] = var.copy(identity=n)
+        nv = var.copy(identity=n)
+        perm_dict[var] = nv
+        by_id[var.identity] = nv
     perm_buckets = [[] for _ in range(len(permutation))]
     for bucket in old_buckets:
         for tensor in bucket:
-            new_indices = tuple(perm_dict[v] for v in tensor.indices)
-            b = min(int(v) for v in new_indices) if new_indices else 0
-            perm_buckets[b].append(tensor.copy(indices=new_indices))
+            new_indices = tuple([by_id[v.identity] for v in tensor.indices])
+            b = min([v.identity for v in new_indices]) if new_indices else 0
+            perm_buckets[b].append(Tensor(tensor.name, new_indices, tensor.data_key, tensor.data))
     return perm_buckets, perm_dict
 
 

@@ -136,7 +136,10 @@ def _line_csr(buckets, ignore_ids):
                     first.append(v)
                     adj.append(set())
                 ids.append(i)
-            if len(ids) > 1:
+            if len(ids) == 2:
+                adj[ids[0]].add(ids[1])
+                adj[ids[1]].add(ids[0])
+            elif len(ids) > 2:
                 for a in ids:
                     adj[a].update(ids)
     ptr = np.zeros(len(first) + 1, dtype=np.int32)
