@@ -156,7 +156,7 @@ class WideEngine:
             self._threads = (n, ThreadPoolExecutor(max_workers=n, thread_name_prefix="qb-wide"))
         return self._threads[1]
 
-    def run(self, wplan, params, stream=None, timing=False, n_streams=4):
+    def run(self, wplan, params, stream=None, timing=False, n_streams=4, graph=False):
         """Sum of the plan over all its slices (complex, FP64 accumulation): a
         complex scalar, or for a plan with b open variables a complex128 numpy
         array of the 2^b amplitudes.
@@ -164,7 +164,13 @@ class WideEngine:
         Slices are independent: they are spread over ``n_streams`` streams, each
         with a private arena, so the latency-bound small steps of one slice
         overlap the bandwidth-bound large steps of another.  With ``timing``
-        also returns the device time in seconds."""
+        also returns the device time in seconds.  With ``graph``, each stream's
+        launches are captured in a HIP graph the first time the plan runs and
+        replayed afterwards (new angles go into the captured angle buffer), so
+        a re-evaluation costs one graph launch per stream instead of a host
+        pass over every op."""
+        if graph:
+            return self._run_graph(wplan, params, stream, timing, n_streams)
         import torch
         lib = _lib.init(self.device_index)
         dev = torch.device("cuda", self.device_index)
@@ -236,6 +242,106 @@ class WideEngine:
             val = t[0::2] + 1j * t[1::2]
         else:
             r = total.cpu().numpy()
+            val = complex(r[0], r[1])
+        if timing:
+            return val, ev0.elapsed_time(ev1) * 1e-3
+        return val
+
+    _GRAPH_PARAMS = 256        # angles a captured plan can take
+
+    def _buffers(self, wplan, n_streams, n_params):
+        """Device buffers of one evaluation (arenas, angles, per-slice results,
+        segment table, terms, total)."""
+        import torch
+        dev = torch.device("cuda", self.device_index)
+        tdtype = torch.complex64 if self.dtype == "complex64" else torch.complex128
+        plan = wplan.plan
+        n = wplan.n_slices
+        words = np.ascontiguousarray(plan.words, dtype=np.int32)
+        rank = int(words[11])
+        if rank > 20:
+            raise ValueError(f"{rank} open variables: at most 20")
+        n_res = 1 << rank
+        return {"words": words, "plan_dev": torch.from_numpy(words).to(dev), "rank": rank, "n_res": n_res,
+                "arenas": [torch.empty(max(plan.glb_need, 1), dtype=tdtype, device=dev) for _ in range(n_streams)],
+                "prm": torch.zeros(n_params + 8, dtype=torch.float64, device=dev),
+                "out": torch.zeros(2 * n * n_res, dtype=torch.float64, device=dev),
+                "seg": torch.arange(0, n * n_res + 1, n, dtype=torch.int32, device=dev),
+                "term": torch.zeros(2 * n_res, dtype=torch.float64, device=dev),
+                "total": torch.zeros(2, dtype=torch.float64, device=dev)}
+
+    def _issue(self, lib, b, wplan, streams, k, n_streams):
+        """Launch the slices of stream k (k, k + n_streams, ...) on it."""
+        n = wplan.n_slices
+        d = _lib.WideDesc()
+        d.plan_host = b["words"].ctypes.data
+        d.plan_words = b["words"].size
+        d.plan_dev = b["plan_dev"].data_ptr()
+        d.params = b["prm"].data_ptr()
+        d.arena = b["arenas"][k].data_ptr()
+        d.arena_elems = b["arenas"][k].numel()
+        d.dtype = _lib.QB_C64 if self.dtype == "complex64" else _lib.QB_C128
+        d.out_stride = n
+        h = ctypes.c_void_p(streams[k].cuda_stream)
+        for s in range(k, n, n_streams):
+            d.slice_bits = s
+            d.out = b["out"].data_ptr() + 16 * s
+            _lib.check(lib.qb_wide_execute(ctypes.byref(d), h))
+
+    def _run_graph(self, wplan, params, stream, timing, n_streams):
+        import torch
+        lib = _lib.init(self.device_index)
+        dev = torch.device("cuda", self.device_index)
+        if len(params) > self._GRAPH_PARAMS:
+            raise ValueError(f"{len(params)} angles: a captured plan takes at most {self._GRAPH_PARAMS}")
+        n_streams = max(1, min(n_streams, wplan.n_slices))
+        if not hasattr(self, "_graphs"):
+            self._graphs = {}
+        key = (id(wplan), n_streams)
+        st = self._graphs.get(key)
+        if st is None or st["plan"] is not wplan:
+            b = self._buffers(wplan, n_streams, self._GRAPH_PARAMS)
+            streams = [torch.cuda.Stream(device=dev) for _ in range(n_streams)]
+            torch.cuda.synchronize(dev)
+            for k in range(n_streams):          # an eager pass first: kernels are loaded outside capture
+                self._issue(lib, b, wplan, streams, k, n_streams)
+            torch.cuda.synchronize(dev)
+            graphs = []
+            for k in range(n_streams):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=streams[k]):
+                    self._issue(lib, b, wplan, streams, k, n_streams)
+                graphs.append(g)
+            torch.cuda.synchronize(dev)
+            st = {"plan": wplan, "buf": b, "streams": streams, "graphs": graphs}
+            self._graphs[key] = st
+        b, streams = st["buf"], st["streams"]
+        main = stream if stream is not None else torch.cuda.current_stream(dev)
+        vals = torch.tensor([float(x) for x in params] + [0.0] * (self._GRAPH_PARAMS + 8 - len(params)),
+                            dtype=torch.float64)
+        with torch.cuda.stream(main):
+            b["prm"].copy_(vals)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(main)
+        for k, g in enumerate(st["graphs"]):
+            streams[k].wait_event(ev0)
+            with torch.cuda.stream(streams[k]):
+                g.replay()
+        for s_ in streams:
+            ev = torch.cuda.Event()
+            ev.record(s_)
+            main.wait_event(ev)
+        h = ctypes.c_void_p(main.cuda_stream)
+        _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(b["out"].data_ptr()), ctypes.c_void_p(b["seg"].data_ptr()),
+                                       b["n_res"], ctypes.c_void_p(b["term"].data_ptr()),
+                                       ctypes.c_void_p(b["total"].data_ptr()), h))
+        ev1.record(main)
+        ev1.synchronize()
+        if b["rank"]:
+            t = b["term"].cpu().numpy()
+            val = t[0::2] + 1j * t[1::2]
+        else:
+            r = b["total"].cpu().numpy()
             val = complex(r[0], r[1])
         if timing:
             return val, ev0.elapsed_time(ev1) * 1e-3

@@ -130,3 +130,27 @@ def test_wide_simulator_amplitude_batches(gpu, dtype, tol):
         ws = compile_wide(comp.circuit, order=order, par_ids=slice_vars_for_order(graph, order, 2), batch_vars=b)
         val = np.atleast_1d(WideEngine(dtype=dtype).run(ws, gamma + beta))
         assert np.max(np.abs(val - ref)) <= tol
+
+
+def test_wide_graph_replays_match_eager(gpu):
+    """WideEngine.run(graph=True): the slices' launches captured once per
+    stream and replayed with new angles.  Every replay, with eager device work
+    and synchronisations between replays, equals the eager evaluation of the
+    same angles, and the first matches the statevector."""
+    import torch
+    G, e, g, b, circ = _edge_circuit()
+    graph = QtreeTensorNet.from_qtree_gates(circ).get_line_graph()
+    order = forced_width_order(graph, target_tw=14, temp=3.0, seed=1)
+    wp = compile_wide(circ, order=order, par_ids=slice_vars_for_order(graph, order, 3))
+    eng = WideEngine(dtype="complex128")
+    first = eng.run(wp, g + b, graph=True)
+    assert abs(first - statevector.zz_expectations(G, g, b)[e]) <= 1e-11
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        prm = list(rng.uniform(0, np.pi, 2)) + list(rng.uniform(0, np.pi / 2, 2))
+        eager = eng.run(wp, prm)
+        x = torch.randn(1 << 16, device="cuda")
+        (x * x).sum().item()                   # unrelated device work between replays
+        torch.cuda.synchronize()
+        replay = eng.run(wp, prm, graph=True)
+        assert abs(replay - eager) <= 1e-13, (replay, eager)
