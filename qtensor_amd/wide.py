@@ -174,74 +174,46 @@ class WideEngine:
         import torch
         lib = _lib.init(self.device_index)
         dev = torch.device("cuda", self.device_index)
-        qb_dtype = _lib.QB_C64 if self.dtype == "complex64" else _lib.QB_C128
-        tdtype = torch.complex64 if self.dtype == "complex64" else torch.complex128
-        plan = wplan.plan
-        n = wplan.n_slices
-        n_streams = max(1, min(n_streams, n))
-        words = np.ascontiguousarray(plan.words, dtype=np.int32)
-        plan_dev = torch.from_numpy(words).to(dev)
-        arenas = [torch.empty(max(plan.glb_need, 1), dtype=tdtype, device=dev) for _ in range(n_streams)]
-        rank = int(words[11])
-        if rank > 20:
-            raise ValueError(f"{rank} open variables: at most 20")
-        n_res = 1 << rank
-        # angles, padded (the gate builder may read a few words past the last angle)
-        prm = torch.zeros(len(params) + 8, dtype=torch.float64, device=dev)
+        n_streams = max(1, min(n_streams, wplan.n_slices))
+        b = self._buffers(wplan, n_streams, len(params))
         if len(params):
-            prm[:len(params)] = torch.tensor([float(x) for x in params], dtype=torch.float64)
-        # element e of slice s at item e * n + s: the slices of each element are one segment
-        out = torch.zeros(2 * n * n_res, dtype=torch.float64, device=dev)
-        seg = torch.arange(0, n * n_res + 1, n, dtype=torch.int32, device=dev)
-        term = torch.zeros(2 * n_res, dtype=torch.float64, device=dev)
-        total = torch.zeros(2, dtype=torch.float64, device=dev)
+            b["prm"][:len(params)] = torch.tensor([float(x) for x in params], dtype=torch.float64)
         main = stream if stream is not None else torch.cuda.current_stream(dev)
         streams = [main] + [torch.cuda.Stream(device=dev) for _ in range(n_streams - 1)]
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0 = torch.cuda.Event(enable_timing=True)
         ev0.record(main)
         for st in streams[1:]:
             st.wait_event(ev0)
-        descs = []
-        for k in range(n_streams):
-            d = _lib.WideDesc()
-            d.plan_host = words.ctypes.data
-            d.plan_words = words.size
-            d.plan_dev = plan_dev.data_ptr()
-            d.params = prm.data_ptr()
-            d.arena = arenas[k].data_ptr()
-            d.arena_elems = arenas[k].numel()
-            d.dtype = qb_dtype
-            d.out_stride = n
-            descs.append(d)
-        def issue(k):
-            # the slices of stream k, in order (the library releases the GIL, so
-            # the streams' host-side parsing and launches proceed in parallel)
-            d = descs[k]
-            h = ctypes.c_void_p(streams[k].cuda_stream)
-            for s in range(k, n, n_streams):
-                d.slice_bits = s
-                d.out = out.data_ptr() + 16 * s
-                _lib.check(lib.qb_wide_execute(ctypes.byref(d), h))
-
         if n_streams == 1:
-            issue(0)
+            self._issue(lib, b, wplan, streams, 0, 1)
         else:
-            for f in [self._pool(n_streams).submit(issue, k) for k in range(n_streams)]:
+            # one host thread per stream: the library releases the GIL, so the
+            # streams' host-side parsing and launches proceed in parallel
+            futs = [self._pool(n_streams).submit(self._issue, lib, b, wplan, streams, k, n_streams)
+                    for k in range(n_streams)]
+            for f in futs:
                 f.result()
-        for st in streams[1:]:
+        return self._finish(lib, b, main, streams[1:], ev0, timing)
+
+    def _finish(self, lib, b, main, others, ev0, timing):
+        """Join the streams, reduce the slices per result element (compensated
+        FP64) and return the value (and the device time)."""
+        import torch
+        for s_ in others:
             ev = torch.cuda.Event()
-            ev.record(st)
+            ev.record(s_)
             main.wait_event(ev)
-        h = ctypes.c_void_p(main.cuda_stream)
-        _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(seg.data_ptr()), n_res,
-                                       ctypes.c_void_p(term.data_ptr()), ctypes.c_void_p(total.data_ptr()), h))
+        _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(b["out"].data_ptr()), ctypes.c_void_p(b["seg"].data_ptr()),
+                                       b["n_res"], ctypes.c_void_p(b["term"].data_ptr()),
+                                       ctypes.c_void_p(b["total"].data_ptr()), ctypes.c_void_p(main.cuda_stream)))
+        ev1 = torch.cuda.Event(enable_timing=True)
         ev1.record(main)
         ev1.synchronize()
-        if rank:
-            t = term.cpu().numpy()
+        if b["rank"]:
+            t = b["term"].cpu().numpy()
             val = t[0::2] + 1j * t[1::2]
         else:
-            r = total.cpu().numpy()
+            r = b["total"].cpu().numpy()
             val = complex(r[0], r[1])
         if timing:
             return val, ev0.elapsed_time(ev1) * 1e-3
@@ -321,28 +293,10 @@ class WideEngine:
                             dtype=torch.float64)
         with torch.cuda.stream(main):
             b["prm"].copy_(vals)
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0 = torch.cuda.Event(enable_timing=True)
         ev0.record(main)
         for k, g in enumerate(st["graphs"]):
             streams[k].wait_event(ev0)
             with torch.cuda.stream(streams[k]):
                 g.replay()
-        for s_ in streams:
-            ev = torch.cuda.Event()
-            ev.record(s_)
-            main.wait_event(ev)
-        h = ctypes.c_void_p(main.cuda_stream)
-        _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(b["out"].data_ptr()), ctypes.c_void_p(b["seg"].data_ptr()),
-                                       b["n_res"], ctypes.c_void_p(b["term"].data_ptr()),
-                                       ctypes.c_void_p(b["total"].data_ptr()), h))
-        ev1.record(main)
-        ev1.synchronize()
-        if b["rank"]:
-            t = b["term"].cpu().numpy()
-            val = t[0::2] + 1j * t[1::2]
-        else:
-            r = b["total"].cpu().numpy()
-            val = complex(r[0], r[1])
-        if timing:
-            return val, ev0.elapsed_time(ev1) * 1e-3
-        return val
+        return self._finish(lib, b, main, streams, ev0, timing)
