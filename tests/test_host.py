@@ -793,3 +793,37 @@ def test_wide_plan_open_variables_vs_statevector(fuse):
         ws = compile_wide(comp.circuit, order=order, par_ids=par, batch_vars=b, fuse=fuse)
         got = sum(run_plan(ws.plan.words, prm, slice_bits=s) for s in range(ws.n_slices))
         assert ws.n_slices == 4 and np.allclose(got, ref, atol=1e-12)
+
+
+def test_small_inputs_multiplied_first_keeps_the_value(monkeypatch):
+    """plan.split_small_inputs: in a wide p=2 lightcone plan under a wide order,
+    steps of three or more inputs get their small inputs multiplied first (the
+    pass fires), and the interpreted program still gives the statevector value;
+    the same plan without the pass agrees."""
+    from qtensor_amd import plan as P
+    from qtensor_amd import wide as W
+    from qtensor_amd.tensornet import QtreeTensorNet
+    counts = []
+    real = P.split_small_inputs
+
+    def counting(steps):
+        out = real(steps)
+        counts.append(len(out) - len(steps))
+        return out
+
+    G = nx.random_regular_graph(3, 16, seed=4)
+    g, b = [0.3, 0.7], [0.5, 0.1]
+    e = list(G.edges())[0]
+    comp = QtreeQAOAComposer(G, gamma=g, beta=b)
+    comp.energy_expectation_lightcone(e)
+    ref = statevector.zz_expectations(G, g, b)[e]
+    graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+    order = W.forced_width_order(graph, target_tw=16, temp=3.0, seed=0)      # greedy: width 10
+    monkeypatch.setattr(P, "split_small_inputs", counting)
+    with_pass = W.compile_wide(comp.circuit, order=order)
+    assert counts and counts[-1] > 0
+    monkeypatch.setattr(P, "split_small_inputs", lambda steps: steps)
+    without = W.compile_wide(comp.circuit, order=order)
+    assert with_pass.plan.n_ops > without.plan.n_ops
+    for wp in (with_pass, without):
+        assert abs(run_plan(wp.plan.words, g + b, lanes=256) - ref) < 1e-10
