@@ -461,7 +461,8 @@ def serve_pool(pool_size=8, seed=5):
 
 def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_choices=(1, 2, 3, 4),
                 slicer="tree_trim", shape4=None, max_tw4=16, slicer4="tree_trim", ordering="greedy",
-                ordering4="greedy", cold_rate=None, cold_requests=0):
+                ordering4="greedy", cold_rate=None, cold_requests=0, rg_repeats4=16, max_slice_vars=None,
+                upgrade4=None):
     """Poisson streams of energy requests (C5) through the continuous-batching server.
     Depths 1-3 use ``shape`` (default 64-thread items) sliced below treewidth
     ``max_tw``; depth 4 uses ``shape4`` (default 256-thread items) sliced below
@@ -482,10 +483,20 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
     eng = LightconeEngine(shape=shape or LaunchShape(64, 4096, 2048), max_tw=max_tw, slicer=slicer,
                           ordering=ordering)
     eng4 = LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=max_tw4, slicer=slicer4,
-                           ordering=ordering4)
+                           ordering=ordering4, rg_repeats=rg_repeats4)
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
-    server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=workers).start()
-    out = {"pool": [k for k, _ in pool], "p_choices": list(p_choices), "planning_workers": workers, "rates": []}
+    upgrades = None
+    if upgrade4 is not None:
+        # depth 4 re-planned in the background: randomized-greedy order, sliced below upgrade4[0]
+        upgrades = {4: LightconeEngine(shape=shape4 or LaunchShape(256, 4096, 2048), max_tw=upgrade4[0],
+                                       slicer=slicer4, ordering="rgreedy", rg_repeats=upgrade4[1])}
+    server = EnergyServer(eng, n_streams=2, engines={4: eng4}, planning_workers=workers,
+                          max_slice_vars=max_slice_vars, upgrades=upgrades).start()
+    out = {"pool": [k for k, _ in pool], "p_choices": list(p_choices), "planning_workers": workers,
+           "depth4": {"ordering": ordering4, "slice_below_tw": max_tw4, "rg_repeats": rg_repeats4,
+                      "upgrade": None if upgrade4 is None else
+                      {"ordering": "rgreedy", "slice_below_tw": upgrade4[0], "rg_repeats": upgrade4[1]}},
+           "max_slice_vars": max_slice_vars, "rates": []}
 
     def trace(rate, n, label):
         server.finished.clear()
@@ -494,6 +505,7 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
         st = server.stats()
         st["rate_per_s"] = rate
         st["trace"] = label
+        st["upgraded_plans"] = server.upgraded
         fails = trace_failures(subs)
         st["failed"] = len(fails)
         st["failure_kinds"] = sorted({m.split(":")[0] + ": " + m.split(":")[1][:40] for m in fails})
@@ -513,6 +525,8 @@ def serve_bench(rates, n_requests, pool_size, seed=5, shape=None, max_tw=11, p_c
                 f.result()
             except MemoryError:
                 pass
+        if upgrade4 is not None and p == 4:
+            server.wait_upgrades()
         cold[p] = round(time.perf_counter() - t0, 2)
         print(f"[serve] pool planned for p={p} in {cold[p]} s", file=sys.stderr, flush=True)
     out["pool_planning_s"] = cold
@@ -558,6 +572,15 @@ def main():
     ap.add_argument("--serve-ordering4", default="greedy", choices=("greedy", "rgreedy"),
                     help="with --serve: elimination order of the depth-4 requests (--ordering sets "
                          "depths 1-3)")
+    ap.add_argument("--serve-tw4", type=int, default=16,
+                    help="with --serve: depth-4 lightcones are sliced below this treewidth")
+    ap.add_argument("--serve-rg4", type=int, default=16,
+                    help="with --serve and --serve-ordering4 rgreedy: randomized-greedy samples per order")
+    ap.add_argument("--serve-max-slice-vars", type=int, default=None,
+                    help="with --serve: reject requests whose lightcones need more slice variables")
+    ap.add_argument("--serve-upgrade4", default=None,
+                    help="with --serve: TW,SAMPLES: depth-4 plans are re-made in the background with the "
+                         "randomized-greedy order (SAMPLES samples) sliced below TW, and used once ready")
     ap.add_argument("--mfma-census", action="store_true",
                     help="host only: count the contractions of the Erdos-Renyi and forced-edge plans that "
                          "could use the matrix cores, and print their JSON line")
@@ -581,7 +604,11 @@ def main():
         rates = [float(x) for x in args.serve_rates.split(",")]
         ordering = args.ordering or "greedy"
         res = serve_bench(rates, args.serve_requests, args.serve_pool, ordering=ordering,
-                          ordering4=args.serve_ordering4, cold_rate=20.0, cold_requests=40)
+                          ordering4=args.serve_ordering4, cold_rate=20.0, cold_requests=40,
+                          max_tw4=args.serve_tw4, rg_repeats4=args.serve_rg4,
+                          max_slice_vars=args.serve_max_slice_vars,
+                          upgrade4=None if args.serve_upgrade4 is None else
+                          tuple(int(x) for x in args.serve_upgrade4.split(",")))
         best = max(res["rates"], key=lambda r: r["terms_per_s"])
         print(json.dumps({"metric": "served edge-energy terms/sec and p99 request latency",
                           "value": best["terms_per_s"], "unit": "terms/s", "n_gpus": 1,

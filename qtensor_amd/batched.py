@@ -118,10 +118,11 @@ def _keys_task(args):
 
 
 def _plan_task(args):
-    composer_name, dtype, shape, max_tw, tw_bias, slicer, ordering, tasks = args
+    composer_name, dtype, shape, max_tw, tw_bias, slicer, ordering, rg_repeats, tasks = args
     from . import composer as _composer
     eng = LightconeEngine(composer=getattr(_composer, composer_name), dtype=dtype, shape=shape,
-                          max_tw=max_tw, tw_bias=tw_bias, slicer=slicer, ordering=ordering)
+                          max_tw=max_tw, tw_bias=tw_bias, slicer=slicer, ordering=ordering,
+                          planner="python", rg_repeats=rg_repeats)
     out = []
     for key, n, ce, i, j, p in tasks:
         try:
@@ -141,13 +142,14 @@ class LightconeEngine:
     qtensor/optimisation/Optimizer.py:90-159) or "tree_trim"
     (``TreeTrimSplitter``, Optimizer.py:178-228).  ``ordering``: "greedy"
     (min-degree, the reference's default ``OrderingOptimizer``) or "rgreedy"
-    (best of 16 seeded randomized-greedy samples and the greedy order,
-    ``qtensor/optimisation/RGreedy.py:15-77``), also the order the slicer
-    works from.  Orders change the work, not the values (up to rounding).
+    (best of ``rg_repeats`` seeded randomized-greedy samples, 16 by default,
+    and the greedy order, ``qtensor/optimisation/RGreedy.py:15-77``), also the
+    order the slicer works from.  Orders change the work, not the values (up
+    to rounding).
     """
 
     def __init__(self, composer=QtreeQAOAComposer, device=0, dtype="complex64", shape=None,
-                 max_tw=None, tw_bias=0, slicer="greedy", ordering="greedy", planner="native"):
+                 max_tw=None, tw_bias=0, slicer="greedy", ordering="greedy", planner="native", rg_repeats=16):
         if slicer not in ("greedy", "tree_trim"):
             raise ValueError(f"unknown slicer {slicer!r}")
         if ordering not in ("greedy", "rgreedy"):
@@ -159,6 +161,9 @@ class LightconeEngine:
         # which other composers always use)
         self.planner = planner
         self.ordering = ordering
+        if int(rg_repeats) < 1:
+            raise ValueError("rg_repeats must be at least 1")
+        self.rg_repeats = int(rg_repeats)
         self.composer = composer
         self.device_index = device
         self.dtype = dtype
@@ -173,16 +178,20 @@ class LightconeEngine:
     def _optimizer(self):
         rg = self.ordering == "rgreedy"
         if self.max_tw is None:
-            return RGreedyOrderingOptimizer() if rg else OrderingOptimizer()
-        if self.slicer == "tree_trim":
-            cls = RGreedyTreeTrimSplitter if rg else TreeTrimSplitter
+            opt = RGreedyOrderingOptimizer() if rg else OrderingOptimizer()
         else:
-            cls = RGreedySlicesOptimizer if rg else SlicesOptimizer
-        return cls(max_tw=self.max_tw, tw_bias=self.tw_bias)
+            if self.slicer == "tree_trim":
+                cls = RGreedyTreeTrimSplitter if rg else TreeTrimSplitter
+            else:
+                cls = RGreedySlicesOptimizer if rg else SlicesOptimizer
+            opt = cls(max_tw=self.max_tw, tw_bias=self.tw_bias)
+        if rg:
+            opt.repeats = self.rg_repeats
+        return opt
 
     def _cache_key(self, key, p, shape):
         return (key, p, shape.key(), self.composer.__name__, self.max_tw, self.tw_bias, self.slicer,
-                self.ordering)
+                self.ordering, self.rg_repeats if self.ordering == "rgreedy" else 0)
 
     def _check_fits(self, plan, shape):
         """The plan's LDS data fit the launch shape in this engine's precision."""
@@ -246,7 +255,7 @@ class LightconeEngine:
                     record, p, self.composer, ordering=self.ordering,
                     slicer=self.slicer if self.max_tw is not None else None, max_tw=self.max_tw,
                     tw_bias=self.tw_bias, lanes=shape.item_threads, lds_tensor_limit=lim,
-                    lds_capacity=shape.lds_capacity)
+                    lds_capacity=shape.lds_capacity, rg_repeats=self.rg_repeats)
                 break
             except ValueError as exc:
                 if "LDS capacity" not in str(exc) or lim <= shape.item_threads:
@@ -327,7 +336,7 @@ class LightconeEngine:
         if todo:
             chunk = max(1, -(-len(todo) // (2 * workers)))
             jobs = [(self.composer.__name__, self.dtype, shape, self.max_tw, self.tw_bias, self.slicer,
-                     self.ordering, todo[k:k + chunk])
+                     self.ordering, self.rg_repeats, todo[k:k + chunk])
                     for k in range(0, len(todo), chunk)]
             for part in pool.map(_plan_task, jobs):
                 for key, plan, err in part:

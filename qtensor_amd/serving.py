@@ -136,15 +136,28 @@ class EnergyServer:
     admission model (None: 80% of free HBM at start).  ``engines``: optional
     ``{p: LightconeEngine}`` overriding ``engine`` for some depths.
     ``planning_workers``: process-pool size for planning new graphs.
+    ``max_slice_vars``: a request whose lightcones need more slice variables
+    than this is rejected (``MemoryError("Estimated OOM ...")``, as the
+    reference's slicer stops when it cannot reach its width), which bounds the
+    device time one request can take; None: up to what a work item carries.
+    ``upgrades``: optional ``{p: LightconeEngine}``: a (graph, p) is served
+    first with the plans of its usual engine, which are quick to make, and
+    planned again in the background with the upgrade engine (for example a
+    randomized-greedy order sliced further, slower to plan but faster to run);
+    requests that arrive after that plan is done use it.  Each request runs
+    whole with one of the two plans.
     """
 
     def __init__(self, engine, n_streams=2, max_batch_items=1 << 16, hbm_budget=None, engines=None,
-                 planning_workers=None):
+                 planning_workers=None, max_slice_vars=None, upgrades=None):
         self.engine = engine
         # per-p engines (launch shape and slicing), e.g. wider items and less
         # slicing for p = 4; requests of one batch share one launch shape
         self.engines = dict(engines or {})
         self.planning_workers = planning_workers
+        self.max_slice_vars = max_slice_vars
+        self.upgrades = dict(upgrades or {})
+        self.upgraded = 0
         self.rejected = 0
         self.n_streams = n_streams
         self.max_batch_items = max_batch_items
@@ -175,9 +188,14 @@ class EnergyServer:
         self.slots = [_Slot(stream=torch.cuda.Stream(device=self.device)) for _ in range(self.n_streams)]
         self._slot_free = threading.Semaphore(self.n_streams)
         from concurrent.futures import ThreadPoolExecutor
-        # one planner thread: each workload's classes compile on the native
-        # planner's own threads
-        self._planner = ThreadPoolExecutor(1)
+        # one planner thread per engine (per launch configuration), so that
+        # slow plans of one engine (or plans that end up rejected) do not hold
+        # up the quick plans of another; each workload's classes compile on
+        # the native planner's own threads
+        self._planners = {}
+        for eng in [self.engine] + list(self.engines.values()) + list(self.upgrades.values()):
+            if id(eng) not in self._planners:
+                self._planners[id(eng)] = ThreadPoolExecutor(1)
         self._threads = [threading.Thread(target=self._schedule_loop, daemon=True),
                          threading.Thread(target=self._complete_loop, daemon=True)]
         for t in self._threads:
@@ -196,8 +214,13 @@ class EnergyServer:
         self._pending.put(None)
         for t in self._threads:
             t.join(timeout=60)
-        if getattr(self, "_planner", None) is not None:
-            self._planner.shutdown(wait=True)
+        for pool in getattr(self, "_planners", {}).values():
+            pool.shutdown(wait=True)
+
+    def wait_upgrades(self, timeout=None):
+        """Block until the upgrade plans queued so far are done."""
+        for eng in self.upgrades.values():
+            self._planners[id(eng)].submit(lambda: None).result(timeout=timeout)
 
     def stats(self):
         with self._lock:
@@ -215,18 +238,43 @@ class EnergyServer:
     def engine_for(self, p):
         return self.engines.get(p, self.engine)
 
-    def _plan(self, G, p, k):
-        """Plan one (graph, depth) on the planner thread."""
+    def _plan_with(self, eng, G, p, workers=None):
         try:
-            wl = self.engine_for(p).workload(G, p, workers=self.planning_workers)
+            wl = eng.workload(G, p, workers=self.planning_workers if workers is None else workers)
         except ValueError as exc:
             # e.g. more slice variables than a work item can carry: the
             # reference's planner stops the same way ("Estimated OOM")
             raise MemoryError(f"Estimated OOM: {exc}") from exc
-        wl.shape_key = self.engine_for(p).shape_for(p).key()
+        wl.shape_key = eng.shape_for(p).key()
+        if self.max_slice_vars is not None:
+            need = max((pl.n_sliced for pl in wl.plans.values()), default=0)
+            if need > self.max_slice_vars:
+                raise MemoryError(f"Estimated OOM: a lightcone needs {need} slice variables "
+                                  f"(limit {self.max_slice_vars})")
+        return wl
+
+    def _plan(self, G, p, k):
+        """Plan one (graph, depth) on its engine's planner thread; with an
+        upgrade engine for this depth, then plan it again with that one on its
+        own thread, and serve later requests with the upgraded plans."""
+        wl = self._plan_with(self.engine_for(p), G, p)
         self._workloads[k] = wl
         self._progress.set()
+        up = self.upgrades.get(p)
+        if up is not None:
+            self._planners[id(up)].submit(self._upgrade, up, G, p, k)
         return wl
+
+    def _upgrade(self, eng, G, p, k):
+        # background work: a quarter of the planning threads, so that the
+        # quick first plans of new requests keep the rest of the host
+        workers = max(1, (self.planning_workers or 4) // 4)
+        try:
+            wl = self._plan_with(eng, G, p, workers)
+        except MemoryError:
+            return                      # keep the first plans
+        self._workloads[k] = wl
+        self.upgraded += 1
 
     def _workload(self, req):
         """The request's workload if planned; otherwise start planning it
@@ -238,7 +286,7 @@ class EnergyServer:
             return wl
         fut = self._planning.get(k)
         if fut is None:
-            fut = self._planner.submit(self._plan, req.G, p, k)
+            fut = self._planners[id(self.engine_for(p))].submit(self._plan, req.G, p, k)
             self._planning[k] = fut
         if not fut.done():
             return None

@@ -81,3 +81,33 @@ def test_depth4_served_with_its_own_shape_and_intractable_rejected(gpu):
     finally:
         server.stop()
     assert server.rejected == 1
+
+
+def test_plan_upgrades_and_slice_cap(gpu):
+    """upgrades={p: engine}: a (graph, p) is served first with its usual
+    (greedy) plans, re-planned in the background with the randomized-greedy
+    order sliced further, and later requests use those plans; both give the
+    statevector values.  max_slice_vars rejects a request whose lightcones
+    need more slice variables ("Estimated OOM")."""
+    eng = LightconeEngine(shape=LaunchShape(64, 4096, 2048), max_tw=11)
+    up = LightconeEngine(shape=LaunchShape(64, 4096, 2048), max_tw=6, slicer="tree_trim", ordering="rgreedy",
+                         rg_repeats=4)
+    G = nx.random_regular_graph(3, 16, seed=9)
+    g, b = [0.4, 0.9], [0.3, 0.6]
+    sv = statevector.zz_expectations(G, g, b)
+    server = EnergyServer(eng, n_streams=2, upgrades={2: up}).start()
+    try:
+        C0, zz0 = server.submit(G, g, b, key="G").result(timeout=120)
+        server.wait_upgrades(timeout=120)
+        assert server.upgraded == 1
+        C1, zz1 = server.submit(G, g, b, key="G").result(timeout=120)
+    finally:
+        server.stop()
+    for zz in (zz0, zz1):
+        assert max(abs(z - sv[e]) for z, e in zip(zz, G.edges())) <= 1e-5
+    capped = EnergyServer(up, n_streams=1, max_slice_vars=0).start()
+    try:
+        with pytest.raises(MemoryError, match="Estimated OOM"):
+            capped.submit(G, g, b, key="G").result(timeout=120)
+    finally:
+        capped.stop()
