@@ -673,12 +673,11 @@ struct GatherLane {
 // used); lanes past the table read later records or, past the program, zeros,
 // and skip the op.
 __device__ __forceinline__ void fetch_lane(__amdgpu_buffer_rsrc_t prog, uint32_t st, int32_t r, int lane,
-                                           GatherLane& g) {
-    const uint32_t n = QB_RL(r, 1);
+                                           uint32_t wave_lane0, uint32_t n, uint32_t ni, GatherLane& g) {
     // a wave none of whose lanes is active loads nothing (the op skips it)
-    if (((uint32_t)__builtin_amdgcn_readfirstlane(lane) & ~63u) >= n) return;
+    if (wave_lane0 >= n) return;
     const uint32_t so = (st + QB_RL(r, 2)) * 4u;
-    const uint32_t vo = __umul24((uint32_t)lane, (QB_RL(r, 3) + 1u) * 4u);   // lane < 2^10
+    const uint32_t vo = __umul24((uint32_t)lane, (ni + 1u) * 4u);   // lane < 2^10
     const auto a = __builtin_amdgcn_raw_buffer_load_b128(prog, vo, so, 0);
 #pragma unroll
     for (int j = 0; j < 4; ++j) g.w[j] = (uint32_t)a[j];
@@ -713,11 +712,13 @@ __device__ __forceinline__ void gather_op_n(const GatherLane& g, QB_LDS typename
     lds[g.w[0]] = cadd(p, q);
 }
 
+// n: the op's active lanes, ni: its input count (both decoded from the record
+// when its lane table was fetched)
 template <typename R>
-__device__ __forceinline__ void gather_op(int32_t r, const GatherLane& g, QB_LDS typename Cx<R>::V* lds,
-                                          int lane) {
-    if ((uint32_t)lane >= QB_RL(r, 1)) return;
-    switch (QB_RL(r, 3)) {
+__device__ __forceinline__ void gather_op(uint32_t n, uint32_t ni, const GatherLane& g,
+                                          QB_LDS typename Cx<R>::V* lds, int lane) {
+    if ((uint32_t)lane >= n) return;
+    switch (ni) {
     case 1: gather_op_n<R, 1>(g, lds); break;
     case 2: gather_op_n<R, 2>(g, lds); break;
     case 3: gather_op_n<R, 3>(g, lds); break;
@@ -991,26 +992,36 @@ template <typename R, int ITEM_THREADS, bool GLB>
 __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st0, int32_t n_ops, int32_t r0,
                                         QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
     if (n_ops <= 0) return;
+    const uint32_t wave_lane0 = (uint32_t)__builtin_amdgcn_readfirstlane(lane) & ~63u;
     uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(r0);
     GatherLane g0;   // loaded and read only for gather ops
-    if (w0 & kOpGather) fetch_lane(prog, st0, r0, lane, g0);
+    uint32_t n0 = 0, ni0 = 0;
+    if (w0 & kOpGather) {
+        n0 = QB_RL(r0, 1);
+        ni0 = QB_RL(r0, 3);
+        fetch_lane(prog, st0, r0, lane, wave_lane0, n0, ni0, g0);
+    }
     uint32_t st1 = st0 + (w0 >> 16);
     int32_t r1 = 0;
     if (n_ops > 1) r1 = fetch_rec(prog, st1, lane);
     for (int32_t k = 0; k < n_ops; ++k) {
-        uint32_t w1 = 0;
+        uint32_t w1 = 0, n1 = 0, ni1 = 0;
         GatherLane g1;
         uint32_t st2 = st1;
         int32_t r2 = 0;
         if (k + 1 < n_ops) {
             w1 = (uint32_t)__builtin_amdgcn_readfirstlane(r1);
-            if (w1 & kOpGather) fetch_lane(prog, st1, r1, lane, g1);
+            if (w1 & kOpGather) {
+                n1 = QB_RL(r1, 1);
+                ni1 = QB_RL(r1, 3);
+                fetch_lane(prog, st1, r1, lane, wave_lane0, n1, ni1, g1);
+            }
             st2 = st1 + (w1 >> 16);
             if (k + 2 < n_ops) r2 = fetch_rec(prog, st2, lane);
         }
         op_sync<ITEM_THREADS>(GLB ? w0 : 0u);
         if (w0 & kOpGather)
-            gather_op<R>(r0, g0, lds, lane);
+            gather_op<R>(n0, ni0, g0, lds, lane);
         else if (w0 & kOpReduce)
             reduce_op<R>(r0, w0, lds, lane);
         else if (!GLB || (w0 & (1u << 12)))
@@ -1020,6 +1031,8 @@ __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st
         w0 = w1;
         r0 = r1;
         g0 = g1;
+        n0 = n1;
+        ni0 = ni1;
         st1 = st2;
         r1 = r2;
     }
