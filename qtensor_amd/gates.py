@@ -39,8 +39,8 @@ class Gate:
     parametric = False
 
     def __init__(self, *qubits, **parameters):
-        self._qubits = tuple(int(q) for q in qubits)
-        self._parameters = dict(parameters)
+        self._qubits = tuple(map(int, qubits))
+        self._parameters = parameters          # a fresh dict per call
         self._dagger = False
 
     @property
