@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # KiB fetched x 2.0 (a 1 GiB device copy reads as 524308 KiB) + 31.0 KiB written
 # x 1.0 (the copy's 1 GiB writes read as 1048576 KiB).  Reported only for that
 # configuration: (item threads, slicing threshold, slicer, order, work items).
-EXEC_TRAFFIC_CONFIG = (256, 9, "tree_trim", "rgreedy", 1013)
+EXEC_TRAFFIC_CONFIG = (256, 1, 9, "tree_trim", "rgreedy", 1013)
 EXEC_TRAFFIC_BYTES = int(14297.3 * 1024 * 2.0 + 31.0 * 1024 * 1.0)
 
 
@@ -547,7 +547,7 @@ def main():
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
     ap.add_argument("--shape", default=None,
-                    help="fixed launch shape item_threads,lds_capacity,lds_tensor_limit (skips the autotuner)")
+                    help="fixed launch shape item_threads,lds_capacity,lds_tensor_limit[,group] (skips the autotuner)")
     ap.add_argument("--precision", action="store_true",
                     help="run only the precision study (Erdos-Renyi N=64, p=3) and print its JSON line")
     ap.add_argument("--precision-sets", type=int, default=16)
@@ -643,13 +643,13 @@ def main():
     from qtensor_amd.batched import LaunchShape
     t0 = time.perf_counter()
     if args.shape:
-        it, cap, lim = (int(x) for x in args.shape.split(","))
-        shape, max_tw, slicer, table = LaunchShape(it, cap, lim), args.slice_tw, args.slicer, []
+        dims = [int(x) for x in args.shape.split(",")]
+        shape, max_tw, slicer, table = LaunchShape(*dims), args.slice_tw, args.slicer, []
     else:
         tuner = Autotuner(verbose=True, workers=16, ordering=ordering)
         shape, max_tw, slicer, table = tuner.tune(G, p, gamma, beta)
     tune_s = time.perf_counter() - t0
-    print(f"[bench] tuned: {shape.key()} max_tw={max_tw} {slicer} {ordering} ({tune_s:.1f} s)",
+    print(f"[bench] tuned: {shape.key()} group={shape.group} max_tw={max_tw} {slicer} {ordering} ({tune_s:.1f} s)",
           file=sys.stderr, flush=True)
 
     # cold: one evaluation from an empty plan cache (planning, upload, launch, terms)
@@ -733,7 +733,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"metric": "executor kernel time", "value": round(kern_ms, 5), "unit": "ms",
                           "higher_is_better": False, "n_gpus": 1, "dtype": "complex64", "data": "synthetic",
-                          "config": {"workload": desc, "item_threads": shape.item_threads,
+                          "config": {"workload": desc, "item_threads": shape.item_threads, "group": shape.group,
                                      "lds_capacity": shape.lds_capacity,
                                      "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw, "slicer": slicer,
                                      "ordering": ordering, "work_items": batch.n_items,
@@ -744,7 +744,9 @@ def main():
 
     # error: every term against the committed complex128 oracle values and
     # against a complex128 run of the same kernels
-    eng128 = LightconeEngine(dtype="complex128", shape=shape)
+    # (complex128 items run one per block)
+    eng128 = LightconeEngine(dtype="complex128",
+                             shape=LaunchShape(shape.item_threads, shape.lds_capacity, shape.lds_tensor_limit))
     b128 = eng128.batch([(eng128.workload(G, p), gamma, beta)])
     b128.run()
     ref128 = np.array(b128.terms()[0])
@@ -757,7 +759,7 @@ def main():
     energy = (n_terms - math.fsum(terms64.real)) / 2
 
     traffic = EXEC_TRAFFIC_BYTES if (args.workload == "c2" and batch.scratch_elems == 0 and
-                                     (shape.item_threads, max_tw, slicer, ordering, batch.n_items)
+                                     (shape.item_threads, shape.group, max_tw, slicer, ordering, batch.n_items)
                                      == EXEC_TRAFFIC_CONFIG) else None
     # which resource limits the executor: HBM if its measured traffic moves at
     # a large fraction of the peak, otherwise instruction issue and the latency
@@ -776,7 +778,7 @@ def main():
         "config": {"workload": desc, "p": p, "edges": n_terms, "composer": "QtreeQAOAComposer",
                    "ordering": ordering, "form": "batched persistent executor, stream launches",
                    "max_treewidth": wl_alg.max_treewidth(), "distinct_plans": len(set(map(id, wl.plans.values()))),
-                   "item_threads": shape.item_threads, "lds_capacity": shape.lds_capacity,
+                   "item_threads": shape.item_threads, "group": shape.group, "lds_capacity": shape.lds_capacity,
                    "lds_tensor_limit": shape.lds_tensor_limit, "slice_below_tw": max_tw, "slicer": slicer,
                    "work_items": batch.n_items},
         "autotune": {"seconds": round(tune_s, 1), "candidates": len(table)},
@@ -788,7 +790,8 @@ def main():
         "graph_replay_ms_per_step": None if graph_ms is None else round(graph_ms, 4),
         "roofline": {"bound": "hbm", "limiter": limiter, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "exec_kernel<float,%d>" % batch.shape.item_threads,
+                     "kernel": "exec_kernel<float,%d,%s,%d>" % (batch.shape.item_threads,
+                                                                "true" if batch.scratch_elems else "false", batch.group),
                      "kernel_ms": round(kern_ms, 5), "step_model_bytes_per_launch": bytes_per_launch,
                      "executed_step_bytes_per_launch": batch.bytes_step_model,
                      "measured_copy_GBs": None if copy_gbs is None else round(copy_gbs, 1),

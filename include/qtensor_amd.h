@@ -132,12 +132,21 @@ typedef struct qb_exec_desc {
                                     size (reads past it return 0); 0 = unbounded */
     double* item_out;            /* device: 2 * n_items doubles */
     uint32_t* queue;             /* device: one uint32 (reset by the call when the
-                                    items outnumber the resident blocks) */
+                                    groups outnumber the resident blocks) */
+    int32_t group;               /* items per block, run in lockstep (0 or 1: one;
+                                    2: complex64 with item_threads 64, 128 or 256).
+                                    Items come in groups of `group` consecutive
+                                    items sharing one plan offset; n_items is a
+                                    multiple of group.  A group keeps its items'
+                                    LDS data (and scratch) interleaved, so each
+                                    block reserves group * lds_elems elements and
+                                    group scratch slots. */
 } qb_exec_desc;
 
 /* Number of concurrently resident work items (= scratch slots needed) for a
- * launch shape; qb_execute uses at most this many slots. */
-int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n_slots);
+ * launch shape with `group` items per block (0 or 1: one); qb_execute uses at
+ * most this many slots. */
+int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t group, int32_t* n_slots);
 
 /* qb_execute checks the descriptor's sizes, but it reads the items and the
  * program through raw device pointers and does NOT validate their contents:
@@ -168,7 +177,9 @@ int qb_execute_noalloc(const qb_exec_desc* desc, void* stream);
 /*
  * Host check of a batched program and its items (no device work; usable
  * without qb_init): every item's plan offset, angle offset and output slot
- * (slots must be a permutation of 0..n_items-1), and for every plan its
+ * (slots must be a permutation of 0..n_items-1), with `group` items per
+ * block (0 or 1: one) that every group's items share a plan offset and that
+ * n_items is a multiple of group, and for every plan its
  * header, gate table (device-built kinds only, inside the LDS data, angle
  * indices inside the item's angle vector), every op record and lane table
  * (inside the program, every LDS element below the plan's LDS need, which
@@ -179,7 +190,8 @@ int qb_execute_noalloc(const qb_exec_desc* desc, void* stream);
  * (ProcessingFrameworks.py:74,137).
  */
 int qb_check_program(const int32_t* program, int64_t program_words, const int32_t* items, int32_t n_items,
-                     int32_t n_params, int32_t item_threads, int32_t lds_elems, int64_t scratch_elems);
+                     int32_t n_params, int32_t item_threads, int32_t lds_elems, int64_t scratch_elems,
+                     int32_t group);
 
 /*
  * Wide plan: one compiled lightcone plan whose tensors (gates included) all
@@ -243,6 +255,9 @@ typedef struct qb_plan_desc {
     int32_t lds_elems;           /* LDS elements per item (>= every plan's need) */
     int32_t dtype;               /* QB_C64 or QB_C128 */
     int64_t scratch_elems;       /* global scratch elements per resident item */
+    int32_t group;               /* items per block, as in qb_exec_desc; the
+                                    segments may end before n_items (padding items
+                                    that complete a group and belong to no term) */
 } qb_plan_desc;
 
 int qb_plan_create(const qb_plan_desc* desc, qb_plan** plan);

@@ -43,7 +43,7 @@ class ExecDesc(ctypes.Structure):
                 ("params", ctypes.c_void_p), ("const_pool", ctypes.c_void_p),
                 ("scratch", ctypes.c_void_p), ("scratch_elems", ctypes.c_int64),
                 ("n_slots", ctypes.c_int32), ("program_words", ctypes.c_int32),
-                ("item_out", ctypes.c_void_p), ("queue", ctypes.c_void_p)]
+                ("item_out", ctypes.c_void_p), ("queue", ctypes.c_void_p), ("group", ctypes.c_int32)]
 
 
 class WideDesc(ctypes.Structure):
@@ -58,7 +58,8 @@ class PlanDesc(ctypes.Structure):
     _fields_ = [("program", ctypes.c_void_p), ("program_words", ctypes.c_int64),
                 ("items", ctypes.c_void_p), ("n_items", ctypes.c_int32), ("n_terms", ctypes.c_int32),
                 ("seg", ctypes.c_void_p), ("n_params", ctypes.c_int32), ("item_threads", ctypes.c_int32),
-                ("lds_elems", ctypes.c_int32), ("dtype", ctypes.c_int32), ("scratch_elems", ctypes.c_int64)]
+                ("lds_elems", ctypes.c_int32), ("dtype", ctypes.c_int32), ("scratch_elems", ctypes.c_int64),
+                ("group", ctypes.c_int32)]
 
 
 class QBError(RuntimeError):
@@ -92,7 +93,7 @@ def load():
         lib.qb_last_error.restype = ctypes.c_char_p
         lib.qb_step.argtypes = [ctypes.POINTER(StepDesc), ctypes.POINTER(ctypes.c_void_p),
                                 ctypes.c_void_p, ctypes.c_void_p]
-        lib.qb_exec_slots.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+        lib.qb_exec_slots.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_int32)]
         lib.qb_execute.argtypes = [ctypes.POINTER(ExecDesc), ctypes.c_void_p]
         lib.qb_execute_noalloc.argtypes = [ctypes.POINTER(ExecDesc), ctypes.c_void_p]
@@ -108,16 +109,17 @@ def load():
         lib.qb_place_intervals.argtypes = [ctypes.c_int32, i64p, i64p, i64p, i64p, ctypes.c_int64, i64p, i64p]
         lib.qb_wide_execute.argtypes = [ctypes.POINTER(WideDesc), ctypes.c_void_p]
         lib.qb_check_program.argtypes = [i32p, ctypes.c_int64, i32p, ctypes.c_int32, ctypes.c_int32,
-                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int64]
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]
         lib.qb_reduce_terms.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib = lib
         return lib
 
 
-def check_program(program, items, n_params, item_threads, lds_elems, scratch_elems):
+def check_program(program, items, n_params, item_threads, lds_elems, scratch_elems, group=1):
     """Host check of a batched program and its items (``qb_check_program``)
-    before ``qb_execute`` reads them on the device; raises ``ValueError``."""
+    before ``qb_execute`` reads them on the device; raises ``ValueError``.
+    ``group``: items per executor block (consecutive items sharing a plan)."""
     import numpy as np
     lib = load()
     program = np.ascontiguousarray(program, dtype=np.int32)
@@ -126,7 +128,7 @@ def check_program(program, items, n_params, item_threads, lds_elems, scratch_ele
     check(lib.qb_check_program(program.ctypes.data_as(i32p), program.size,
                                items.ctypes.data_as(i32p) if items.size else ctypes.cast(program.ctypes.data, i32p),
                                items.size // QB_ITEM_WORDS, int(n_params), int(item_threads), int(lds_elems),
-                               int(scratch_elems)))
+                               int(scratch_elems), int(group)))
 
 
 def check(code):

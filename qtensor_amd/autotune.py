@@ -35,8 +35,11 @@ def _signature(G, p, composer, dtype, ordering):
 
 
 BASE_SHAPE = LaunchShape(128, 4096, 2048)
+# the last two run two items of one plan per block (more items resident, so
+# finer slicing and shorter op chains fit one round)
 OTHER_SHAPES = (LaunchShape(128, 2048, 1024), LaunchShape(256, 4096, 2048), LaunchShape(256, 2048, 1024),
-                LaunchShape(64, 4096, 2048), LaunchShape(256, 8192, 4096))
+                LaunchShape(64, 4096, 2048), LaunchShape(256, 8192, 4096), LaunchShape(256, 4096, 2048, group=2),
+                LaunchShape(128, 4096, 2048, group=2))
 # slice-variable rules (LightconeEngine's ``slicer``)
 SLICERS = ("greedy", "tree_trim")
 # (threshold, slicer) pairs of stage 1 timed again under every other shape:
@@ -128,7 +131,8 @@ class Autotuner:
             table.append({"shape": asdict(shape), "max_tw": mtw, "slicer": sl, "eval_ms": ms, "plan_s": plan_s})
             timed.append((ms, (mtw, sl)))
             if self.verbose:
-                print(f"[autotune] {shape.key()} max_tw={mtw} {sl}: {ms:.4f} ms", file=sys.stderr, flush=True)
+                print(f"[autotune] {shape.key()} group={shape.group} max_tw={mtw} {sl}: {ms:.4f} ms",
+                      file=sys.stderr, flush=True)
             if best is None or ms < best[0]:
                 best = (ms, shape, mtw, sl)
 

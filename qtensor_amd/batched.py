@@ -39,8 +39,11 @@ class LaunchShape:
     item_threads: int = 64
     lds_capacity: int = 4096          # elements per item (complex64: 32 KiB)
     lds_tensor_limit: int = 2048      # largest intermediate kept in LDS
+    group: int = 1                    # items of one plan per executor block, in lockstep
 
     def key(self):
+        """What the compiled plans depend on (the group size only changes the
+        launch)."""
         return (self.item_threads, self.lds_capacity, self.lds_tensor_limit)
 
 
@@ -383,6 +386,42 @@ def order_items(items, costs, band=None):
     return np.ascontiguousarray(items[order])
 
 
+def group_items(items, costs, group, band=None):
+    """Work items in launch order for blocks of ``group`` items each: items of
+    one plan (word 0) form groups of ``group`` consecutive items, the last group
+    of a plan padded with copies of its last item whose output slots follow the
+    real items' (they belong to no term).  Groups are ordered longest first and
+    serpentine over ``band`` groups, as ``order_items`` orders single items.
+    Returns the items (slot in word 3) and the number of padding items."""
+    if group <= 1:
+        return order_items(items, costs, band), 0
+    items = np.array(items, dtype=np.int32).reshape(-1, 4)
+    items[:, 3] = np.arange(items.shape[0], dtype=np.int32)
+    costs = np.asarray(costs, dtype=np.int64)
+    by_plan = {}
+    for k in range(items.shape[0]):
+        by_plan.setdefault(int(items[k, 0]), []).append(k)
+    groups, gcost, pad = [], [], []
+    next_slot = items.shape[0]
+    for members in by_plan.values():
+        for a in range(0, len(members), group):
+            chunk = [items[k] for k in members[a:a + group]]
+            while len(chunk) < group:
+                extra = chunk[-1].copy()
+                extra[3] = next_slot
+                next_slot += 1
+                chunk.append(extra)
+                pad.append(1)
+            groups.append(np.stack(chunk))
+            gcost.append(int(costs[members[a]]))
+    order = np.argsort(-np.asarray(gcost, dtype=np.int64), kind="stable")
+    if band:
+        for k in range(band, len(order), 2 * band):
+            order[k:k + band] = order[k:k + band][::-1].copy()
+    out = np.concatenate([groups[g] for g in order]).astype(np.int32)
+    return np.ascontiguousarray(out), len(pad)
+
+
 class Batch:
     """Device buffers and launch of one set of (workload, angles) requests."""
 
@@ -436,21 +475,24 @@ class Batch:
                     order.append(plan.n_ops)
                 seg.append(len(items))
             self.term_slices.append((start, len(seg) - 1))
-        self.n_items = len(items)
         self.n_terms = len(seg) - 1
         # no slicing anywhere: item slot s holds term s and the reduction needs
         # no segment table (one launch)
-        self.one_item_per_term = self.n_items == self.n_terms
+        self.one_item_per_term = len(items) == self.n_terms
         self.lds_elems = max(lds_need, 2)
         self.scratch_elems = int(glb_need)
+        self.group = max(1, int(self.shape.group))
         slots = ctypes.c_int32(0)
-        _lib.check(self.lib.qb_exec_slots(self.shape.item_threads, self.lds_elems, self.qb_dtype,
+        _lib.check(self.lib.qb_exec_slots(self.shape.item_threads, self.lds_elems, self.qb_dtype, self.group,
                                           ctypes.byref(slots)))
         self.n_slots = slots.value
-        items_np = order_items(items, order, band=_cu_count(self.device))
+        band = _cu_count(self.device)
+        items_np, self.n_padding = group_items(items, order, self.group, band=band)
+        # every item the launch runs, padding included
+        self.n_items = items_np.shape[0]
         # qb_execute trusts the program and items: check them on the host first
         _lib.check_program(program, items_np, len(params), self.shape.item_threads, self.lds_elems,
-                           self.scratch_elems)
+                           self.scratch_elems, self.group)
         self._host = (program, items_np, np.asarray(seg, dtype=np.int32), len(params))
         self.program = torch.from_numpy(program).to(dev)
         self.items = torch.from_numpy(np.ascontiguousarray(items_np)).to(dev)
@@ -486,6 +528,7 @@ class Batch:
         d.program_words = self.program.numel()
         d.item_out = self.item_out.data_ptr()
         d.queue = self.queue.data_ptr()
+        d.group = self.group
         return d
 
     def set_params(self, requests_params):
@@ -569,6 +612,7 @@ class PlanHandle:
         d.lds_elems = batch.lds_elems
         d.dtype = batch.qb_dtype
         d.scratch_elems = batch.scratch_elems
+        d.group = batch.group
         h = ctypes.c_void_p()
         _lib.check(self.lib.qb_plan_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h

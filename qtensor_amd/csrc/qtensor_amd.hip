@@ -86,6 +86,55 @@ __device__ __forceinline__ V cadd(V a, V b) {
     return r;
 }
 
+// A group of M work items of one plan runs in lockstep in one executor block:
+// their LDS (and scratch) elements are interleaved, element e of item i at
+// e * M + i, so one group element holds element e of every item of the group
+// and the group shares one pass over the plan's op records and lane tables.
+// Groups of two complex64 items: (re0, im0, re1, im1), one 16-byte access.
+typedef float cf32x2 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ cf32x2 cmul(cf32x2 a, cf32x2 b) {
+    cf32x2 r;
+    r[0] = a[0] * b[0] - a[1] * b[1];
+    r[1] = a[0] * b[1] + a[1] * b[0];
+    r[2] = a[2] * b[2] - a[3] * b[3];
+    r[3] = a[2] * b[3] + a[3] * b[2];
+    return r;
+}
+
+__device__ __forceinline__ cf32x2 cadd(cf32x2 a, cf32x2 b) {
+    cf32x2 r;
+    r[0] = a[0] + b[0];
+    r[1] = a[1] + b[1];
+    r[2] = a[2] + b[2];
+    r[3] = a[3] + b[3];
+    return r;
+}
+
+// the executor's element type for items of precision R run in groups of M
+template <typename R, int M> struct PkOf;
+template <typename R> struct PkOf<R, 1> { using T = typename Cx<R>::V; };
+template <> struct PkOf<float, 2> { using T = cf32x2; };
+
+template <typename V>
+__device__ __forceinline__ V pk_fill(V x, V*) { return x; }
+__device__ __forceinline__ cf32x2 pk_fill(cf32 x, cf32x2*) { return cf32x2{x.x, x.y, x.x, x.y}; }
+
+// s + (s of the lane whose index differs in bit b), within a wave
+template <typename V>
+__device__ __forceinline__ V pk_xor_add(V s, int b) {
+    s.x += __shfl_xor(s.x, b, 64);
+    s.y += __shfl_xor(s.y, b, 64);
+    return s;
+}
+__device__ __forceinline__ cf32x2 pk_xor_add(cf32x2 s, int b) {
+    s[0] += __shfl_xor(s[0], b, 64);
+    s[1] += __shfl_xor(s[1], b, 64);
+    s[2] += __shfl_xor(s[2], b, 64);
+    s[3] += __shfl_xor(s[3], b, 64);
+    return s;
+}
+
 __device__ __forceinline__ uint32_t pext_loop(uint32_t x, uint32_t m) {
     uint32_t r = 0, bit = 1;
     while (m) {
@@ -637,6 +686,21 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t ele
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t elem, cf64 v) {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, elem * 16u, 0, 0);
 }
+// a group's element (two complex64 values): one 16-byte access
+__device__ __forceinline__ cf32x2 buf_load(__amdgpu_buffer_rsrc_t r, uint32_t elem, cf32x2*) {
+    return __builtin_bit_cast(cf32x2, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 16u, 0, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t elem, cf32x2 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, elem * 16u, 0, 0);
+}
+
+// Item i's view of a group's interleaved LDS data: element e at e * M + i
+// (the destination of its gate tensors).
+template <typename V, int M>
+struct LdsLane {
+    QB_LDS V* p;
+    __device__ __forceinline__ QB_LDS V& operator[](int32_t e) const { return p[(uint32_t)e * (uint32_t)M]; }
+};
 
 // Offset of union element o in an input whose mask has at most two runs: one
 // bit-field extract and one shift per run (an absent run has width 0 and adds 0).
@@ -686,16 +750,15 @@ __device__ __forceinline__ void fetch_lane(__amdgpu_buffer_rsrc_t prog, uint32_t
 
 // One gather op: lane l computes its table entry's element,
 // x_0 ... x_{NI-1} + y_0 ... y_{NI-1} (y_0 = 0 for a step without a sum).
-template <typename R, int NI>
-__device__ __forceinline__ void gather_op_n(const GatherLane& g, QB_LDS typename Cx<R>::V* lds) {
-    using V = typename Cx<R>::V;
-    V x[NI], y[NI];
+template <typename P, int NI>
+__device__ __forceinline__ void gather_op_n(const GatherLane& g, QB_LDS P* lds) {
+    P x[NI], y[NI];
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
         x[j] = lds[g.w[1 + j] & 0xffffu];
         y[j] = lds[g.w[1 + j] >> 16];
     }
-    V p, q;
+    P p, q;
     if (NI == 1) {
         p = x[0];
         q = y[0];
@@ -714,15 +777,14 @@ __device__ __forceinline__ void gather_op_n(const GatherLane& g, QB_LDS typename
 
 // n: the op's active lanes, ni: its input count (both decoded from the record
 // when its lane table was fetched)
-template <typename R>
-__device__ __forceinline__ void gather_op(uint32_t n, uint32_t ni, const GatherLane& g,
-                                          QB_LDS typename Cx<R>::V* lds, int lane) {
+template <typename P>
+__device__ __forceinline__ void gather_op(uint32_t n, uint32_t ni, const GatherLane& g, QB_LDS P* lds, int lane) {
     if ((uint32_t)lane >= n) return;
     switch (ni) {
-    case 1: gather_op_n<R, 1>(g, lds); break;
-    case 2: gather_op_n<R, 2>(g, lds); break;
-    case 3: gather_op_n<R, 3>(g, lds); break;
-    default: gather_op_n<R, 4>(g, lds); break;
+    case 1: gather_op_n<P, 1>(g, lds); break;
+    case 2: gather_op_n<P, 2>(g, lds); break;
+    case 3: gather_op_n<P, 3>(g, lds); break;
+    default: gather_op_n<P, 4>(g, lds); break;
     }
 }
 
@@ -735,9 +797,8 @@ __device__ __forceinline__ void gather_op(uint32_t n, uint32_t ni, const GatherL
 // One reduction op, run by the item's first wave (n_out = 2^rank_out <= 64):
 // lane l takes output o = l mod n_out and every (64 / n_out)-th term from
 // l / n_out on; the lanes sharing an output then combine in a butterfly.
-template <typename R>
-__device__ __forceinline__ void reduce_op(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds, int lane) {
-    using V = typename Cx<R>::V;
+template <typename P>
+__device__ __forceinline__ void reduce_op(int32_t r, uint32_t w0, QB_LDS P* lds, int lane) {
     if (lane >= 64) return;
     const uint32_t ro = (w0 >> 4) & 31u;
     const uint32_t terms = 1u << (QB_RL(r, 1) & 31u);
@@ -751,43 +812,39 @@ __device__ __forceinline__ void reduce_op(int32_t r, uint32_t w0, QB_LDS typenam
     // or one when terms < step, where lanes past the terms add element
     // LOC_ZERO (0 + 0i) -- a uniform loop with no per-lane exit
     const uint32_t trips = terms >= step ? terms / step : 1u;
-    V s = V{0, 0};
+    P s = {};
 #pragma unroll 4
     for (uint32_t t = 0; t < trips; ++t) {
         const uint32_t x = g + t * step;
         s = cadd(s, lds[x < terms ? in_loc + o + x * n_out : kLocZero]);
     }
-    for (uint32_t b = n_out; b < 64u; b <<= 1) {
-        s.x += __shfl_xor(s.x, (int)b, 64);
-        s.y += __shfl_xor(s.y, (int)b, 64);
-    }
+    for (uint32_t b = n_out; b < 64u; b <<= 1) s = pk_xor_add(s, (int)b);
     if (g == 0) lds[out_loc + o] = s;
 }
 
 // Output blocks [base, base + U*ITEM_THREADS) of an all-LDS step with NI inputs,
 // for every base below n_out in steps of U*ITEM_THREADS (n_out is a multiple of
 // that when U > 1).
-template <typename R, int ITEM_THREADS, int U, int NI>
+template <typename P, int ITEM_THREADS, int U, int NI>
 __device__ __forceinline__ void step_lds_pass(uint32_t n_out, uint32_t n_sum, const uint32_t* loc,
                                               const uint32_t* top, const uint32_t* mask, const uint32_t* runs,
-                                              uint32_t out_loc, QB_LDS typename Cx<R>::V* lds, int lane) {
-    using V = typename Cx<R>::V;
+                                              uint32_t out_loc, QB_LDS P* lds, int lane) {
     uint32_t la[NI];
 #pragma unroll
     for (int j = 0; j < NI; ++j) la[j] = loc[j] + in_offset((uint32_t)lane, mask[j], runs[j]);
     for (uint32_t base = 0; base < n_out; base += U * ITEM_THREADS) {
-        V p[U];
+        P p[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t hb = base + u * ITEM_THREADS;
-            V x[NI], y[NI];
+            P x[NI], y[NI];
 #pragma unroll
             for (int j = 0; j < NI; ++j) {
                 const uint32_t a = la[j] + in_offset(hb, mask[j], runs[j]);
                 x[j] = lds[a];
                 if (n_sum == 2) y[j] = lds[a + top[j]];
             }
-            V q;
+            P q;
             if (NI == 1) {
                 p[u] = x[0];
                 if (n_sum == 2) q = y[0];
@@ -811,48 +868,48 @@ __device__ __forceinline__ void step_lds_pass(uint32_t n_out, uint32_t n_sum, co
     }
 }
 
-template <typename R, int ITEM_THREADS, int NI>
+template <typename P, int ITEM_THREADS, int NI>
 __device__ __forceinline__ void step_lds_n(uint32_t n_out, uint32_t n_sum, const uint32_t* loc,
                                            const uint32_t* top, const uint32_t* mask, const uint32_t* runs,
-                                           uint32_t out_loc, QB_LDS typename Cx<R>::V* lds, int lane) {
-    if (NI < 4 && n_out >= 4u * ITEM_THREADS)
-        step_lds_pass<R, ITEM_THREADS, 4, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
+                                           uint32_t out_loc, QB_LDS P* lds, int lane) {
+    // a group of items unrolls less (its values are wider)
+    constexpr bool wide = sizeof(P) > 16;
+    if (NI < 4 && !wide && n_out >= 4u * ITEM_THREADS)
+        step_lds_pass<P, ITEM_THREADS, 4, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
     else if (n_out >= 2u * ITEM_THREADS)
-        step_lds_pass<R, ITEM_THREADS, 2, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
+        step_lds_pass<P, ITEM_THREADS, 2, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
     else if (((uint32_t)__builtin_amdgcn_readfirstlane(lane) & ~63u) < n_out)   // waves past the outputs idle
-        step_lds_pass<R, ITEM_THREADS, 1, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
+        step_lds_pass<P, ITEM_THREADS, 1, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane);
 }
 
 // Output blocks of a step whose inputs or output live in global scratch
 // (accessed through the slot's buffer resource `srd`).
-template <typename R, int ITEM_THREADS, int U, int NI>
+template <typename P, int ITEM_THREADS, int U, int NI>
 __device__ __forceinline__ void step_mixed_pass(uint32_t n_out, uint32_t n_sum,
                                                 const uint32_t* loc, const uint32_t* top, const uint32_t* mask,
                                                 const uint32_t* runs, uint32_t out_loc,
-                                                QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd,
-                                                int lane) {
-    using V = typename Cx<R>::V;
+                                                QB_LDS P* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
     uint32_t la[NI];
 #pragma unroll
     for (int j = 0; j < NI; ++j) la[j] = (loc[j] & ~kGlobal) + in_offset((uint32_t)lane, mask[j], runs[j]);
     for (uint32_t base = 0; base < n_out; base += U * ITEM_THREADS) {
-        V p[U];
+        P p[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t hb = base + u * ITEM_THREADS;
-            V x[NI], y[NI];
+            P x[NI], y[NI];
 #pragma unroll
             for (int j = 0; j < NI; ++j) {
                 const uint32_t a = la[j] + in_offset(hb, mask[j], runs[j]);
                 if (loc[j] & kGlobal) {
-                    x[j] = buf_load(srd, a, (V*)nullptr);
-                    if (n_sum == 2) y[j] = buf_load(srd, a + top[j], (V*)nullptr);
+                    x[j] = buf_load(srd, a, (P*)nullptr);
+                    if (n_sum == 2) y[j] = buf_load(srd, a + top[j], (P*)nullptr);
                 } else {
                     x[j] = lds[a];
                     if (n_sum == 2) y[j] = lds[a + top[j]];
                 }
             }
-            V q;
+            P q;
             if (NI == 1) {
                 p[u] = x[0];
                 if (n_sum == 2) q = y[0];
@@ -881,24 +938,24 @@ __device__ __forceinline__ void step_mixed_pass(uint32_t n_out, uint32_t n_sum,
     }
 }
 
-template <typename R, int ITEM_THREADS, int NI>
+template <typename P, int ITEM_THREADS, int NI>
 __device__ __forceinline__ void step_mixed_n(uint32_t n_out, uint32_t n_sum, const uint32_t* loc,
                                              const uint32_t* top, const uint32_t* mask, const uint32_t* runs,
-                                             uint32_t out_loc, QB_LDS typename Cx<R>::V* lds,
-                                             __amdgpu_buffer_rsrc_t srd, int lane) {
-    if (n_out >= 4u * ITEM_THREADS)
-        step_mixed_pass<R, ITEM_THREADS, 4, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
+                                             uint32_t out_loc, QB_LDS P* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
+    constexpr bool wide = sizeof(P) > 16;
+    if (!wide && n_out >= 4u * ITEM_THREADS)
+        step_mixed_pass<P, ITEM_THREADS, 4, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
     else
-        step_mixed_pass<R, ITEM_THREADS, 1, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
+        step_mixed_pass<P, ITEM_THREADS, 1, NI>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane);
 }
 
 // A step op with global scratch: every input is uniformly in LDS or in the
 // item's scratch slot; reads of a pass are issued before its products.  Only
 // the executor variant for batches with global scratch contains it (inline: a
 // call would save and restore the callee-saved registers it uses every time).
-template <typename R, int ITEM_THREADS>
-__device__ __forceinline__ void step_mixed(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds,
-                                                     __amdgpu_buffer_rsrc_t srd, int lane) {
+template <typename P, int ITEM_THREADS>
+__device__ __forceinline__ void step_mixed(int32_t r, uint32_t w0, QB_LDS P* lds, __amdgpu_buffer_rsrc_t srd,
+                                           int lane) {
     const uint32_t out_loc = QB_RL(r, 2);
     const uint32_t n_in = w0 & 15u;
     const uint32_t n_out = 1u << ((w0 >> 4) & 31u);
@@ -916,19 +973,18 @@ __device__ __forceinline__ void step_mixed(int32_t r, uint32_t w0, QB_LDS typena
         }
     }
     switch (n_in) {
-    case 1: step_mixed_n<R, ITEM_THREADS, 1>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
-    case 2: step_mixed_n<R, ITEM_THREADS, 2>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
-    case 3: step_mixed_n<R, ITEM_THREADS, 3>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
-    default: step_mixed_n<R, ITEM_THREADS, 4>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
+    case 1: step_mixed_n<P, ITEM_THREADS, 1>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
+    case 2: step_mixed_n<P, ITEM_THREADS, 2>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
+    case 3: step_mixed_n<P, ITEM_THREADS, 3>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
+    default: step_mixed_n<P, ITEM_THREADS, 4>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, srd, lane); break;
     }
 }
 
 // One step op: all lanes of the item cover the step's output elements.  Lane l
 // of each wave holds word l of the record; fields reach scalar registers by
 // readlane.
-template <typename R, int ITEM_THREADS>
-__device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename Cx<R>::V* lds, int lane) {
-    using V = typename Cx<R>::V;
+template <typename P, int ITEM_THREADS>
+__device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS P* lds, int lane) {
     const uint32_t out_loc = QB_RL(r, 2);
     const uint32_t n_in = w0 & 15u;
     const uint32_t rank_out = (w0 >> 4) & 31u;
@@ -950,10 +1006,10 @@ __device__ __forceinline__ void step_op(int32_t r, uint32_t w0, QB_LDS typename 
     // of a pass issued before its first product and its stores (outputs never
     // alias inputs); specialised by input count
     switch (n_in) {
-    case 1: step_lds_n<R, ITEM_THREADS, 1>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
-    case 2: step_lds_n<R, ITEM_THREADS, 2>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
-    case 3: step_lds_n<R, ITEM_THREADS, 3>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
-    default: step_lds_n<R, ITEM_THREADS, 4>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+    case 1: step_lds_n<P, ITEM_THREADS, 1>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+    case 2: step_lds_n<P, ITEM_THREADS, 2>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+    case 3: step_lds_n<P, ITEM_THREADS, 3>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
+    default: step_lds_n<P, ITEM_THREADS, 4>(n_out, n_sum, loc, top, mask, runs, out_loc, lds, lane); break;
     }
 }
 
@@ -988,9 +1044,9 @@ __device__ __forceinline__ void op_sync(uint32_t w0) {
 // op bodies keeps the kernel inside the instruction cache.  GLB = false is the
 // variant for batches without global scratch (every op all-LDS): no
 // global-scratch step and no global fences.
-template <typename R, int ITEM_THREADS, bool GLB>
+template <typename P, int ITEM_THREADS, bool GLB>
 __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st0, int32_t n_ops, int32_t r0,
-                                        QB_LDS typename Cx<R>::V* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
+                                        QB_LDS P* lds, __amdgpu_buffer_rsrc_t srd, int lane) {
     if (n_ops <= 0) return;
     const uint32_t wave_lane0 = (uint32_t)__builtin_amdgcn_readfirstlane(lane) & ~63u;
     uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane(r0);
@@ -1021,13 +1077,13 @@ __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st
         }
         op_sync<ITEM_THREADS>(GLB ? w0 : 0u);
         if (w0 & kOpGather)
-            gather_op<R>(n0, ni0, g0, lds, lane);
+            gather_op<P>(n0, ni0, g0, lds, lane);
         else if (w0 & kOpReduce)
-            reduce_op<R>(r0, w0, lds, lane);
+            reduce_op<P>(r0, w0, lds, lane);
         else if (!GLB || (w0 & (1u << 12)))
-            step_op<R, ITEM_THREADS>(r0, w0, lds, lane);
+            step_op<P, ITEM_THREADS>(r0, w0, lds, lane);
         else
-            step_mixed<R, ITEM_THREADS>(r0, w0, lds, srd, lane);
+            step_mixed<P, ITEM_THREADS>(r0, w0, lds, srd, lane);
         w0 = w1;
         r0 = r1;
         g0 = g1;
@@ -1041,95 +1097,130 @@ __device__ __forceinline__ void run_ops(__amdgpu_buffer_rsrc_t prog, uint32_t st
 // complex64 items keep at most 96 VGPRs, so five waves fit on each SIMD (20
 // per CU): as many items per CU as the LDS of the usual plans allows (for
 // example ten 128-thread items of 16 KiB, or five 256-thread items of 26 KiB).
-template <typename R>
-constexpr int kExecWavesPerEU = sizeof(R) == 4 ? 5 : 1;
+// A group of items holds wider values and needs more LDS per block: 128 VGPRs
+// (four waves per SIMD).
+template <typename R, int M>
+constexpr int kExecWavesPerEU = sizeof(R) == 4 ? (M == 1 ? 5 : 4) : 1;
 
 // LDS of an executor block, all dynamic so that the data region starts at
 // address 0 (an element's byte address is then its index times the element
-// size, one shift): lds_elems data elements, then each wave's copy of the
-// item's first kParamRegs angles, then the queue ticket.
-__host__ __device__ constexpr int exec_aux_bytes(int item_threads) {
-    return ((item_threads + 63) / 64) * kParamRegs * 8 + 16;
+// size, one shift): lds_elems data elements per item of the group
+// (interleaved), then each wave's copy of the first kParamRegs angles of every
+// item of the group, then the queue ticket.
+__host__ __device__ constexpr int exec_aux_bytes(int item_threads, int group) {
+    return ((item_threads + 63) / 64) * group * kParamRegs * 8 + 16;
 }
 
-template <typename R, int ITEM_THREADS, bool GLB>
-__global__ __launch_bounds__(ITEM_THREADS, kExecWavesPerEU<R>) void exec_kernel(
+// The executor.  Block b runs group b first: items [b M, b M + M), which share
+// one plan (the host checks that); later groups come from the queue.
+template <typename R, int ITEM_THREADS, bool GLB, int M>
+__global__ __launch_bounds__(ITEM_THREADS, (kExecWavesPerEU<R, M>)) void exec_kernel(
     const int32_t* __restrict__ program, uint32_t program_bytes, const int32_t* __restrict__ items,
     int32_t n_items, int32_t lds_elems, const double* __restrict__ params, const void* __restrict__ pool_v,
     void* __restrict__ scratch, int64_t scratch_elems, double* __restrict__ item_out,
     uint32_t* __restrict__ queue) {
     using V = typename Cx<R>::V;
+    using P = typename PkOf<R, M>::T;
+    static_assert(sizeof(P) == M * sizeof(V), "a group element is M item elements");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // The kernel declares no static LDS, so its dynamic LDS starts at address
     // 0 (the launch checks that); element e of the data region is at byte
-    // e * sizeof(V), which the compiler then folds into one shift.
-    QB_LDS V* lds = (QB_LDS V*)(uintptr_t)0u;
-    QB_LDS double* angles_sh = (QB_LDS double*)(smem_raw + (size_t)lds_elems * sizeof(V));
-    QB_LDS int32_t* item_sh = (QB_LDS int32_t*)(angles_sh + ((ITEM_THREADS + 63) / 64) * kParamRegs);
+    // e * sizeof(P), which the compiler then folds into one shift.
+    QB_LDS P* lds = (QB_LDS P*)(uintptr_t)0u;
+    QB_LDS V* lds_v = (QB_LDS V*)(uintptr_t)0u;   // item i's element e at e * M + i
+    QB_LDS double* angles_sh = (QB_LDS double*)(smem_raw + (size_t)lds_elems * sizeof(P));
+    QB_LDS int32_t* item_sh = (QB_LDS int32_t*)(angles_sh + ((ITEM_THREADS + 63) / 64) * M * kParamRegs);
     const int lane = threadIdx.x;
-    QB_GLB V* glb = (QB_GLB V*)(scratch) + (int64_t)blockIdx.x * scratch_elems;
-    // the item's scratch slot as a bounds-checked buffer (qb_execute keeps a
-    // slot below 4 GiB)
+    QB_GLB P* glb = (QB_GLB P*)(scratch) + (int64_t)blockIdx.x * scratch_elems;
+    // the group's scratch slots as one bounds-checked buffer (qb_execute keeps
+    // it below 4 GiB)
     const __amdgpu_buffer_rsrc_t srd = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)glb, (short)0, (int)(uint32_t)(scratch_elems * (int64_t)sizeof(V)), (int)kBufferDword3);
+        (void*)glb, (short)0, (int)(uint32_t)(scratch_elems * (int64_t)sizeof(P)), (int)kBufferDword3);
     const __amdgpu_buffer_rsrc_t prog = __builtin_amdgcn_make_buffer_rsrc(
         (void*)program, (short)0, (int)program_bytes, (int)kBufferDword3);
     const V* pool = reinterpret_cast<const V*>(pool_v);
-    // the first item of every block is static (item = block index), so the
+    const int32_t n_groups = n_items / M;
+    // the first group of every block is static (group = block index), so the
     // launch does not start with a burst of atomics on one address; later
-    // items come from the queue.  A ticket is taken only when an item is done
-    // (items finish at different times, so the atomics do not pile up), and
-    // not at all when every item had a block of its own.  (Vector memory
+    // groups come from the queue.  A ticket is taken only when a group is done
+    // (groups finish at different times, so the atomics do not pile up), and
+    // not at all when every group had a block of its own.  (Vector memory
     // operations complete in order, so an atomic issued early would hold up
-    // every later load of the item until it returns.)
-    int32_t item = (int32_t)blockIdx.x;
-    while (item < n_items) {
-        const int32_t* __restrict__ it = items + (size_t)item * QB_ITEM_WORDS;
+    // every later load of the group until it returns.)
+    int32_t grp = (int32_t)blockIdx.x;
+    while (grp < n_groups) {
+        const int32_t* __restrict__ it = items + (size_t)grp * M * QB_ITEM_WORDS;
         const int32_t* __restrict__ plan = program + it[0];
-        const double* __restrict__ prm = params + it[1];
-        const uint32_t slice_bits = (uint32_t)it[2];
-        const int32_t slot = it[3];
         const int32_t n_gates = plan[0];
         const int32_t* __restrict__ gates = plan + plan[2];
         // op 0's record is in flight while the gates are built
         const uint32_t st0 = (uint32_t)(it[0] + plan[3]);
         const int32_t r0 = fetch_rec(prog, st0, lane);
         if (lane == 0) {
-            // materialised per item: a copy held across the item loop would be
-            // spilled under the register cap
+            // materialised per group: a copy held across the group loop would
+            // be spilled under the register cap
             R one = 1, zero = 0;
             asm volatile("" : "+v"(one), "+v"(zero));
-            lds[kLocOne] = V{one, zero};
-            lds[kLocZero] = V{zero, zero};
+            lds[kLocOne] = pk_fill(V{one, zero}, (P*)nullptr);
+            lds[kLocZero] = pk_fill(V{zero, zero}, (P*)nullptr);
         }
-        double th[kParamRegs];
+        QB_LDS double* my_angles = angles_sh + (lane >> 6) * M * kParamRegs;
 #pragma unroll
-        for (int q = 0; q < kParamRegs; ++q) th[q] = prm[q];
-        QB_LDS double* my_angles = angles_sh + (lane >> 6) * kParamRegs;
-        if ((lane & 63) == 0) {
+        for (int i = 0; i < M; ++i) {
+            const double* __restrict__ prm = params + it[i * QB_ITEM_WORDS + 1];
+            double th[kParamRegs];
 #pragma unroll
-            for (int q = 0; q < kParamRegs; ++q) my_angles[q] = th[q];
+            for (int q = 0; q < kParamRegs; ++q) th[q] = prm[q];
+            if ((lane & 63) == 0) {
+#pragma unroll
+                for (int q = 0; q < kParamRegs; ++q) my_angles[i * kParamRegs + q] = th[q];
+            }
         }
         // the wave's own LDS writes are visible to its later LDS reads
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
-            build_gate<R>(gates + (size_t)g * kGateWords, prm, my_angles, kParamRegs, slice_bits, lds, pool);
+        if constexpr (M == 1) {
+            const double* __restrict__ prm = params + it[1];
+            const uint32_t slice_bits = (uint32_t)it[2];
+            for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
+                build_gate<R>(gates + (size_t)g * kGateWords, prm, my_angles, kParamRegs, slice_bits, lds, pool);
+        } else {
+            // lane t builds gate t / M of item t % M
+            for (int32_t t = lane; t < n_gates * M; t += ITEM_THREADS) {
+                const int32_t g = t / M, i = t % M;
+                int32_t aoff = it[1];
+                uint32_t slice_bits = (uint32_t)it[2];
+#pragma unroll
+                for (int k = 1; k < M; ++k) {
+                    if (i == k) {
+                        aoff = it[k * QB_ITEM_WORDS + 1];
+                        slice_bits = (uint32_t)it[k * QB_ITEM_WORDS + 2];
+                    }
+                }
+                build_gate<R>(gates + (size_t)g * kGateWords, params + aoff, my_angles + i * kParamRegs, kParamRegs,
+                              slice_bits, LdsLane<V, M>{lds_v + i}, pool);
+            }
+        }
         group_sync<ITEM_THREADS>();
-        run_ops<R, ITEM_THREADS, GLB>(prog, st0, plan[1], r0, lds, srd, lane);
+        run_ops<P, ITEM_THREADS, GLB>(prog, st0, plan[1], r0, lds, srd, lane);
         group_sync<ITEM_THREADS>();
-        if (lane == 0) {
-            // the result: LDS, or the item's scratch slot (through its buffer)
+        if (lane < M) {
+            // item `lane`'s result: LDS, or the group's scratch (through its buffer)
             const uint32_t rl = (uint32_t)plan[4];
-            const V r = (rl & kGlobal) ? buf_load(srd, rl & ~kGlobal, (V*)nullptr) : lds[rl];
+            int32_t slot = it[3];
+#pragma unroll
+            for (int k = 1; k < M; ++k)
+                if (lane == k) slot = it[k * QB_ITEM_WORDS + 3];
+            const uint32_t e = (rl & ~kGlobal) * (uint32_t)M + (uint32_t)lane;
+            const V r = (rl & kGlobal) ? buf_load(srd, e, (V*)nullptr) : lds_v[e];
             item_out[2 * (size_t)slot] = (double)r.x;
             item_out[2 * (size_t)slot + 1] = (double)r.y;
         }
-        if ((int64_t)gridDim.x >= (int64_t)n_items) break;
+        if ((int64_t)gridDim.x >= (int64_t)n_groups) break;
         if (lane == 0) *item_sh = (int32_t)(gridDim.x + atomicAdd(queue, 1u));
         group_sync<ITEM_THREADS>();
-        item = __builtin_amdgcn_readfirstlane(*item_sh);
-        // item_sh is rewritten only after the next item's barriers
+        grp = __builtin_amdgcn_readfirstlane(*item_sh);
+        // item_sh is rewritten only after the next group's barriers
     }
 }
 
@@ -1210,78 +1301,85 @@ __global__ __launch_bounds__(256) void reduce_total(const double* __restrict__ t
     }
 }
 
-template <typename R, int T>
-int exec_occupancy(int lds_bytes, int* blocks) {
-    QB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, exec_kernel<R, T, true>, T, lds_bytes));
-    return QB_OK;
-}
+// One executor instantiation: precision R, ITEM_THREADS T, global-scratch
+// variant GLB, group size M.
+template <typename R, int T, bool GLB, int M>
+struct ExecCfg {
+    static const void* kernel() { return reinterpret_cast<const void*>(&exec_kernel<R, T, GLB, M>); }
+};
 
-template <typename R>
-int exec_occupancy_dispatch(int item_threads, int lds_bytes, int* blocks) {
-    switch (item_threads) {
-    case 64: return exec_occupancy<R, 64>(lds_bytes, blocks);
-    case 128: return exec_occupancy<R, 128>(lds_bytes, blocks);
-    case 256: return exec_occupancy<R, 256>(lds_bytes, blocks);
-    case 512: return exec_occupancy<R, 512>(lds_bytes, blocks);
-    case 1024: return exec_occupancy<R, 1024>(lds_bytes, blocks);
-    default: return set_error(QB_ERR_INVALID, "item_threads %d not in {64,128,256,512,1024}", item_threads);
+// Calls f(ExecCfg<...>{}) for a launch shape.  Groups of 2 items run complex64
+// items of 64, 128 or 256 threads; every other shape runs single items.
+template <typename F>
+int exec_dispatch(int32_t dtype, int item_threads, int group, bool glb, F&& f) {
+#define QB_EXEC_T(R, G, M)                                             \
+    switch (item_threads) {                                            \
+    case 64: return f(ExecCfg<R, 64, G, M>{});                         \
+    case 128: return f(ExecCfg<R, 128, G, M>{});                       \
+    case 256: return f(ExecCfg<R, 256, G, M>{});                       \
+    default: break;                                                    \
     }
+#define QB_EXEC_T1(R, G)                                               \
+    QB_EXEC_T(R, G, 1)                                                 \
+    if (item_threads == 512) return f(ExecCfg<R, 512, G, 1>{});        \
+    if (item_threads == 1024) return f(ExecCfg<R, 1024, G, 1>{});
+    if (group == 1) {
+        if (dtype == QB_C128) {
+            if (glb) { QB_EXEC_T1(double, true) } else { QB_EXEC_T1(double, false) }
+        } else {
+            if (glb) { QB_EXEC_T1(float, true) } else { QB_EXEC_T1(float, false) }
+        }
+        return set_error(QB_ERR_INVALID, "item_threads %d not in {64,128,256,512,1024}", item_threads);
+    }
+    if (group == 2 && dtype == QB_C64) {
+        if (glb) { QB_EXEC_T(float, true, 2) } else { QB_EXEC_T(float, false, 2) }
+        return set_error(QB_ERR_INVALID, "groups of 2 items run item_threads 64, 128 or 256, not %d", item_threads);
+    }
+#undef QB_EXEC_T1
+#undef QB_EXEC_T
+    return set_error(QB_ERR_INVALID, "group %d: 1, or 2 for complex64", group);
 }
 
-template <typename R, int T, bool GLB>
-int exec_launch_variant(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
+template <typename R, int T, bool GLB, int M>
+int exec_launch_cfg(ExecCfg<R, T, GLB, M> cfg, const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
     // the kernel addresses its LDS data from address 0: it must declare no
     // static LDS (checked once per instantiation)
     static int static_lds = -1;
     if (static_lds < 0) {
         hipFuncAttributes attr;
-        QB_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&exec_kernel<R, T, GLB>)));
+        QB_HIP(hipFuncGetAttributes(&attr, cfg.kernel()));
         static_lds = (int)attr.sharedSizeBytes;
     }
     if (static_lds != 0)
         return set_error(QB_ERR_UNSUPPORTED, "executor kernel declares %d bytes of static LDS", static_lds);
-    hipLaunchKernelGGL((exec_kernel<R, T, GLB>), dim3(grid), dim3(T), lds_bytes, st, a.program, a.program_bytes,
+    hipLaunchKernelGGL((exec_kernel<R, T, GLB, M>), dim3(grid), dim3(T), lds_bytes, st, a.program, a.program_bytes,
                        a.items, a.n_items, a.lds_elems, a.params, a.pool, a.scratch, a.scratch_elems, a.item_out,
                        a.queue);
     QB_HIP(hipGetLastError());
     return QB_OK;
 }
 
-// Batches whose plans need no global scratch (scratch_elems == 0) run the
-// variant without the global-scratch path.
-template <typename R, int T>
-int exec_launch(const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
-    if (a.scratch_elems > 0) return exec_launch_variant<R, T, true>(a, grid, lds_bytes, st);
-    return exec_launch_variant<R, T, false>(a, grid, lds_bytes, st);
-}
-
-template <typename R>
-int exec_launch_dispatch(int item_threads, const ExecArgs& a, int grid, int lds_bytes, hipStream_t st) {
-    switch (item_threads) {
-    case 64: return exec_launch<R, 64>(a, grid, lds_bytes, st);
-    case 128: return exec_launch<R, 128>(a, grid, lds_bytes, st);
-    case 256: return exec_launch<R, 256>(a, grid, lds_bytes, st);
-    case 512: return exec_launch<R, 512>(a, grid, lds_bytes, st);
-    case 1024: return exec_launch<R, 1024>(a, grid, lds_bytes, st);
-    default: return set_error(QB_ERR_INVALID, "item_threads %d not in {64,128,256,512,1024}", item_threads);
-    }
-}
-
-int slots_for(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n_slots, int* grid_per_cu) {
+// Blocks resident per CU (*grid_per_cu) and items resident on the device
+// (*n_slots: blocks x CUs x group) for a launch shape.  Batches whose plans need
+// no global scratch run the variant without the global-scratch path (glb).
+int slots_for(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t group, bool glb, int32_t* n_slots,
+              int* grid_per_cu) {
     if (g_device < 0) return set_error(QB_ERR_NOT_INIT, "qb_init was not called");
     if (lds_elems < 0) return set_error(QB_ERR_INVALID, "lds_elems < 0");
+    if (group < 1) return set_error(QB_ERR_INVALID, "group %d < 1", group);
     const int elem = dtype == QB_C128 ? 16 : 8;
-    const int groups = 1;
-    const int64_t lds_bytes = (int64_t)lds_elems * elem + exec_aux_bytes(item_threads);
+    const int64_t lds_bytes = (int64_t)lds_elems * elem * group + exec_aux_bytes(item_threads, group);
     if (lds_bytes > 160 * 1024)
         return set_error(QB_ERR_UNSUPPORTED, "LDS request %lld bytes exceeds 160 KiB", (long long)lds_bytes);
     int blocks = 0;
-    int rc = dtype == QB_C128 ? exec_occupancy_dispatch<double>(item_threads, (int)lds_bytes, &blocks)
-                              : exec_occupancy_dispatch<float>(item_threads, (int)lds_bytes, &blocks);
+    const int rc = exec_dispatch(dtype, item_threads, group, glb, [&](auto cfg) {
+        QB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, cfg.kernel(), item_threads, (size_t)lds_bytes));
+        return QB_OK;
+    });
     if (rc) return rc;
     if (blocks < 1) return set_error(QB_ERR_UNSUPPORTED, "executor shape does not fit on a CU");
     *grid_per_cu = blocks;
-    *n_slots = blocks * g_num_cus * groups;
+    *n_slots = blocks * g_num_cus * group;
     return QB_OK;
 }
 
@@ -1728,18 +1826,24 @@ int execute_impl(const qb_exec_desc* d, void* scratch, int32_t n_slots_avail, ui
     if (d->n_items < 0) return set_error(QB_ERR_INVALID, "n_items < 0");
     if (!d->program || !d->items || !d->params || !d->item_out || !queue)
         return set_error(QB_ERR_INVALID, "null device pointer in descriptor");
+    const int32_t group = d->group > 1 ? d->group : 1;
+    if (d->n_items % group != 0)
+        return set_error(QB_ERR_INVALID, "n_items %d is not a multiple of group %d", d->n_items, group);
+    const bool glb = d->scratch_elems > 0;
     int32_t n_slots = 0;
     int per_cu = 0;
-    int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, &n_slots, &per_cu);
+    int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, group, glb, &n_slots, &per_cu);
     if (rc) return rc;
+    const int32_t n_groups = d->n_items / group;
     int grid = per_cu * g_num_cus;
-    if (d->n_items < grid) grid = d->n_items;
+    if (n_groups < grid) grid = n_groups;
     if (grid < 1) grid = 1;
-    if ((int64_t)grid > n_slots_avail && d->scratch_elems > 0)
-        return set_error(QB_ERR_INVALID, "scratch holds %d slots, launch needs %d", n_slots_avail, grid);
+    if ((int64_t)grid * group > n_slots_avail && glb)
+        return set_error(QB_ERR_INVALID, "scratch holds %d slots, launch needs %lld", n_slots_avail,
+                         (long long)grid * group);
     if (d->scratch_elems > 0 && !scratch) return set_error(QB_ERR_INVALID, "scratch is null");
-    if (d->scratch_elems < 0 || d->scratch_elems * (d->dtype == QB_C128 ? 16 : 8) > (int64_t)0xFFFFFFFF)
-        return set_error(QB_ERR_UNSUPPORTED, "scratch slot of %lld elements exceeds 4 GiB",
+    if (d->scratch_elems < 0 || d->scratch_elems * (d->dtype == QB_C128 ? 16 : 8) * group > (int64_t)0xFFFFFFFF)
+        return set_error(QB_ERR_UNSUPPORTED, "scratch of a group (%d x %lld elements) exceeds 4 GiB", group,
                          (long long)d->scratch_elems);
     if (d->program_words < 0 || d->program_words > (1 << 30))
         return set_error(QB_ERR_INVALID, "program_words %d outside 0..2^30", d->program_words);
@@ -1756,20 +1860,20 @@ int execute_impl(const qb_exec_desc* d, void* scratch, int32_t n_slots_avail, ui
     a.item_out = d->item_out;
     a.queue = queue;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (grid < d->n_items) {
+    if (grid < n_groups) {
         // a fill kernel, not a memset: the launch may be captured in a graph
         hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, st, queue);
         QB_HIP(hipGetLastError());
     }
     const int elem = d->dtype == QB_C128 ? 16 : 8;
-    const int lds_bytes = d->lds_elems * elem + exec_aux_bytes(d->item_threads);
-    return d->dtype == QB_C128 ? exec_launch_dispatch<double>(d->item_threads, a, grid, lds_bytes, st)
-                               : exec_launch_dispatch<float>(d->item_threads, a, grid, lds_bytes, st);
+    const int lds_bytes = d->lds_elems * elem * group + exec_aux_bytes(d->item_threads, group);
+    return exec_dispatch(d->dtype, d->item_threads, group, glb,
+                         [&](auto cfg) { return exec_launch_cfg(cfg, a, grid, lds_bytes, st); });
 }
 
 int resident_items(const qb_exec_desc* d, int32_t* n_slots) {
     int per_cu = 0;
-    return slots_for(d->item_threads, d->lds_elems, d->dtype, n_slots, &per_cu);
+    return slots_for(d->item_threads, d->lds_elems, d->dtype, d->group > 1 ? d->group : 1, true, n_slots, &per_cu);
 }
 
 int current_device() { return g_device; }
@@ -1854,10 +1958,10 @@ int qb_step(const qb_step_desc* d, const void* const* inputs, void* out, void* s
     return QB_OK;
 }
 
-int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t* n_slots) {
+int qb_exec_slots(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t group, int32_t* n_slots) {
     if (!n_slots) return set_error(QB_ERR_INVALID, "n_slots is null");
     int per_cu = 0;
-    return slots_for(item_threads, lds_elems, dtype, n_slots, &per_cu);
+    return slots_for(item_threads, lds_elems, dtype, group > 1 ? group : 1, true, n_slots, &per_cu);
 }
 
 int qb_execute(const qb_exec_desc* d, void* stream) {
@@ -2005,6 +2109,7 @@ struct qb_plan {
     int32_t n_terms = 0;
     int32_t n_params = 0;
     int32_t n_slots = 0;
+    int32_t group = 1;
     int64_t scratch_elems = 0;
     int64_t program_words = 0;
     bool unit_seg = false;       // every term has exactly one item
@@ -2031,9 +2136,19 @@ static void plan_free(qb_plan* p) {
 }
 
 int qb_check_program(const int32_t* program, int64_t program_words, const int32_t* items, int32_t n_items,
-                     int32_t n_params, int32_t item_threads, int32_t lds_elems, int64_t scratch_elems) {
-    if (!program || !items || n_items < 0 || program_words < 1 || n_params < 0 || lds_elems < 0 || scratch_elems < 0)
+                     int32_t n_params, int32_t item_threads, int32_t lds_elems, int64_t scratch_elems,
+                     int32_t group) {
+    if (!program || !items || n_items < 0 || program_words < 1 || n_params < 0 || lds_elems < 0 || scratch_elems < 0 ||
+        group < 0)
         return set_error(QB_ERR_INVALID, "null or negative program, items or sizes");
+    if (group > 1) {
+        if (group != 2) return set_error(QB_ERR_INVALID, "group %d: 1 or 2", group);
+        if (n_items % group != 0)
+            return set_error(QB_ERR_INVALID, "n_items %d is not a multiple of group %d", n_items, group);
+        for (int32_t i = 0; i < n_items; ++i)
+            if (items[(size_t)i * QB_ITEM_WORDS] != items[(size_t)(i - i % group) * QB_ITEM_WORDS])
+                return set_error(QB_ERR_INVALID, "item %d: its group's items do not share one plan", i);
+    }
     if (program_words > (1 << 30))
         return set_error(QB_ERR_INVALID, "program_words=%lld > 2^30", (long long)program_words);
     if (item_threads != 64 && item_threads != 128 && item_threads != 256 && item_threads != 512 && item_threads != 1024)
@@ -2079,16 +2194,19 @@ int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
     // host-side checks of what the kernels will index
     for (int32_t t = 0; t < d->n_terms; ++t)
         if (d->seg[t] > d->seg[t + 1]) return set_error(QB_ERR_INVALID, "segments must not decrease");
-    if (d->seg[0] != 0 || d->seg[d->n_terms] != d->n_items)
-        return set_error(QB_ERR_INVALID, "segments must cover items 0..n_items");
+    const int32_t group = d->group > 1 ? d->group : 1;
+    // every item of a term in some segment; padding items (completing a group)
+    // may follow the last segment
+    if (d->seg[0] != 0 || d->seg[d->n_terms] > d->n_items || d->seg[d->n_terms] + group <= d->n_items)
+        return set_error(QB_ERR_INVALID, "segments must cover items 0..n_items (up to one group's padding)");
     bool unit_seg = true;
     for (int32_t t = 0; t < d->n_terms; ++t) unit_seg = unit_seg && d->seg[t + 1] - d->seg[t] == 1;
     int rc0 = qb_check_program(d->program, d->program_words, d->items, d->n_items, d->n_params, d->item_threads,
-                               d->lds_elems, d->scratch_elems);
+                               d->lds_elems, d->scratch_elems, group);
     if (rc0) return rc0;
     int32_t n_slots = 0;
     int per_cu = 0;
-    int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, &n_slots, &per_cu);
+    int rc = slots_for(d->item_threads, d->lds_elems, d->dtype, group, true, &n_slots, &per_cu);
     if (rc) return rc;
     qb_plan* p = new qb_plan();
     p->device = g_device;
@@ -2098,6 +2216,7 @@ int qb_plan_create(const qb_plan_desc* d, qb_plan** out) {
     p->n_items = d->n_items;
     p->n_terms = d->n_terms;
     p->n_params = d->n_params;
+    p->group = group;
     p->n_slots = n_slots < d->n_items ? n_slots : d->n_items;
     p->scratch_elems = d->scratch_elems;
     p->program_words = d->program_words;
@@ -2150,6 +2269,7 @@ int qb_plan_execute(qb_plan* p, const double* params, double* term_out, double* 
     d.program_words = (int32_t)p->program_words;
     d.item_out = p->item_out;
     d.queue = p->queue;
+    d.group = p->group;
     int rc = qb_execute(&d, stream);
     if (rc) return rc;
     return qb_reduce_terms(p->item_out, p->unit_seg ? nullptr : p->seg, p->n_terms, term_out, total, stream);

@@ -31,7 +31,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .batched import order_items
+from .batched import group_items
 from .plan import PARAM_PAD, PROGRAM_PAD
 
 
@@ -421,17 +421,21 @@ class EnergyServer:
             costs.append(cost)
             sizes.append(seg_sizes)
             lds_need, glb_need = max(lds_need, ln), max(glb_need, gn)
-        items = order_items(np.concatenate(parts), np.concatenate(costs),
-                            band=torch.cuda.get_device_properties(self.device).multi_processor_count)
+        p0 = len(batch[0].gamma)
+        shape = self.engine_for(p0).shape_for(p0)
+        group = max(1, int(shape.group))
+        # items of one plan (requests on one graph) share a block in groups
+        items, _ = group_items(np.concatenate(parts), np.concatenate(costs), group,
+                               band=torch.cuda.get_device_properties(self.device).multi_processor_count)
         params += [0.0] * PARAM_PAD      # the executor reads 8 angles per item at once
         seg = np.concatenate([[0], np.cumsum(np.concatenate(sizes))]).astype(np.int32)
         n_items, n_terms = items.shape[0], seg.size - 1
+        n_real = int(seg[-1])            # items of terms (the rest pad the last groups)
         # newly stored plans were copied on this thread's stream
         torch.cuda.current_stream(self.device).synchronize()
-        p0 = len(batch[0].gamma)
-        shape = self.engine_for(p0).shape_for(p0)
         slots_c = ctypes.c_int32(0)
-        _lib.check(self.lib.qb_exec_slots(shape.item_threads, lds_need, self.qb_dtype, ctypes.byref(slots_c)))
+        _lib.check(self.lib.qb_exec_slots(shape.item_threads, lds_need, self.qb_dtype, group,
+                                          ctypes.byref(slots_c)))
         used_slots = min(slots_c.value, n_items)
         d_items, h_items = self._ensure(slot, "items", 4 * n_items, torch.int32)
         d_params, h_params = self._ensure(slot, "params", len(params), torch.float64)
@@ -470,10 +474,11 @@ class EnergyServer:
             d.program_words = self.store.buf.numel()
             d.item_out = d_out.data_ptr()
             d.queue = d_queue.data_ptr()
+            d.group = group
             h = ctypes.c_void_p(st.cuda_stream)
             _lib.check(self.lib.qb_execute(ctypes.byref(d), h))
             _lib.check(self.lib.qb_reduce_terms(ctypes.c_void_p(d_out.data_ptr()),
-                                                None if n_items == n_terms else ctypes.c_void_p(d_seg.data_ptr()),
+                                                None if n_real == n_terms else ctypes.c_void_p(d_seg.data_ptr()),
                                                 n_terms,
                                                 ctypes.c_void_p(d_terms.data_ptr()),
                                                 ctypes.c_void_p(d_total.data_ptr()), h))

@@ -30,6 +30,7 @@ def _check(vals, refs, tol_fn=pf.tol):
 
 @pytest.mark.parametrize("config", [
     ("tuned", LaunchShape(256, 4096, 2048), 9, "tree_trim", "rgreedy", QtreeQAOAComposer),
+    ("tuned_pairs", LaunchShape(256, 4096, 2048, group=2), 9, "tree_trim", "rgreedy", QtreeQAOAComposer),
     ("greedy", LaunchShape(256, 8192, 4096), None, "greedy", "greedy", QtreeQAOAComposer),
     ("cli_composer", LaunchShape(256, 4096, 2048), 10, "tree_trim", "rgreedy", DefaultQAOAComposer),
 ], ids=lambda c: c[0])

@@ -195,7 +195,7 @@ def test_library_exports_header_symbols():
     assert lib.qb_version() >= 100
     # calls that need no device report "not initialised" cleanly
     n = ctypes.c_int32(0)
-    assert lib.qb_exec_slots(64, 16, 0, ctypes.byref(n)) == _lib.QB_ERR_NOT_INIT
+    assert lib.qb_exec_slots(64, 16, 0, 1, ctypes.byref(n)) == _lib.QB_ERR_NOT_INIT
     assert b"qb_init" in lib.qb_last_error()
 
 
@@ -306,7 +306,7 @@ def test_autotuner_two_stage_search(monkeypatch):
     calls = []
 
     def fake_time(self, G, p, gamma, beta, shape, mtw, slicer="greedy"):
-        calls.append((shape.key(), mtw, slicer))
+        calls.append((shape, mtw, slicer))
         if shape.key() == (64, 4096, 2048):
             raise ValueError("LDS capacity exhausted")
         cost = {None: 5.0, 13: 4.0, 12: 3.0, 11: 2.0, 10: 2.5}[mtw] - 0.6 * (slicer == "tree_trim" and mtw == 10)
@@ -315,11 +315,11 @@ def test_autotuner_two_stage_search(monkeypatch):
     monkeypatch.setattr(autotune.Autotuner, "time_config", fake_time)
     monkeypatch.setattr(autotune, "slicing_candidates", lambda max_tw: [None, 13, 12, 11, 10])
     shape, mtw, slicer, table = autotune.Autotuner().tune(G, 2, [0.1, 0.2], [0.3, 0.4])
-    base = [c for c in calls if c[0] == autotune.BASE_SHAPE.key()]
+    base = [c for c in calls if c[0] == autotune.BASE_SHAPE]
     # every threshold with each slicer, unsliced once
     assert [(m, sl) for _, m, sl in base] == [(None, "greedy")] + [(m, sl) for m in (13, 12, 11, 10)
                                                                   for sl in autotune.SLICERS]
-    rest = [c for c in calls if c[0] != autotune.BASE_SHAPE.key()]
+    rest = [c for c in calls if c[0] != autotune.BASE_SHAPE]
     # the best (threshold, slicer) pairs of stage 1 under every other shape
     # (10, tree_trim) costs 1.9, then (11, greedy) and (11, tree_trim) at 2.0
     assert sorted({(m, sl) for _, m, sl in rest}) == [(10, "tree_trim"), (11, "greedy"), (11, "tree_trim")]
@@ -441,6 +441,47 @@ def test_order_items_serpentine_bands():
     snake = order_items(items, costs, band=4)          # runs [1 5 3 7] [0 8 4 6] [2 9], the second reversed
     assert list(snake[:, 0]) == [1, 5, 3, 7, 6, 4, 8, 0, 2, 9]
     assert list(snake[:, 3]) == list(snake[:, 0])
+
+
+def test_group_items_pairs_items_of_one_plan():
+    """Groups of two: both items of a group share a plan, every real item
+    appears once with its own slot, padding items copy their group's last item
+    and take the slots after the real ones; groups are longest first."""
+    import numpy as np
+    from qtensor_amd.batched import group_items
+    # plan offsets 100 (3 items, cost 5), 200 (2 items, cost 9), 300 (1 item, cost 1)
+    items = [(100, 0, 0, 0), (200, 4, 0, 0), (100, 0, 1, 0), (300, 8, 0, 0), (100, 0, 2, 0), (200, 4, 1, 0)]
+    costs = [5, 9, 5, 1, 5, 9]
+    out, n_pad = group_items(items, costs, 2)
+    assert n_pad == 2 and out.shape == (8, 4)
+    for g in range(0, 8, 2):
+        assert out[g, 0] == out[g + 1, 0]
+    assert list(out[::2, 0]) == [200, 100, 100, 300]
+    slots = sorted(out[:, 3])
+    assert slots == list(range(8))
+    real = out[out[:, 3] < 6]
+    assert sorted(map(tuple, real[:, :3])) == sorted(items_[:3] for items_ in items)
+    for k in np.nonzero(out[:, 3] >= 6)[0]:
+        assert tuple(out[k, :3]) == tuple(out[k - 1, :3])    # a copy of its group's last real item
+    one, n0 = group_items(items, costs, 1)
+    assert n0 == 0 and one.shape == (6, 4)
+
+
+def test_check_program_groups():
+    """With groups, every group's items share a plan and the item count is a
+    multiple of the group size."""
+    plans, prog, items, lds, glb = _checked_program()
+    pair = np.array([items[0], items[0], items[1], items[1]], dtype=np.int32)
+    pair[:, 3] = [0, 1, 2, 3]
+    _lib.check_program(prog, pair, 4, 64, lds, glb, group=2)
+    mixed = pair.copy()
+    mixed[1, 0] = items[1, 0]
+    with pytest.raises(ValueError):
+        _lib.check_program(prog, mixed, 4, 64, lds, glb, group=2)
+    with pytest.raises(ValueError):
+        _lib.check_program(prog, pair[:3], 4, 64, lds, glb, group=2)
+    with pytest.raises(ValueError):
+        _lib.check_program(prog, pair, 4, 64, lds, glb, group=3)
 
 
 def _checked_program(lanes=64, lim=64, cap=4096, p=2, seed=3):
