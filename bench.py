@@ -34,15 +34,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per executor launch on the default workload under the launch
-# configuration the autotuner picks for it (256-thread items, randomized-greedy
-# order, tree-trimming slices below treewidth 9: 1013 work items, no global
-# scratch; the LDS budgets 4096 and 8192 compile the same plans), from rocprofv3
-# FETCH_SIZE / WRITE_SIZE runs (profiles/exec_kernel_qaoa_p3_n256.md): 14297.3
-# KiB fetched x 2.0 (a 1 GiB device copy reads as 524308 KiB) + 31.0 KiB written
+# configuration the autotuner picks for it (256-thread items run in pairs of one
+# plan per block, randomized-greedy order, tree-trimming slices below treewidth
+# 8: 1702 work items with padding, no global scratch), from rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs (profiles/exec_kernel_qaoa_p3_n256.md): 13372.3
+# KiB fetched x 2.0 (a 1 GiB device copy reads as 524306 KiB) + 34.8 KiB written
 # x 1.0 (the copy's 1 GiB writes read as 1048576 KiB).  Reported only for that
-# configuration: (item threads, slicing threshold, slicer, order, work items).
-EXEC_TRAFFIC_CONFIG = (256, 1, 9, "tree_trim", "rgreedy", 1013)
-EXEC_TRAFFIC_BYTES = int(14297.3 * 1024 * 2.0 + 31.0 * 1024 * 1.0)
+# configuration: (item threads, group, slicing threshold, slicer, order, work items).
+EXEC_TRAFFIC_CONFIG = (256, 2, 8, "tree_trim", "rgreedy", 1702)
+EXEC_TRAFFIC_BYTES = int(13372.3 * 1024 * 2.0 + 34.8 * 1024 * 1.0)
 
 
 def workload(name):

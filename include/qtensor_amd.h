@@ -134,7 +134,7 @@ typedef struct qb_exec_desc {
     uint32_t* queue;             /* device: one uint32 (reset by the call when the
                                     groups outnumber the resident blocks) */
     int32_t group;               /* items per block, run in lockstep (0 or 1: one;
-                                    2: complex64 with item_threads 64, 128 or 256).
+                                    2: complex64 with item_threads 64 ... 512).
                                     Items come in groups of `group` consecutive
                                     items sharing one plan offset; n_items is a
                                     multiple of group.  A group keeps its items'

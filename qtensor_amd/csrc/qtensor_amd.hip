@@ -641,6 +641,52 @@ __device__ __forceinline__ void build_gate(const int32_t* g, const double* param
     }
 }
 
+// Gate g for every item of a group whose items share one angle vector: the
+// FP64 values are computed once, and item i's copy (its slice shift) goes to
+// element e * M + i of the interleaved LDS data.
+template <typename R, int M>
+__device__ __forceinline__ void build_gate_group(const int32_t* g, const double* params,
+                                                 const QB_LDS double* lds_angles, int32_t n_lds,
+                                                 const uint32_t* slice_bits, QB_LDS typename Cx<R>::V* lds_v,
+                                                 const typename Cx<R>::V* pool) {
+    using V = typename Cx<R>::V;
+    const int32_t kind = g[0];
+    const int32_t dst = g[1];
+    const int32_t n = g[2];
+    const uint32_t src_map = (uint32_t)g[3];
+    const int32_t sl = g[11];
+    const int32_t n_slice = sl & 0xff;
+    uint32_t shift[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        shift[i] = 0;
+        if (n_slice > 0) shift[i] += ((slice_bits[i] >> ((sl >> 8) & 0xff)) & 1u) * (uint32_t)g[12];
+        if (n_slice > 1) shift[i] += ((slice_bits[i] >> ((sl >> 16) & 0xff)) & 1u) * (uint32_t)g[13];
+    }
+    if (kind == kConst) {
+        for (int e = 0; e < n; ++e)
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+                lds_v[(uint32_t)(dst + e) * M + i] = pool[g[10] + ((src_map >> (4 * e)) & 15u) + shift[i]];
+        return;
+    }
+    double alpha = __hiloint2double(g[9], g[8]);
+    if (g[4] >= 0) alpha += (double)__int_as_float(g[5]) * angle_at(lds_angles, n_lds, params, g[4]);
+    if (g[6] >= 0) alpha += (double)__int_as_float(g[7]) * angle_at(lds_angles, n_lds, params, g[6]);
+    double re[8], im[8];
+    natural_gate(kind, alpha, re, im);
+    for (int e = 0; e < n; ++e) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const uint32_t s = ((src_map >> (4 * e)) & 15u) + shift[i];
+            V v;
+            v.x = (R)re[s & 7];
+            v.y = (R)im[s & 7];
+            lds_v[(uint32_t)(dst + e) * M + i] = v;
+        }
+    }
+}
+
 struct ExecArgs {
     const int32_t* program;
     uint32_t program_bytes;
@@ -1179,11 +1225,22 @@ __global__ __launch_bounds__(ITEM_THREADS, (kExecWavesPerEU<R, M>)) void exec_ke
         // the wave's own LDS writes are visible to its later LDS reads
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        bool same_angles = true;
+#pragma unroll
+        for (int i = 1; i < M; ++i) same_angles = same_angles && it[i * QB_ITEM_WORDS + 1] == it[1];
         if constexpr (M == 1) {
             const double* __restrict__ prm = params + it[1];
             const uint32_t slice_bits = (uint32_t)it[2];
             for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
                 build_gate<R>(gates + (size_t)g * kGateWords, prm, my_angles, kParamRegs, slice_bits, lds, pool);
+        } else if (same_angles) {
+            // items of one request: one FP64 evaluation per gate for the group
+            uint32_t sb[M];
+#pragma unroll
+            for (int i = 0; i < M; ++i) sb[i] = (uint32_t)it[i * QB_ITEM_WORDS + 2];
+            for (int32_t g = lane; g < n_gates; g += ITEM_THREADS)
+                build_gate_group<R, M>(gates + (size_t)g * kGateWords, params + it[1], my_angles, kParamRegs, sb,
+                                       lds_v, pool);
         } else {
             // lane t builds gate t / M of item t % M
             for (int32_t t = lane; t < n_gates * M; t += ITEM_THREADS) {
@@ -1309,7 +1366,7 @@ struct ExecCfg {
 };
 
 // Calls f(ExecCfg<...>{}) for a launch shape.  Groups of 2 items run complex64
-// items of 64, 128 or 256 threads; every other shape runs single items.
+// items of 64 to 512 threads; every other shape runs single items.
 template <typename F>
 int exec_dispatch(int32_t dtype, int item_threads, int group, bool glb, F&& f) {
 #define QB_EXEC_T(R, G, M)                                             \
@@ -1317,11 +1374,11 @@ int exec_dispatch(int32_t dtype, int item_threads, int group, bool glb, F&& f) {
     case 64: return f(ExecCfg<R, 64, G, M>{});                         \
     case 128: return f(ExecCfg<R, 128, G, M>{});                       \
     case 256: return f(ExecCfg<R, 256, G, M>{});                       \
+    case 512: return f(ExecCfg<R, 512, G, M>{});                       \
     default: break;                                                    \
     }
 #define QB_EXEC_T1(R, G)                                               \
     QB_EXEC_T(R, G, 1)                                                 \
-    if (item_threads == 512) return f(ExecCfg<R, 512, G, 1>{});        \
     if (item_threads == 1024) return f(ExecCfg<R, 1024, G, 1>{});
     if (group == 1) {
         if (dtype == QB_C128) {
@@ -1333,7 +1390,7 @@ int exec_dispatch(int32_t dtype, int item_threads, int group, bool glb, F&& f) {
     }
     if (group == 2 && dtype == QB_C64) {
         if (glb) { QB_EXEC_T(float, true, 2) } else { QB_EXEC_T(float, false, 2) }
-        return set_error(QB_ERR_INVALID, "groups of 2 items run item_threads 64, 128 or 256, not %d", item_threads);
+        return set_error(QB_ERR_INVALID, "groups of 2 items run item_threads 64, 128, 256 or 512, not %d", item_threads);
     }
 #undef QB_EXEC_T1
 #undef QB_EXEC_T
