@@ -111,6 +111,30 @@ __device__ __forceinline__ cf32x2 cadd(cf32x2 a, cf32x2 b) {
     return r;
 }
 
+// acc + a b in four fused multiply-adds (instead of a product and an add)
+template <typename V>
+__device__ __forceinline__ V cfma(V acc, V a, V b) {
+    V r;
+    r.x = __builtin_fmaf(-a.y, b.y, __builtin_fmaf(a.x, b.x, acc.x));
+    r.y = __builtin_fmaf(a.y, b.x, __builtin_fmaf(a.x, b.y, acc.y));
+    return r;
+}
+template <>
+__device__ __forceinline__ cf64 cfma(cf64 acc, cf64 a, cf64 b) {
+    cf64 r;
+    r.x = __builtin_fma(-a.y, b.y, __builtin_fma(a.x, b.x, acc.x));
+    r.y = __builtin_fma(a.y, b.x, __builtin_fma(a.x, b.y, acc.y));
+    return r;
+}
+__device__ __forceinline__ cf32x2 cfma(cf32x2 acc, cf32x2 a, cf32x2 b) {
+    cf32x2 r;
+    r[0] = __builtin_fmaf(-a[1], b[1], __builtin_fmaf(a[0], b[0], acc[0]));
+    r[1] = __builtin_fmaf(a[1], b[0], __builtin_fmaf(a[0], b[1], acc[1]));
+    r[2] = __builtin_fmaf(-a[3], b[3], __builtin_fmaf(a[2], b[2], acc[2]));
+    r[3] = __builtin_fmaf(a[3], b[2], __builtin_fmaf(a[2], b[3], acc[3]));
+    return r;
+}
+
 // the executor's element type for items of precision R run in groups of M
 template <typename R, int M> struct PkOf;
 template <typename R> struct PkOf<R, 1> { using T = typename Cx<R>::V; };
@@ -804,21 +828,18 @@ __device__ __forceinline__ void gather_op_n(const GatherLane& g, QB_LDS P* lds) 
         x[j] = lds[g.w[1 + j] & 0xffffu];
         y[j] = lds[g.w[1 + j] >> 16];
     }
-    P p, q;
+    // the summed term's last product is fused into the sum
+    P out;
     if (NI == 1) {
-        p = x[0];
-        q = y[0];
+        out = cadd(x[0], y[0]);
     } else if (NI == 2) {
-        p = cmul(x[0], x[1]);
-        q = cmul(y[0], y[1]);
+        out = cfma(cmul(x[0], x[1]), y[0], y[1]);
     } else if (NI == 3) {
-        p = cmul(cmul(x[0], x[1]), x[2]);
-        q = cmul(cmul(y[0], y[1]), y[2]);
+        out = cfma(cmul(cmul(x[0], x[1]), x[2]), cmul(y[0], y[1]), y[2]);
     } else {
-        p = cmul(cmul(x[0], x[1]), cmul(x[2], x[3]));
-        q = cmul(cmul(y[0], y[1]), cmul(y[2], y[3]));
+        out = cfma(cmul(cmul(x[0], x[1]), cmul(x[2], x[3])), cmul(y[0], y[1]), cmul(y[2], y[3]));
     }
-    lds[g.w[0]] = cadd(p, q);
+    lds[g.w[0]] = out;
 }
 
 // n: the op's active lanes, ni: its input count (both decoded from the record
