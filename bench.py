@@ -572,6 +572,11 @@ def main():
     ap.add_argument("--serve-ordering4", default="greedy", choices=("greedy", "rgreedy"),
                     help="with --serve: elimination order of the depth-4 requests (--ordering sets "
                          "depths 1-3)")
+    ap.add_argument("--serve-shape", default=None,
+                    help="served depths 1-3: launch shape item_threads,lds_capacity,lds_tensor_limit[,group]")
+    ap.add_argument("--serve-tw", type=int, default=11, help="served depths 1-3: slice below this treewidth")
+    ap.add_argument("--serve-shape4", default=None,
+                    help="served depth 4: launch shape item_threads,lds_capacity,lds_tensor_limit[,group]")
     ap.add_argument("--serve-tw4", type=int, default=16,
                     help="with --serve: depth-4 lightcones are sliced below this treewidth")
     ap.add_argument("--serve-rg4", type=int, default=16,
@@ -603,7 +608,12 @@ def main():
     if args.serve:
         rates = [float(x) for x in args.serve_rates.split(",")]
         ordering = args.ordering or "greedy"
+        from qtensor_amd.batched import LaunchShape
+
+        def _shape(text):
+            return None if text is None else LaunchShape(*[int(x) for x in text.split(",")])
         res = serve_bench(rates, args.serve_requests, args.serve_pool, ordering=ordering,
+                          shape=_shape(args.serve_shape), max_tw=args.serve_tw, shape4=_shape(args.serve_shape4),
                           ordering4=args.serve_ordering4, cold_rate=20.0, cold_requests=40,
                           max_tw4=args.serve_tw4, rg_repeats4=args.serve_rg4,
                           max_slice_vars=args.serve_max_slice_vars,
@@ -836,12 +846,19 @@ def main():
                                    np.max(np.abs(t_cli - oracle_vals) / np.maximum(1.0, np.abs(oracle_vals))))}
         print(f"[bench] CLI composer: {n_terms / cli_s:.1f} terms/s", file=sys.stderr, flush=True)
     if not args.no_baselines:
+        # the two host-bound legs (the plain backend and the numpy oracle on the
+        # host) run under the same collector state: what the earlier legs left
+        # alive is frozen out of the collector's scans for both
+        import gc
+        gc.collect()
+        gc.freeze()
         n_plain = min(args.plain_edges, n_terms)
         pv, pdt = plain_form(G, gamma, beta, n_plain)
         out["plain"] = {"value": round(n_plain / pdt, 2), "unit": "terms/s", "edges": n_plain,
                         "max_abs_err_vs_batched": float(np.max(np.abs(np.array(pv) - terms64[:n_plain])))}
         print(f"[bench] plain: {n_plain / pdt:.2f} terms/s", file=sys.stderr, flush=True)
         cb = cpu_baseline(G, gamma, beta, args.cpu_budget)
+        gc.unfreeze()
         cv = np.array(cb["vals"])
         n = cb["n"]
         out["cpu_baseline"] = {

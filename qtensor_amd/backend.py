@@ -9,7 +9,10 @@ Drop-in for ``NumpyBackend`` (``qtensor/ProcessingFrameworks.py:13-37``):
 * ``process_bucket(bucket, no_sum=False)`` launches one ``qb_step`` kernel: the
   broadcast product of the bucket's tensors over the union of their indices,
   summed over the bucket variable.  Returns a new ``Tensor('E<id>')`` whose
-  data is a device buffer; inputs are not modified.
+  data is a device buffer; inputs are not modified.  A bucket of one tensor
+  with nothing to add up (no sum, or a sum over a size-1 variable: the
+  ket/bra wires) returns a view of that tensor's device data instead, with no
+  launch.
 * ``get_result_data(result)`` copies the result to the host (numpy).
 
 Device memory comes from PyTorch's caching allocator; kernels run on the
@@ -54,6 +57,12 @@ class DeviceSpan:
     def dtype(self):
         return self.base.dtype
 
+    def reshaped(self, shape):
+        """The same elements under another shape of the same size."""
+        r = DeviceSpan.__new__(DeviceSpan)
+        r.base, r.offset, r.shape, r.ptr = self.base, self.offset, shape, self.ptr
+        return r
+
     def tensor(self):
         """A torch view of the span."""
         n = 1
@@ -74,6 +83,10 @@ class DeviceSpan:
 
 _SLAB_ELEMS = 1 << 20       # elements per result slab
 _SLAB_MAX_OUT = 1 << 14     # results up to this size come from slabs
+
+
+_FULL = (None, None, None)
+_PERM1, _PERM2, _PERM2R = (0,), (0, 1), (1, 0)
 
 
 def _bounds_key(b):
@@ -110,6 +123,7 @@ class HipBackend:
             torch = _torch()
             if not torch.cuda.is_available():
                 raise RuntimeError("HipBackend needs a ROCm GPU (torch.cuda.is_available() is False)")
+            self._torch = torch
             self._lib = _lib.init(self.device_index)
             self.torch_dtype = torch.complex64 if self.dtype == "complex64" else torch.complex128
             self.np_dtype = np.complex64 if self.dtype == "complex64" else np.complex128
@@ -130,11 +144,11 @@ class HipBackend:
     # -- BucketBackend API
     def get_sliced_buckets(self, buckets, data_dict, slice_dict):
         self._ensure()
-        torch = _torch()
+        torch = self._torch
         staged, flat, total = [], [], 0
         cache = self._slice_cache
         bound_of = {int(v): _bounds_key(b) for v, b in slice_dict.items()}
-        full = (None, None, None)
+        full = _FULL
         content = {}          # id -> values of the gate arrays seen in this call
         for bucket in buckets:
             nb = []
@@ -143,10 +157,18 @@ class HipBackend:
                 # transposed and sliced host array is computed once per pattern,
                 # keyed by the gate's values rather than its data_key
                 src = data_dict[t.data_key]
-                ids = [v.identity for v in t.indices]
-                perm = tuple(sorted(range(len(ids)), key=ids.__getitem__))
-                bounds = tuple(bound_of.get(ids[a], full) for a in perm)
-                small = isinstance(src, np.ndarray) and src.size <= 256
+                tix = t.indices
+                n_ax = len(tix)
+                # the tensor's axes in ascending variable order
+                if n_ax == 1:
+                    perm = _PERM1
+                elif n_ax == 2:
+                    perm = _PERM2 if tix[0].identity < tix[1].identity else _PERM2R
+                else:
+                    ids = [v.identity for v in tix]
+                    perm = tuple(sorted(range(n_ax), key=ids.__getitem__))
+                bounds = tuple([bound_of.get(tix[a].identity, full) for a in perm])
+                small = type(src) is np.ndarray and src.size <= 256
                 hit = None
                 if small:
                     val = content.get(id(src))
@@ -159,38 +181,39 @@ class HipBackend:
                     data = np.ascontiguousarray(data, dtype=self.np_dtype)
                     sizes = tuple(v.size for v in indices)
                     kept = tuple(a for a, b in zip(perm, bounds) if not isinstance(b, int))
-                    hit = (data, kept, sizes)
+                    hit = (data.reshape(-1), tuple(zip(kept, sizes)), data.shape)
                     if small and len(cache) < 65536:
                         cache[key] = hit
-                data, kept, sizes = hit
-                indices = tuple(t.indices[a].copy(size=sz) if t.indices[a].size != sz else t.indices[a]
-                                for a, sz in zip(kept, sizes))
-                nb.append((t, indices, data, total))
-                flat.append(data.reshape(-1))
-                total += data.size
+                fdata, kept_sizes, shape = hit
+                indices = tuple([tix[a] if tix[a].size == sz else tix[a].copy(size=sz) for a, sz in kept_sizes])
+                nb.append((t, indices, shape, total, fdata.size))
+                flat.append(fdata)
+                total += fdata.size
             staged.append(nb)
         if not total:
-            return [[t.copy(indices=ix, data=None) for t, ix, _, _ in nb] for nb in staged]
+            return [[t.copy(indices=ix, data=None) for t, ix, _, _, _ in nb] for nb in staged]
         host = np.concatenate(flat)
         dev = torch.from_numpy(host).to(self.device, non_blocking=False)
         base_ptr = dev.data_ptr()
         elem = host.itemsize
-        return [[t.copy(indices=ix, data=DeviceSpan(dev, base_ptr, off, d.shape, elem) if d.size else None)
-                 for t, ix, d, off in nb] for nb in staged]
+        return [[Tensor(t.name, ix, t.data_key, DeviceSpan(dev, base_ptr, off, shape, elem) if n else t.data)
+                 for t, ix, shape, off, n in nb] for nb in staged]
 
     def _descriptor(self):
+        """This thread's step descriptor and input-pointer array, with their
+        addresses."""
         tls = self._tls
         try:
-            return tls.desc, tls.ptrs
+            return tls.step
         except AttributeError:
-            tls.desc = _lib.StepDesc()
-            tls.ptrs = (ctypes.c_void_p * _lib.QB_MAX_STEP_INPUTS)()
-            return tls.desc, tls.ptrs
+            desc = _lib.StepDesc()
+            ptrs = (ctypes.c_void_p * _lib.QB_MAX_STEP_INPUTS)()
+            tls.step = (desc, ptrs, ctypes.addressof(desc), ctypes.addressof(ptrs))
+            return tls.step
 
     def process_bucket(self, bucket, no_sum=False):
         if not self._ready:
             self._ensure()
-        torch = _torch()
         if len(bucket) > _lib.QB_MAX_STEP_INPUTS:
             # multiply the head of the bucket first (no sum), then finish the step
             head = self.process_bucket(bucket[:_lib.QB_MAX_STEP_INPUTS], no_sum=True)
@@ -200,19 +223,17 @@ class HipBackend:
             for v in t.indices:
                 union[v.identity] = v
         U = [union[k] for k in sorted(union)]
-        rank_u = 0
-        for v in U:
+        # union axis a (size 2) is bit rank_u-1-a of a union offset: the last
+        # size-2 axis is bit 0
+        bit, rank_u = {}, 0
+        for v in reversed(U):
             if v.size == 2:
+                bit[v.identity] = 1 << rank_u
                 rank_u += 1
             elif v.size != 1:
                 raise ValueError(f"variable {v} has size {v.size}; only sizes 1 and 2 are supported")
         if rank_u > _lib.QB_MAX_RANK:
             raise ValueError(f"bucket union rank {rank_u} exceeds {_lib.QB_MAX_RANK}")
-        bit, b = {}, rank_u
-        for v in U:
-            if v.size == 2:
-                b -= 1
-                bit[v.identity] = 1 << b
         if U:
             tag = U[0].identity
             do_sum = not no_sum
@@ -220,7 +241,17 @@ class HipBackend:
             tag, do_sum = "f", False
         sum_flag = 1 if (do_sum and U and U[0].size == 2) else 0
         out_indices = U[1:] if do_sum else U
-        desc, ptrs = self._descriptor()
+        if sum_flag == 0 and len(bucket) == 1:
+            # one tensor and nothing to add up (no sum, or a sum over a size-1
+            # variable): the result holds the input's elements in the input's
+            # order, so it is a view of the input's device data (no launch);
+            # data are never written after they are made, so sharing is safe
+            data = bucket[0].data
+            if type(data) is DeviceSpan:
+                return Tensor(f"E{tag}", out_indices, data=data.reshaped(tuple(v.size for v in out_indices)))
+            if isinstance(data, self._torch.Tensor) and data.is_cuda and data.dtype == self.torch_dtype:
+                return Tensor(f"E{tag}", out_indices, data=data.reshape([v.size for v in out_indices]))
+        desc, ptrs, desc_addr, ptrs_addr = self._descriptor()
         desc.n_inputs = len(bucket)
         desc.rank_u = rank_u
         desc.sum = sum_flag
@@ -235,6 +266,7 @@ class HipBackend:
             if type(data) is DeviceSpan:
                 ptrs[j] = data.ptr
                 continue
+            torch = self._torch
             if not isinstance(data, torch.Tensor) or not data.is_cuda:
                 data = torch.as_tensor(np.asarray(data), dtype=self.torch_dtype, device=self.device)
             elif data.dtype != self.torch_dtype:
@@ -251,9 +283,9 @@ class HipBackend:
             out = self._slab_span(n_out, tuple(v.size for v in out_indices), stream)
             out_ptr = out.ptr
         else:
-            out = torch.empty([v.size for v in out_indices], dtype=self.torch_dtype, device=self.device)
+            out = self._torch.empty([v.size for v in out_indices], dtype=self.torch_dtype, device=self.device)
             out_ptr = out.data_ptr()
-        rc = self._qb_step(ctypes.addressof(desc), ctypes.addressof(ptrs), out_ptr, stream)
+        rc = self._qb_step(desc_addr, ptrs_addr, out_ptr, stream)
         if rc:
             _lib.check(rc)
         return Tensor(f"E{tag}", out_indices, data=out)
