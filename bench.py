@@ -375,16 +375,18 @@ def oracle_fixture(name, G):
     return vals
 
 
-def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
+def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14, ordering="rgreedy"):
     """C4: Erdos-Renyi N=64, average degree 4 (seed 4), p=3, n_sets angle sets from
     default_rng(4); every edge term in complex64 (FP64 gate construction, FP64
     compensated slice sums) against the same plans in complex128; the complex128
     path is itself checked against the numpy oracle on the edges with treewidth
-    <= oracle_tw."""
+    <= oracle_tw.  ``ordering``: "greedy" (min-degree) or "rgreedy" (best of 64
+    randomized-greedy samples at temperature 0.3 and the greedy order, which
+    needs about a third of the greedy order's work on these lightcones)."""
     import networkx as nx
     import numpy as np
     from qtensor_amd.composer import DefaultQAOAComposer
-    from qtensor_amd.ordering import greedy_peo
+    from qtensor_amd.ordering import greedy_peo, rgreedy_best
     from qtensor_amd.tensornet import QtreeTensorNet
     from qtensor_amd.wide import WideEngine, compile_wide, slice_vars_for_order
     G = nx.erdos_renyi_graph(64, 4 / 63, seed=4)
@@ -396,7 +398,7 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
         comp = DefaultQAOAComposer(G, gamma=[0.0] * 3, beta=[0.0] * 3)
         comp.energy_expectation_lightcone(e)
         graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
-        order, degs = greedy_peo(graph)
+        order, degs = greedy_peo(graph) if ordering == "greedy" else rgreedy_best(graph, temp=0.3, repeats=64)
         s = max(0, max(degs) - max_tw_slice)
         plans.append(compile_wide(comp.circuit, order=order,
                                   par_ids=slice_vars_for_order(graph, order, s) if s else []))
@@ -434,7 +436,7 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14):
         print(f"[precision] set {si}: max |dZZ| {abs_err.max():.2e}, |dC| {abs(C64 - C128):.2e}", file=sys.stderr, flush=True)
     worst = max(per_set, key=lambda r: r["edge_abs_err_max"])
     return {"workload": "erdos_renyi_n64_d4_seed4_p3", "edges": G.number_of_edges(), "angle_sets": n_sets,
-            "composer": "DefaultQAOAComposer (ZZ + simplify)", "treewidth_max": max(tws),
+            "composer": "DefaultQAOAComposer (ZZ + simplify)", "ordering": ordering, "treewidth_max": max(tws),
             "slice_to_treewidth": max_tw_slice, "plan_seconds": round(plan_s, 1),
             "gpu_seconds_c64": round(t64, 2), "gpu_seconds_c128": round(t128, 2),
             "step_model_GB_per_set": round(sum(wp.plan.bytes_step_model * wp.n_slices for wp in plans) / 1e9, 1),
@@ -600,7 +602,7 @@ def main():
     from qtensor_amd.batched import LightconeEngine
 
     if args.precision:
-        res = precision_study(n_sets=args.precision_sets)
+        res = precision_study(n_sets=args.precision_sets, ordering=args.ordering or "rgreedy")
         print(json.dumps({"metric": "complex64 edge-term error vs complex128", "value": res["edge_abs_err_max"],
                           "unit": "abs error", "higher_is_better": False, "n_gpus": 1, "dtype": "complex64",
                           "data": "synthetic", "config": {"workload": res["workload"]}, "precision": res}), flush=True)
