@@ -409,14 +409,12 @@ def precision_study(n_sets=16, max_tw_slice=26, oracle_tw=14, ordering="rgreedy"
     oracle_err = 0.0
     oracle_edges = [k for k, tw in enumerate(tws) if tw <= oracle_tw]
     for si, (gamma, beta) in enumerate(sets):
-        z64, z128 = [], []
-        for wp in plans:
-            v, dt = e64.run(wp, gamma + beta, timing=True)
-            z64.append(v)
-            t64 += dt
-            v, dt = e128.run(wp, gamma + beta, timing=True)
-            z128.append(v)
-            t128 += dt
+        # all edges of one angle set at once: their (edge, slice) networks over
+        # the wide engine's four streams
+        z64, dt = e64.run_many(plans, gamma + beta, timing=True)
+        t64 += dt
+        z128, dt = e128.run_many(plans, gamma + beta, timing=True)
+        t128 += dt
         z64, z128 = np.array(z64), np.array(z128)
         abs_err = np.abs(z64 - z128)
         rel_err = abs_err / np.maximum(np.abs(z128), 1e-300)

@@ -219,6 +219,100 @@ class WideEngine:
             return val, ev0.elapsed_time(ev1) * 1e-3
         return val
 
+    def run_many(self, wplans, params, stream=None, timing=False, n_streams=4):
+        """Values of several scalar wide plans for one angle vector (a list of
+        complex, and the device time with ``timing``).  Every (plan, slice) is
+        independent: they are spread over ``n_streams`` streams, longest first
+        onto the least loaded stream (cost: the plan's executed-op bytes), each
+        stream with one arena large enough for its plans, so the launch-bound
+        small steps of many networks overlap; one segmented compensated FP64
+        reduction then gives each plan's sum over its slices."""
+        import torch
+        lib = _lib.init(self.device_index)
+        dev = torch.device("cuda", self.device_index)
+        tdtype = torch.complex64 if self.dtype == "complex64" else torch.complex128
+        if not wplans:
+            return ([], 0.0) if timing else []
+        works, seg = [], [0]
+        for i, wp in enumerate(wplans):
+            if int(wp.plan.words[11]) != 0:
+                raise ValueError("run_many takes scalar plans (no open variables)")
+            for s in range(wp.n_slices):
+                works.append((i, s))
+            seg.append(len(works))
+        n_streams = max(1, min(n_streams, len(works)))
+        cost = [float(max(wp.plan.glb_need, 1)) for wp in wplans]
+        order = sorted(range(len(works)), key=lambda w: -cost[works[w][0]])
+        load, per_stream = [0.0] * n_streams, [[] for _ in range(n_streams)]
+        for w in order:
+            k = min(range(n_streams), key=load.__getitem__)
+            per_stream[k].append(w)
+            load[k] += cost[works[w][0]]
+        words, dev_words = [], []
+        for wp in wplans:
+            cached = getattr(wp, "_dev_words", None)
+            if cached is None or cached[0] != self.device_index:
+                w_np = np.ascontiguousarray(wp.plan.words, dtype=np.int32)
+                cached = (self.device_index, w_np, torch.from_numpy(w_np).to(dev))
+                wp._dev_words = cached
+            words.append(cached[1])
+            dev_words.append(cached[2])
+        arenas = [torch.empty(max([max(wplans[works[w][0]].plan.glb_need, 1) for w in ws] + [1]), dtype=tdtype,
+                              device=dev) for ws in per_stream]
+        prm = torch.zeros(len(params) + 8, dtype=torch.float64)
+        prm[:len(params)] = torch.tensor([float(x) for x in params], dtype=torch.float64)
+        prm = prm.to(dev)
+        out = torch.zeros(2 * len(works), dtype=torch.float64, device=dev)
+        seg_dev = torch.tensor(seg, dtype=torch.int32, device=dev)
+        term = torch.zeros(2 * len(wplans), dtype=torch.float64, device=dev)
+        total = torch.zeros(2, dtype=torch.float64, device=dev)
+        main = stream if stream is not None else torch.cuda.current_stream(dev)
+        streams = [main] + [torch.cuda.Stream(device=dev) for _ in range(n_streams - 1)]
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev0.record(main)
+        for st in streams[1:]:
+            st.wait_event(ev0)
+        qb_dtype = _lib.QB_C64 if self.dtype == "complex64" else _lib.QB_C128
+
+        def issue(k):
+            d = _lib.WideDesc()
+            d.params = prm.data_ptr()
+            d.arena = arenas[k].data_ptr()
+            d.arena_elems = arenas[k].numel()
+            d.dtype = qb_dtype
+            d.out_stride = 1
+            h = ctypes.c_void_p(streams[k].cuda_stream)
+            for w in per_stream[k]:
+                i, s = works[w]
+                d.plan_host = words[i].ctypes.data
+                d.plan_words = words[i].size
+                d.plan_dev = dev_words[i].data_ptr()
+                d.slice_bits = s
+                d.out = out.data_ptr() + 16 * w
+                _lib.check(lib.qb_wide_execute(ctypes.byref(d), h))
+
+        if n_streams == 1:
+            issue(0)
+        else:
+            futs = [self._pool(n_streams).submit(issue, k) for k in range(n_streams)]
+            for f in futs:
+                f.result()
+        for s_ in streams[1:]:
+            ev = torch.cuda.Event()
+            ev.record(s_)
+            main.wait_event(ev)
+        _lib.check(lib.qb_reduce_terms(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(seg_dev.data_ptr()),
+                                       len(wplans), ctypes.c_void_p(term.data_ptr()),
+                                       ctypes.c_void_p(total.data_ptr()), ctypes.c_void_p(main.cuda_stream)))
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev1.record(main)
+        ev1.synchronize()
+        t = term.cpu().numpy()
+        vals = [complex(t[2 * i], t[2 * i + 1]) for i in range(len(wplans))]
+        if timing:
+            return vals, ev0.elapsed_time(ev1) * 1e-3
+        return vals
+
     _GRAPH_PARAMS = 256        # angles a captured plan can take
 
     def _buffers(self, wplan, n_streams, n_params):

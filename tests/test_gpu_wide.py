@@ -154,3 +154,32 @@ def test_wide_graph_replays_match_eager(gpu):
         torch.cuda.synchronize()
         replay = eng.run(wp, prm, graph=True)
         assert abs(replay - eager) <= 1e-13, (replay, eager)
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex64", 1e-5), ("complex128", 1e-11)])
+def test_wide_run_many_matches_run(gpu, dtype, tol):
+    """Several wide plans at once (their (plan, slice) networks spread over the
+    streams) give each plan's value: equal to running the plans one by one
+    (the same slices, reduced in the same order) and to the statevector."""
+    G = nx.random_regular_graph(3, 14, seed=2)
+    rng = np.random.default_rng(7)
+    g, b = list(rng.uniform(0, np.pi, 2)), list(rng.uniform(0, np.pi / 2, 2))
+    sv = statevector.zz_expectations(G, g, b)
+    plans, refs = [], []
+    for k, e in enumerate(list(G.edges())[:6]):
+        comp = QtreeQAOAComposer(G, gamma=g, beta=b)
+        comp.energy_expectation_lightcone(e)
+        if k % 2:
+            graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+            order = forced_width_order(graph, target_tw=12, temp=3.0, seed=k)
+            plans.append(compile_wide(comp.circuit, order=order, par_ids=slice_vars_for_order(graph, order, 2)))
+        else:
+            plans.append(compile_wide(comp.circuit))
+        refs.append(sv[e])
+    assert any(wp.n_slices > 1 for wp in plans)
+    eng = WideEngine(dtype=dtype)
+    vals, dt = eng.run_many(plans, g + b, timing=True)
+    assert dt > 0 and len(vals) == len(plans)
+    for wp, v, r in zip(plans, vals, refs):
+        assert v == eng.run(wp, g + b)
+        assert abs(v - r) <= tol
