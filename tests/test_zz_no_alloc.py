@@ -71,6 +71,18 @@ def test_noalloc_matches_execute_on_shrinking_calls(gpu):
     assert np.all(np.abs(terms.imag) < 1e-5)
 
 
+def test_noalloc_pairs_with_scratch(gpu):
+    """Pairs of items per block (interleaved LDS data and scratch slots): a
+    plan set with intermediates in global scratch, then a smaller one, each
+    equal to qb_execute's pairs."""
+    for n, seed in ((30, 5), (12, 6)):
+        b = _batch(nx.random_regular_graph(3, n, seed=seed), 2, LaunchShape(128, 4096, 16, group=2), seed=seed)
+        assert b.group == 2 and b.scratch_elems > 0
+        got = _noalloc_items(b)
+        assert np.all(np.isfinite(got))
+        assert np.array_equal(got, _reference_items(b))
+
+
 def test_noalloc_with_queue_tickets(gpu):
     """More items than resident blocks: later items come from the static
     queue ticket, which the call resets with a fill kernel."""
