@@ -48,7 +48,12 @@ EXEC_TRAFFIC_BYTES = int(13372.3 * 1024 * 2.0 + 34.8 * 1024 * 1.0)
 def workload(name):
     import networkx as nx
     import numpy as np
-    if name == "c2":
+    if name == "c1":
+        G = nx.random_regular_graph(3, 10, seed=42)
+        p = 1
+        gamma, beta = [np.pi / 3], [np.pi / 2]
+        desc = "qaoa_p1_3reg_n10_seed42"
+    elif name == "c2":
         G = nx.random_regular_graph(3, 256, seed=1)
         rng = np.random.default_rng(2)
         p = 3
