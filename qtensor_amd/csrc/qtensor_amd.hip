@@ -454,8 +454,9 @@ __global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a, const int
 }
 
 // Two-input fused op with register blocking: out[o] = sum_x A[a(o, x)] B[b(o, x)].
-// Up to two high output bits that A carries and B does not (regA) and up to two
-// that B carries and A does not (regB) are handled in registers: per summed
+// Up to three (complex64; two for complex128) high output bits that A carries
+// and B does not (regA) and as many that B carries and A does not (regB) are
+// handled in registers: per summed
 // assignment a thread loads 2^RA values of A and 2^RB of B and forms their
 // 2^(RA+RB) products, instead of two loads per output.  Blocks visit the
 // remaining high bits hk; hi = deposit(hk) over the non-register high bits
@@ -463,8 +464,8 @@ __global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a, const int
 constexpr int kFusedDepRuns = 5;
 
 struct Fused2Args {
-    uint32_t roff_a[4], roff_b[4];   // offset in A (B) of register pattern p of regA (regB)
-    uint32_t rdep_a[4], rdep_b[4];   // high-bit pattern of register pattern p
+    uint32_t roff_a[8], roff_b[8];   // offset in A (B) of register pattern p of regA (regB)
+    uint32_t rdep_a[8], rdep_b[8];   // high-bit pattern of register pattern p
     uint32_t dep_run[kFusedDepRuns];
     uint32_t n_dep, hk_bits;
 };
@@ -1688,12 +1689,31 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
         uint32_t ubit[32], n_ob = 0;
         for (uint32_t u = 0; u < 32; ++u)
             if ((omask >> u) & 1u) ubit[n_ob++] = u;
-        uint32_t reg_a[2], reg_b[2], ra = 0, rb = 0;
-        for (uint32_t b = hi_bits; b-- > 0;) {          // the highest bits first
-            const uint32_t u = ubit[b + lo_bits];
-            const bool in_a = (a.mask[0] >> u) & 1u, in_b = (a.mask[1] >> u) & 1u;
-            if (in_a && !in_b && ra < 2) reg_a[ra++] = b;
-            else if (in_b && !in_a && rb < 2) reg_b[rb++] = b;
+        // complex64: up to three register bits per input (an 8 x 8 tile of
+        // outputs per thread: a quarter of a load per output and summed
+        // assignment); complex128 holds at most 4 x 4 in registers.  Fewer
+        // register bits when the remaining high bits would split into more
+        // deposit runs than the kernel takes.
+        uint32_t reg_a[3], reg_b[3], ra = 0, rb = 0;
+        for (uint32_t max_reg = dtype == QB_C128 ? 2u : 3u; max_reg > 0; --max_reg) {
+            ra = rb = 0;
+            uint32_t regmask = 0;
+            for (uint32_t b = hi_bits; b-- > 0;) {          // the highest bits first
+                const uint32_t u = ubit[b + lo_bits];
+                const bool in_a = (a.mask[0] >> u) & 1u, in_b = (a.mask[1] >> u) & 1u;
+                if (in_a && !in_b && ra < max_reg) {
+                    reg_a[ra++] = b;
+                    regmask |= 1u << b;
+                } else if (in_b && !in_a && rb < max_reg) {
+                    reg_b[rb++] = b;
+                    regmask |= 1u << b;
+                }
+            }
+            const uint32_t keep = ((1u << hi_bits) - 1u) & ~regmask;
+            uint32_t runs = 0;
+            for (uint32_t b = 0; b < hi_bits; ++b)
+                if (((keep >> b) & 1u) && (b == 0 || !((keep >> (b - 1)) & 1u))) ++runs;
+            if (runs <= (uint32_t)kFusedDepRuns) break;
         }
         if (ra + rb > 0) {
             Fused2Args e;
@@ -1747,8 +1767,14 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
                     hipLaunchKernelGGL((fused2_kernel<float, RA_, RB_>), dim3(grid2), dim3(256), 0, st, a, lo_tab,  \
                                        x_tab, e);                                                        \
             }
+#define QB_FUSED2_CASE_F(RA_, RB_)                                                                       \
+            if (ra == RA_ && rb == RB_)                                                                  \
+                hipLaunchKernelGGL((fused2_kernel<float, RA_, RB_>), dim3(grid2), dim3(256), 0, st, a, lo_tab, x_tab, e);
             QB_FUSED2_CASE(0, 1) QB_FUSED2_CASE(0, 2) QB_FUSED2_CASE(1, 0) QB_FUSED2_CASE(1, 1)
             QB_FUSED2_CASE(1, 2) QB_FUSED2_CASE(2, 0) QB_FUSED2_CASE(2, 1) QB_FUSED2_CASE(2, 2)
+            QB_FUSED2_CASE_F(0, 3) QB_FUSED2_CASE_F(1, 3) QB_FUSED2_CASE_F(2, 3) QB_FUSED2_CASE_F(3, 0)
+            QB_FUSED2_CASE_F(3, 1) QB_FUSED2_CASE_F(3, 2) QB_FUSED2_CASE_F(3, 3)
+#undef QB_FUSED2_CASE_F
 #undef QB_FUSED2_CASE
             QB_HIP(hipGetLastError());
             return QB_OK;
