@@ -548,7 +548,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--plain-edges", type=int, default=24)
+    ap.add_argument("--plain-edges", type=int, default=96)
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default run")
     ap.add_argument("--shape", default=None,
