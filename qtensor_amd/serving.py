@@ -245,7 +245,8 @@ class EnergyServer:
             # e.g. more slice variables than a work item can carry: the
             # reference's planner stops the same way ("Estimated OOM")
             raise MemoryError(f"Estimated OOM: {exc}") from exc
-        wl.shape_key = eng.shape_for(p).key()
+        wl.shape = eng.shape_for(p)
+        wl.shape_key = wl.shape.key()
         if self.max_slice_vars is not None:
             need = max((pl.n_sliced for pl in wl.plans.values()), default=0)
             if need > self.max_slice_vars:
@@ -256,11 +257,22 @@ class EnergyServer:
     def _plan(self, G, p, k):
         """Plan one (graph, depth) on its engine's planner thread; with an
         upgrade engine for this depth, then plan it again with that one on its
-        own thread, and serve later requests with the upgraded plans."""
-        wl = self._plan_with(self.engine_for(p), G, p)
+        own thread, and serve later requests with the upgraded plans.  When the
+        quick plans exceed the limits (slice variables), the upgrade engine's
+        plans are made at once instead and the request waits for them."""
+        up = self.upgrades.get(p)
+        try:
+            wl = self._plan_with(self.engine_for(p), G, p)
+        except MemoryError:
+            if up is None:
+                raise
+            wl = self._plan_with(up, G, p)
+            self._workloads[k] = wl
+            self.upgraded += 1
+            self._progress.set()
+            return wl
         self._workloads[k] = wl
         self._progress.set()
-        up = self.upgrades.get(p)
         if up is not None:
             self._planners[id(up)].submit(self._upgrade, up, G, p, k)
         return wl
@@ -421,8 +433,11 @@ class EnergyServer:
             costs.append(cost)
             sizes.append(seg_sizes)
             lds_need, glb_need = max(lds_need, ln), max(glb_need, gn)
+        # the launch shape the batch's plans were compiled for (every request of
+        # a batch has the same shape key; an upgrade engine may differ from the
+        # depth's usual engine)
         p0 = len(batch[0].gamma)
-        shape = self.engine_for(p0).shape_for(p0)
+        shape = getattr(batch[0].workload, "shape", None) or self.engine_for(p0).shape_for(p0)
         group = max(1, int(shape.group))
         # items of one plan (requests on one graph) share a block in groups
         items, _ = group_items(np.concatenate(parts), np.concatenate(costs), group,

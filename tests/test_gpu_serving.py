@@ -111,3 +111,33 @@ def test_plan_upgrades_and_slice_cap(gpu):
             capped.submit(G, g, b, key="G").result(timeout=120)
     finally:
         capped.stop()
+
+
+def test_upgrade_engine_plans_when_quick_plans_exceed_the_cap(gpu):
+    """When a request's quick plans need more slice variables than
+    max_slice_vars allows, the upgrade engine plans it at once (its own launch
+    shape, here 128-thread items instead of 64) and the request is served with
+    those plans, matching the statevector; without an upgrade engine the same
+    request is rejected."""
+    G = nx.random_regular_graph(3, 16, seed=11)
+    g, b = [0.25, 0.7], [0.45, 0.15]
+    quick = LightconeEngine(shape=LaunchShape(64, 4096, 2048), max_tw=4)
+    up = LightconeEngine(shape=LaunchShape(128, 4096, 2048), max_tw=7, slicer="tree_trim", ordering="rgreedy",
+                         rg_repeats=4)
+    need_quick = max(pl.n_sliced for pl in quick.workload(G, 2).plans.values())
+    need_up = max(pl.n_sliced for pl in up.workload(G, 2).plans.values())
+    assert need_up < need_quick
+    sv = statevector.zz_expectations(G, g, b)
+    server = EnergyServer(quick, n_streams=2, upgrades={2: up}, max_slice_vars=need_up).start()
+    try:
+        C, zz = server.submit(G, g, b, key="G").result(timeout=120)
+        assert server.upgraded == 1
+    finally:
+        server.stop()
+    assert max(abs(z - sv[e]) for z, e in zip(zz, G.edges())) <= 1e-5
+    plain = EnergyServer(quick, n_streams=1, max_slice_vars=need_up).start()
+    try:
+        with pytest.raises(MemoryError, match="Estimated OOM"):
+            plain.submit(G, g, b, key="G").result(timeout=120)
+    finally:
+        plain.stop()
