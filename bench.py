@@ -283,11 +283,25 @@ def mfma_census():
     return res
 
 
+def chain_ops(wp):
+    """Chain ops in a wide plan (a contraction merged with the one reading its
+    output)."""
+    import numpy as np
+    from qtensor_amd import plan as P
+    w = np.asarray(wp.plan.words)
+    pos, n = int(w[3]), 0
+    for _ in range(int(w[1])):
+        w0 = int(w[pos]) & 0xFFFFFFFF
+        n += bool((w0 & P.OP_FUSED) and (w0 & P.OP_CHAIN))
+        pos += w0 >> 16
+    return n
+
+
 def forced_edge(G, gamma, beta, reference=None, target_tw=30, steps=5):
     """C2's extra edge: the first edge under a deliberately wide order (seeded
     randomized greedy, temperature 8, first sample of width 30), sliced over 2-4
     variables with the order kept for the rest, on the wide path (grid-wide
-    fused step kernels, slices on 4 streams).  Slice variables: for 2, 3 and 4
+    fused step kernels and chain ops, every launch covering all slices).  Slice variables: for 2, 3 and 4
     variables both the reference's GreedyParvars rule (max degree first,
     qtensor/optimisation/Greedy.py:78-92) and the order-aware neighbourhood rule
     (wide.slice_vars_for_order) are evaluated, and the choice with the smallest
@@ -305,23 +319,25 @@ def forced_edge(G, gamma, beta, reference=None, target_tw=30, steps=5):
     if reference is None:
         reference = eng.run(compile_wide(comp.circuit), prm)
         ref_kind = "greedy-ordered wide plan"
-    eng.run(wp, prm)                                            # warm
+    # all slices in lockstep (one launch per op covers every slice) on one
+    # stream, launched op by op and replayed from a captured HIP graph
+    per = eng.lockstep_slices(wp)
+    eng.run(wp, prm, n_streams=1, slices_per_launch=per)                  # warm
     times, val = [], None
     for _ in range(steps):
-        val, secs = eng.run(wp, prm, timing=True)
+        val, secs = eng.run(wp, prm, timing=True, n_streams=1, slices_per_launch=per)
         times.append(secs)
     secs_eager = float(np.median(times))
-    # warm re-evaluation: each stream's launches replayed from a HIP graph
-    # captured once (the angles are uploaded into the captured buffer per run)
     val_e = val
-    eng.run(wp, prm, graph=True)
+    eng.run(wp, prm, graph=True, n_streams=1, slices_per_launch=per)
     times = []
     for _ in range(steps):
-        val, secs = eng.run(wp, prm, timing=True, graph=True)
+        val, secs = eng.run(wp, prm, timing=True, graph=True, n_streams=1, slices_per_launch=per)
         times.append(secs)
-    secs = float(np.median(times))
+    secs_graph = float(np.median(times))
     if abs(val - val_e) > 1e-6:
         raise RuntimeError(f"graph replay {val} differs from the eager evaluation {val_e}")
+    secs = min(secs_eager, secs_graph)
     nbytes = wp.plan.bytes_step_model * wp.n_slices
     op_bytes = wp.plan.bytes_ops * wp.n_slices
     gbs = op_bytes / secs / 1e9
@@ -331,9 +347,11 @@ def forced_edge(G, gamma, beta, reference=None, target_tw=30, steps=5):
             "treewidth_unsliced": target_tw, "slices": wp.n_slices, "treewidth_per_slice": wp.treewidth,
             "arena_GB": round(wp.plan.glb_need * 8 / 1e9, 3), "step_model_GB": round(nbytes / 1e9, 3),
             "op_GB": round(op_bytes / 1e9, 3), "plan_seconds": round(plan_s, 3), "seconds": round(secs, 6),
-            "seconds_eager": round(secs_eager, 6),
-            "timing": "median of warm evaluations replaying each stream's captured launches; seconds_eager: "
-                      "the same evaluation launched op by op from the host",
+            "seconds_eager": round(secs_eager, 6), "seconds_graph": round(secs_graph, 6),
+            "slices_per_launch": per, "streams": 1, "chain_ops": chain_ops(wp),
+            "timing": "seconds = the faster of seconds_eager (warm evaluation launched op by op from the host) "
+                      "and seconds_graph (the same launches replayed from a captured HIP graph), medians of "
+                      f"{steps} runs; every launch covers all slices",
             "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
             "bytes_note": "achieved = bytes the executed ops read and write (inputs and output of every "
                           "fused or plain op, per slice x slices) / seconds; step_model_GB counts every "

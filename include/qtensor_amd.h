@@ -204,6 +204,13 @@ int qb_check_program(const int32_t* program, int64_t program_words, const int32_
  * element e of the 2^b result to out[2 e out_stride .. + 1] (first open
  * variable = most significant bit of e).  The host copy of the plan is parsed
  * and every tensor range is checked against `arena_elems` before any launch.
+ * With n_slices = S > 1, one call evaluates the S slice assignments
+ * slice_bits .. slice_bits + S - 1 in lockstep: every launch covers all S
+ * (the grid's z dimension), slice z uses arena elements
+ * [z arena_elems, (z + 1) arena_elems) (the arena then holds S * arena_elems
+ * elements) and writes result element e to out[2 (e out_stride + z) .. + 1]
+ * (a plan with open variables needs out_stride >= S).  One call per S slices
+ * instead of S calls: S times fewer launches, each S times wider.
  */
 typedef struct qb_wide_desc {
     const int32_t* plan_host;    /* host: plan words */
@@ -216,6 +223,7 @@ typedef struct qb_wide_desc {
     int32_t dtype;
     double* out;                 /* device: 2 doubles per result element */
     int64_t out_stride;          /* complex128 values between result elements (0 or 1: adjacent) */
+    int32_t n_slices;            /* slices per call (0 or 1: one; at most 1024) */
 } qb_wide_desc;
 
 int qb_wide_execute(const qb_wide_desc* desc, void* stream);

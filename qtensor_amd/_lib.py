@@ -51,7 +51,7 @@ class WideDesc(ctypes.Structure):
                 ("plan_dev", ctypes.c_void_p), ("params", ctypes.c_void_p),
                 ("arena", ctypes.c_void_p), ("arena_elems", ctypes.c_int64),
                 ("slice_bits", ctypes.c_uint32), ("dtype", ctypes.c_int32),
-                ("out", ctypes.c_void_p), ("out_stride", ctypes.c_int64)]
+                ("out", ctypes.c_void_p), ("out_stride", ctypes.c_int64), ("n_slices", ctypes.c_int32)]
 
 
 class PlanDesc(ctypes.Structure):

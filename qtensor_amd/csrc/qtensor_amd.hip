@@ -270,6 +270,7 @@ struct StepArgsK {
     void* out;
     uint32_t rank_out;  // output rank without the register bits (loop extent)
     uint32_t sum;
+    int64_t zbytes;     // bytes between the arenas of consecutive slices (grid z)
 };
 
 // Deposit of the low bits of x into the bit runs listed in dep (at most 5 runs,
@@ -315,8 +316,9 @@ __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
     const uint32_t n_reg = 1u << a.n_reg;
     const uint64_t n_out = 1ull << a.rank_out;
     const uint64_t stride = (uint64_t)gridDim.x * 256u;
-    V* out = reinterpret_cast<V*>(a.out);
-    const V* p0 = reinterpret_cast<const V*>(a.ptr[0]);
+    const int64_t zo = (int64_t)blockIdx.z * a.zbytes;       // this block's slice
+    V* out = reinterpret_cast<V*>(reinterpret_cast<char*>(a.out) + zo);
+    const V* p0 = reinterpret_cast<const V*>(reinterpret_cast<const char*>(a.ptr[0]) + zo);
     for (uint64_t base0 = (uint64_t)blockIdx.x * 256u; base0 < n_out; base0 += kU * stride) {
         uint32_t off[kU][K];
         uint64_t o0[kU];
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
                 V x[K], y[K];
 #pragma unroll
                 for (int j = 1; j < K; ++j) {
-                    const V* p = reinterpret_cast<const V*>(a.ptr[j]);
+                    const V* p = reinterpret_cast<const V*>(reinterpret_cast<const char*>(a.ptr[j]) + zo);
                     const uint32_t oj = off[u][j] + a.roff[j][r];
                     x[j] = p[oj];
                     if (a.sum) y[j] = p[oj + a.top[j]];
@@ -391,6 +393,7 @@ struct FusedArgs {
     char* out;
     uint32_t omask;               // output variables as bits of the union offset
     uint32_t rank_out, k, lo_bits;
+    int64_t zbytes;               // bytes between the arenas of consecutive slices (grid z)
 };
 
 template <int NI>
@@ -426,11 +429,12 @@ __global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a, const int
     for (int j = 0; j < NI; ++j) lo_off[j] = (uint32_t)lo_tab[tid * NI + j];
     const uint32_t n_x = 1u << a.k;
     const uint32_t n_hi = 1u << (a.rank_out - a.lo_bits);
+    const int64_t zo = (int64_t)blockIdx.z * a.zbytes;       // this block's slice
     for (uint32_t hi = blockIdx.x; hi < n_hi; hi += gridDim.x) {
         const V* in[NI];
 #pragma unroll
         for (int j = 0; j < NI; ++j)
-            in[j] = reinterpret_cast<const V*>(a.in[j]) + lo_off[j] + fused_hi_offset<NI>(a, j, hi);
+            in[j] = reinterpret_cast<const V*>(a.in[j] + zo) + lo_off[j] + fused_hi_offset<NI>(a, j, hi);
         V acc = {0, 0};
         if (n_x == 1u) {
             V p = in[0][x_tab[0]];
@@ -449,7 +453,7 @@ __global__ __launch_bounds__(256) void fused_kernel(const FusedArgs a, const int
                 acc = cadd(acc, cadd(p, q));
             }
         }
-        reinterpret_cast<V*>(a.out)[((uint64_t)hi << a.lo_bits) | tid] = acc;
+        reinterpret_cast<V*>(a.out + zo)[((uint64_t)hi << a.lo_bits) | tid] = acc;
     }
 }
 
@@ -478,8 +482,9 @@ __global__ __launch_bounds__(256) void fused2_kernel(const FusedArgs a, const in
     const uint32_t tid = threadIdx.x;
     if (tid >= (1u << a.lo_bits)) return;
     const uint32_t lo_a = (uint32_t)lo_tab[tid * 2], lo_b = (uint32_t)lo_tab[tid * 2 + 1];
-    const V* __restrict__ in_a = reinterpret_cast<const V*>(a.in[0]);
-    const V* __restrict__ in_b = reinterpret_cast<const V*>(a.in[1]);
+    const int64_t zo = (int64_t)blockIdx.z * a.zbytes;       // this block's slice
+    const V* __restrict__ in_a = reinterpret_cast<const V*>(a.in[0] + zo);
+    const V* __restrict__ in_b = reinterpret_cast<const V*>(a.in[1] + zo);
     const uint32_t n_x = 1u << a.k;
     const uint32_t n_hk = 1u << e.hk_bits;
     for (uint32_t hk = blockIdx.x; hk < n_hk; hk += gridDim.x) {
@@ -507,14 +512,114 @@ __global__ __launch_bounds__(256) void fused2_kernel(const FusedArgs a, const in
 #pragma unroll
             for (int i = 0; i < NA; ++i)
 #pragma unroll
-                for (int j = 0; j < NB; ++j) acc[i][j] = cadd(acc[i][j], cmul(va[i], vb[j]));
+                for (int j = 0; j < NB; ++j) acc[i][j] = cfma(acc[i][j], va[i], vb[j]);
         }
-        V* out = reinterpret_cast<V*>(a.out);
+        V* out = reinterpret_cast<V*>(a.out + zo);
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
             for (int j = 0; j < NB; ++j)
                 out[((uint64_t)(hi | e.rdep_a[i] | e.rdep_b[j]) << a.lo_bits) | tid] = acc[i][j];
+    }
+}
+
+// Chain op: a fused three-input op out[o] = sum_{y, x} A[a] B[b] D[d] (the
+// compiler's merge of a contraction T = sum A B with the contraction
+// out = sum T D consuming it).  With x = the (at most kChainMaxX lowest)
+// summed variables that D does not carry and y = the other summed variables,
+// the sum factorises as out[o] = sum_y D[d(o, y)] (sum_x A[a(o, y, x)] B[b(o, y, x)]),
+// so each inner sum is formed in registers and T is never stored.  Register tile: up to two high
+// output bits that A carries and B does not (RR), as many of B's, and one
+// output bit that only D carries (RD): per outer assignment y a thread forms
+// 2^RR x 2^RR values of T (2^RR loads of A and of B per inner assignment) and
+// multiplies each into 2^RD outputs; D (at most 2^kChainMaxD elements) is
+// staged in LDS.  Blocks visit the remaining high bits hk through deposit runs.
+constexpr uint32_t kFusedChain = 1u << 12;
+constexpr int kChainMaxX = 4, kChainMaxY = 5, kChainMaxD = 10;
+
+struct ChainArgs {
+    uint32_t xoff[2][1 << kChainMaxX];   // byte offset in A, B of inner summed assignment x
+    uint32_t yoff[3][1 << kChainMaxY];   // byte offset in A, B (element offset in D) of outer assignment y
+    uint32_t roff_a[4], roff_b[4];       // byte offset in A (B) of register pattern i of A's (B's) bits
+    uint32_t droff_a[4], droff_b[4], droff_d[2];  // offset in D of each register pattern
+    uint32_t rdep_a[4], rdep_b[4], rdep_d[2];     // high-bit pattern of each register pattern
+    uint32_t dep_run[kFusedDepRuns];
+    uint32_t n_dep, hk_bits, kx, ky, d_elems;
+};
+
+template <typename R, int RR, int RD>
+__global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const int32_t* __restrict__ lo_tab,
+                                                    const ChainArgs e) {
+    using V = typename Cx<R>::V;
+    constexpr int NR = 1 << RR, ND = 1 << RD;
+    __shared__ V dsh[1 << kChainMaxD];
+    const uint32_t tid = threadIdx.x;           // lo_bits = 8: every thread owns an output
+    const int64_t zo = (int64_t)blockIdx.z * a.zbytes;       // this block's slice
+    const V* __restrict__ dsrc = reinterpret_cast<const V*>(a.in[2] + zo);
+    for (uint32_t i = tid; i < e.d_elems; i += 256u) dsh[i] = dsrc[i];
+    __syncthreads();
+    const uint32_t lo_a = (uint32_t)lo_tab[tid * 3], lo_b = (uint32_t)lo_tab[tid * 3 + 1],
+                   lo_d = (uint32_t)lo_tab[tid * 3 + 2];
+    // A and B (each below 4 GiB) are read through 32-bit byte offsets from a
+    // block-uniform base: one vector add per load
+    const char* __restrict__ in_a = a.in[0] + zo;
+    const char* __restrict__ in_b = a.in[1] + zo;
+    V* out = reinterpret_cast<V*>(a.out + zo);
+    constexpr uint32_t EB = sizeof(V);
+    const uint32_t n_x = 1u << e.kx, n_y = 1u << e.ky, n_hk = 1u << e.hk_bits;
+    for (uint32_t hk = blockIdx.x; hk < n_hk; hk += gridDim.x) {
+        uint32_t hi = 0;
+#pragma unroll
+        for (int r = 0; r < kFusedDepRuns; ++r) {
+            if ((uint32_t)r >= e.n_dep) break;
+            const uint32_t f = e.dep_run[r];
+            hi |= bfe(hk, f & 31u, (f >> 5) & 31u) << (f >> 10);
+        }
+        const uint32_t oa = (lo_a + fused_hi_offset<3>(a, 0, hi)) * EB;
+        const uint32_t ob = (lo_b + fused_hi_offset<3>(a, 1, hi)) * EB;
+        const uint32_t od = lo_d + fused_hi_offset<3>(a, 2, hi);
+        V acc[NR][NR][ND];
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) acc[i][j][d] = V{0, 0};
+        for (uint32_t y = 0; y < n_y; ++y) {
+            V t[NR][NR];
+#pragma unroll
+            for (int i = 0; i < NR; ++i)
+#pragma unroll
+                for (int j = 0; j < NR; ++j) t[i][j] = V{0, 0};
+            const uint32_t ya = oa + e.yoff[0][y], yb = ob + e.yoff[1][y];
+            for (uint32_t x = 0; x < n_x; ++x) {
+                const uint32_t xa = ya + e.xoff[0][x], xb = yb + e.xoff[1][x];
+                V va[NR], vb[NR];
+#pragma unroll
+                for (int i = 0; i < NR; ++i) va[i] = *reinterpret_cast<const V*>(in_a + (xa + e.roff_a[i]));
+#pragma unroll
+                for (int j = 0; j < NR; ++j) vb[j] = *reinterpret_cast<const V*>(in_b + (xb + e.roff_b[j]));
+#pragma unroll
+                for (int i = 0; i < NR; ++i)
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) t[i][j] = cfma(t[i][j], va[i], vb[j]);
+            }
+            const uint32_t yd = od + e.yoff[2][y];
+#pragma unroll
+            for (int i = 0; i < NR; ++i)
+#pragma unroll
+                for (int j = 0; j < NR; ++j)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d)
+                        acc[i][j][d] = cfma(acc[i][j][d], t[i][j], dsh[yd + e.droff_a[i] + e.droff_b[j] + e.droff_d[d]]);
+        }
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                    out[((uint64_t)(hi | e.rdep_a[i] | e.rdep_b[j] | e.rdep_d[d]) << 8) | tid] = acc[i][j][d];
     }
 }
 
@@ -1463,27 +1568,33 @@ int slots_for(int32_t item_threads, int32_t lds_elems, int32_t dtype, int32_t gr
 }
 
 // Wide plans (every tensor in global memory): gate build and result copy.
+// Slice z (grid z) uses slice assignment slice_bits + z and the arena at
+// arena + z zbytes.
 template <typename R>
 __global__ void build_gates_global(const int32_t* __restrict__ gates, int32_t n_gates,
                                    const double* __restrict__ params, uint32_t slice_bits,
-                                   void* __restrict__ arena) {
+                                   void* __restrict__ arena, int64_t zbytes) {
     using V = typename Cx<R>::V;
     const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n_gates)
-        build_gate<R>(gates + (size_t)g * kGateWords, params, (const QB_LDS double*)nullptr, 0, slice_bits,
-                      reinterpret_cast<V*>(arena), (const V*)nullptr);
+        build_gate<R>(gates + (size_t)g * kGateWords, params, (const QB_LDS double*)nullptr, 0,
+                      slice_bits + blockIdx.z,
+                      reinterpret_cast<V*>(reinterpret_cast<char*>(arena) + (int64_t)blockIdx.z * zbytes),
+                      (const V*)nullptr);
 }
 
-// Result of a wide plan: n elements from arena[loc], element e as complex128
-// to out[2 e stride].
+// Result of a wide plan: n elements from arena[loc], element e of slice z
+// (grid z; its arena at arena + z zbytes) as complex128 to out[2 (e stride + z)].
 template <typename R>
 __global__ void store_result(const void* __restrict__ arena, uint32_t loc, uint32_t n, int64_t stride,
-                             double* __restrict__ out) {
+                             double* __restrict__ out, int64_t zbytes) {
     using V = typename Cx<R>::V;
+    const V* src = reinterpret_cast<const V*>(reinterpret_cast<const char*>(arena) + (int64_t)blockIdx.z * zbytes);
+    double* dst = out + 2 * (int64_t)blockIdx.z;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const V r = reinterpret_cast<const V*>(arena)[loc + e];
-        out[2 * (int64_t)e * stride] = (double)r.x;
-        out[2 * (int64_t)e * stride + 1] = (double)r.y;
+        const V r = src[loc + e];
+        dst[2 * (int64_t)e * stride] = (double)r.x;
+        dst[2 * (int64_t)e * stride + 1] = (double)r.y;
     }
 }
 
@@ -1520,7 +1631,8 @@ uint32_t pdep_host(uint32_t x, uint32_t m) {
 // pext(tid, m_j & 255) + pext(pdep(base >> 8, kept >> 8), m_j >> 8) << popc(m_j & 255)
 // plus the register pattern's pext(pdep(r, reg), m_j).
 template <int K>
-int launch_step_k(const StepArgs& a, const uint32_t* runs_unused, int32_t dtype, int grid_unused, hipStream_t st) {
+int launch_step_k(const StepArgs& a, const uint32_t* runs_unused, int32_t dtype, int grid_unused, hipStream_t st,
+                  int nz = 1, int64_t zbytes = 0) {
     (void)runs_unused;
     (void)grid_unused;
     int order[K];
@@ -1577,26 +1689,165 @@ int launch_step_k(const StepArgs& a, const uint32_t* runs_unused, int32_t dtype,
     k.out = a.out;
     k.rank_out = a.rank_out - (uint32_t)n_reg;
     k.sum = a.n_sum == 2 ? 1u : 0u;
+    k.zbytes = zbytes;
     const uint64_t n_out = 1ull << k.rank_out;
     const uint64_t want = (n_out + 255) / 256;
     const uint64_t cap = (uint64_t)g_num_cus * 8;
     const int grid = (int)(want < cap ? (want > 0 ? want : 1) : cap);
     if (dtype == QB_C128)
-        hipLaunchKernelGGL((step_kernel_k<double, K>), dim3(grid), dim3(256), 0, st, k);
+        hipLaunchKernelGGL((step_kernel_k<double, K>), dim3(grid, 1, nz), dim3(256), 0, st, k);
     else
-        hipLaunchKernelGGL((step_kernel_k<float, K>), dim3(grid), dim3(256), 0, st, k);
+        hipLaunchKernelGGL((step_kernel_k<float, K>), dim3(grid, 1, nz), dim3(256), 0, st, k);
     QB_HIP(hipGetLastError());
     return QB_OK;
 }
 
 }  // namespace
 
+// Deposit runs of the set bits of `keep` (below hi_bits) for the kernels'
+// block loop over hk: run r copies bits [src, src + w) of hk to [dst, dst + w)
+// of hi (src | w << 5 | dst << 10).  Checked against pdep for every bit and for
+// all of them.  Returns false when more than kFusedDepRuns runs are needed.
+static bool deposit_runs(uint32_t keep, uint32_t hi_bits, uint32_t* dep_run, uint32_t* n_dep, uint32_t* hk_bits) {
+    uint32_t n = 0, src = 0;
+    for (uint32_t b = 0; b < hi_bits;) {
+        if (!((keep >> b) & 1u)) { ++b; continue; }
+        const uint32_t b0 = b;
+        while (b < hi_bits && ((keep >> b) & 1u)) ++b;
+        if (n >= (uint32_t)kFusedDepRuns) return false;
+        dep_run[n++] = src | ((b - b0) << 5) | (b0 << 10);
+        src += b - b0;
+    }
+    for (uint32_t b = 0; b <= src; ++b) {
+        const uint32_t hk = b < src ? (1u << b) : ((1u << src) - 1u);
+        uint32_t hi = 0;
+        for (uint32_t r = 0; r < n; ++r) {
+            const uint32_t f = dep_run[r], wd = (f >> 5) & 31u;
+            hi |= ((hk >> (f & 31u)) & ((1u << wd) - 1u)) << (f >> 10);
+        }
+        if (hi != pdep_host(hk, keep)) return false;
+    }
+    *n_dep = n;
+    *hk_bits = src;
+    return true;
+}
+
+// Launch a chain op (kFusedChain): inputs A, B, D; the inner summed variables
+// are those D lacks.  `a` holds the checked inputs, masks and high-bit runs.
+static int launch_chain(const FusedArgs& a, const int32_t* lo_tab, uint32_t n_in, uint32_t smask, int32_t k_op,
+                        int32_t dtype, hipStream_t st, int nz) {
+    const uint32_t rank_out = a.rank_out, lo_bits = a.lo_bits, omask = a.omask;
+    // inner: the lowest (at most kChainMaxX) summed variables D lacks; outer: the rest
+    uint32_t xbits = 0;
+    for (uint32_t m = smask & ~a.mask[2], n = 0; m && n < (uint32_t)kChainMaxX; m &= m - 1, ++n) xbits |= m & (0u - m);
+    const uint32_t ybits = smask & ~xbits;
+    const uint32_t kx = (uint32_t)__builtin_popcount(xbits), ky = (uint32_t)__builtin_popcount(ybits);
+    const uint32_t rd = (uint32_t)__builtin_popcount(a.mask[2]);
+    if (n_in != 3 || lo_bits != 8 || kx < 1 || kx > (uint32_t)kChainMaxX || ky > (uint32_t)kChainMaxY ||
+        rd > (uint32_t)kChainMaxD)
+        return set_error(QB_ERR_INVALID, "chain op %d: %u inputs, rank %u, %u inner and %u outer summed variables, "
+                         "third input of rank %u", k_op, n_in, rank_out, kx, ky, rd);
+    ChainArgs e;
+    memset(&e, 0, sizeof(e));
+    e.kx = kx;
+    e.ky = ky;
+    e.d_elems = 1u << rd;
+    const uint32_t eb = dtype == QB_C128 ? 16u : 8u;
+    // A and B are addressed with 32-bit byte offsets
+    if (__builtin_popcount(a.mask[0]) + (eb == 16u ? 4 : 3) > 31 || __builtin_popcount(a.mask[1]) + (eb == 16u ? 4 : 3) > 31)
+        return set_error(QB_ERR_INVALID, "chain op %d: an input of 2 GiB or more", k_op);
+    for (uint32_t x = 0; x < (1u << kx); ++x) {
+        const uint32_t u = pdep_host(x, xbits);
+        e.xoff[0][x] = pext_host(u, a.mask[0]) * eb;
+        e.xoff[1][x] = pext_host(u, a.mask[1]) * eb;
+    }
+    for (uint32_t y = 0; y < (1u << ky); ++y) {
+        const uint32_t u = pdep_host(y, ybits);
+        e.yoff[0][y] = pext_host(u, a.mask[0]) * eb;
+        e.yoff[1][y] = pext_host(u, a.mask[1]) * eb;
+        e.yoff[2][y] = pext_host(u, a.mask[2]);
+    }
+    const uint32_t hi_bits = rank_out - lo_bits;
+    uint32_t ubit[32], n_ob = 0;
+    for (uint32_t u = 0; u < 32; ++u)
+        if ((omask >> u) & 1u) ubit[n_ob++] = u;
+    // register bits (highest first): A's and B's in equal numbers (at most 2
+    // each), one that only D carries; fewer when the rest would split into more
+    // deposit runs than the kernel takes
+    uint32_t rr = 0, rdb = 0, reg_a[2], reg_b[2], reg_d[1];
+    uint32_t ca[32], cb[32], cd[32], na = 0, nb = 0, nd = 0;
+    for (uint32_t b = hi_bits; b-- > 0;) {
+        const uint32_t u = ubit[b + lo_bits];
+        const bool in_a = (a.mask[0] >> u) & 1u, in_b = (a.mask[1] >> u) & 1u, in_d = (a.mask[2] >> u) & 1u;
+        if (in_a && !in_b) ca[na++] = b;
+        else if (in_b && !in_a) cb[nb++] = b;
+        else if (in_d && !in_a && !in_b) cd[nd++] = b;
+    }
+    bool ok = false;
+    for (int want_r = 2; want_r >= 0 && !ok; --want_r) {
+        for (int want_d = 1; want_d >= 0 && !ok; --want_d) {
+            rr = (uint32_t)want_r;
+            if (rr > na) rr = na;
+            if (rr > nb) rr = nb;
+            rdb = (uint32_t)want_d < nd ? (uint32_t)want_d : nd;
+            uint32_t regmask = 0;
+            for (uint32_t i = 0; i < rr; ++i) {
+                reg_a[i] = ca[i];
+                reg_b[i] = cb[i];
+                regmask |= (1u << ca[i]) | (1u << cb[i]);
+            }
+            for (uint32_t i = 0; i < rdb; ++i) {
+                reg_d[i] = cd[i];
+                regmask |= 1u << cd[i];
+            }
+            const uint32_t keep = (hi_bits >= 32 ? 0xffffffffu : ((1u << hi_bits) - 1u)) & ~regmask;
+            ok = deposit_runs(keep, hi_bits, e.dep_run, &e.n_dep, &e.hk_bits);
+        }
+    }
+    if (!ok) return set_error(QB_ERR_UNSUPPORTED, "chain op %d: deposit runs", k_op);
+    for (uint32_t p = 0; p < (1u << rr); ++p) {
+        uint32_t ha = 0, hb = 0;
+        for (uint32_t i = 0; i < rr; ++i)
+            if ((p >> i) & 1u) {
+                ha |= 1u << reg_a[i];
+                hb |= 1u << reg_b[i];
+            }
+        const uint32_t ua = pdep_host(ha << lo_bits, omask), ub = pdep_host(hb << lo_bits, omask);
+        e.rdep_a[p] = ha;
+        e.rdep_b[p] = hb;
+        e.roff_a[p] = pext_host(ua, a.mask[0]) * eb;
+        e.roff_b[p] = pext_host(ub, a.mask[1]) * eb;
+        e.droff_a[p] = pext_host(ua, a.mask[2]);
+        e.droff_b[p] = pext_host(ub, a.mask[2]);
+    }
+    for (uint32_t p = 0; p < (1u << rdb); ++p) {
+        const uint32_t hd = p ? (1u << reg_d[0]) : 0u;
+        e.rdep_d[p] = hd;
+        e.droff_d[p] = pext_host(pdep_host(hd << lo_bits, omask), a.mask[2]);
+    }
+    const uint64_t gcap = (uint64_t)g_num_cus * 16;
+    const uint64_t want = 1ull << e.hk_bits;
+    const int grid = (int)(want < gcap ? want : gcap);
+#define QB_CHAIN_CASE(RR_, RD_)                                                                           \
+    if (rr == RR_ && rdb == RD_) {                                                                        \
+        if (dtype == QB_C128)                                                                             \
+            hipLaunchKernelGGL((chain_kernel<double, RR_, RD_>), dim3(grid, 1, nz), dim3(256), 0, st, a, lo_tab, e); \
+        else                                                                                              \
+            hipLaunchKernelGGL((chain_kernel<float, RR_, RD_>), dim3(grid, 1, nz), dim3(256), 0, st, a, lo_tab, e);  \
+    }
+    QB_CHAIN_CASE(0, 0) QB_CHAIN_CASE(0, 1) QB_CHAIN_CASE(1, 0) QB_CHAIN_CASE(1, 1) QB_CHAIN_CASE(2, 0)
+    QB_CHAIN_CASE(2, 1)
+#undef QB_CHAIN_CASE
+    QB_HIP(hipGetLastError());
+    return QB_OK;
+}
+
 // Parse, check and launch one fused op of a wide plan (host copy `w`, device
 // copy `plan_dev`).  Every offset the kernel forms is a pext of a union offset
 // by an input's mask: the tables are recomputed here from the masks and must
 // match, and each input and the output must lie inside the arena.
 int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, int64_t pos, int32_t k_op,
-                 char* arena, int64_t elem, int64_t cap, int32_t dtype, hipStream_t st) {
+                 char* arena, int64_t elem, int64_t cap, int32_t dtype, hipStream_t st, int nz, int64_t zbytes) {
     if (pos + kFusedHdr > plan_words) return set_error(QB_ERR_INVALID, "fused op %d outside the plan", k_op);
     const uint32_t w0 = (uint32_t)w[pos];
     const uint32_t n_in = w0 & 15u;
@@ -1606,7 +1857,8 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
     const uint32_t out_loc = (uint32_t)w[pos + 2];
     const uint32_t omask = (uint32_t)w[pos + 4], smask = (uint32_t)w[pos + 5];
     if (n_in < 1 || n_in > (uint32_t)kFusedMaxIn) return set_error(QB_ERR_INVALID, "fused op %d: %u inputs", k_op, n_in);
-    if (rank_u > 30 || rank_out + k != rank_u || k > 8 || lo_bits != (rank_out < 8 ? rank_out : 8u) ||
+    const bool chain = (w0 & kFusedChain) != 0;
+    if (rank_u > (chain ? 31u : 30u) || rank_out + k != rank_u || k > 8 || lo_bits != (rank_out < 8 ? rank_out : 8u) ||
         (omask & smask) || (uint32_t)__builtin_popcount(omask) != rank_out ||
         (uint32_t)__builtin_popcount(smask) != k || ((omask | smask) >> rank_u))
         return set_error(QB_ERR_INVALID, "fused op %d: inconsistent shape", k_op);
@@ -1682,6 +1934,8 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
     a.rank_out = rank_out;
     a.k = k;
     a.lo_bits = lo_bits;
+    a.zbytes = zbytes;
+    if (chain) return launch_chain(a, lo_tab, n_in, smask, k_op, dtype, st, nz);
     const uint64_t gcap = (uint64_t)g_num_cus * 16;
     if (n_in == 2 && rank_out >= lo_bits + 1) {
         // register bits: high output bits carried by one input only
@@ -1761,15 +2015,15 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
 #define QB_FUSED2_CASE(RA_, RB_)                                                                         \
             if (ra == RA_ && rb == RB_) {                                                                \
                 if (dtype == QB_C128)                                                                    \
-                    hipLaunchKernelGGL((fused2_kernel<double, RA_, RB_>), dim3(grid2), dim3(256), 0, st, a, lo_tab, \
+                    hipLaunchKernelGGL((fused2_kernel<double, RA_, RB_>), dim3(grid2, 1, nz), dim3(256), 0, st, a, lo_tab, \
                                        x_tab, e);                                                        \
                 else                                                                                     \
-                    hipLaunchKernelGGL((fused2_kernel<float, RA_, RB_>), dim3(grid2), dim3(256), 0, st, a, lo_tab,  \
+                    hipLaunchKernelGGL((fused2_kernel<float, RA_, RB_>), dim3(grid2, 1, nz), dim3(256), 0, st, a, lo_tab,  \
                                        x_tab, e);                                                        \
             }
 #define QB_FUSED2_CASE_F(RA_, RB_)                                                                       \
             if (ra == RA_ && rb == RB_)                                                                  \
-                hipLaunchKernelGGL((fused2_kernel<float, RA_, RB_>), dim3(grid2), dim3(256), 0, st, a, lo_tab, x_tab, e);
+                hipLaunchKernelGGL((fused2_kernel<float, RA_, RB_>), dim3(grid2, 1, nz), dim3(256), 0, st, a, lo_tab, x_tab, e);
             QB_FUSED2_CASE(0, 1) QB_FUSED2_CASE(0, 2) QB_FUSED2_CASE(1, 0) QB_FUSED2_CASE(1, 1)
             QB_FUSED2_CASE(1, 2) QB_FUSED2_CASE(2, 0) QB_FUSED2_CASE(2, 1) QB_FUSED2_CASE(2, 2)
             QB_FUSED2_CASE_F(0, 3) QB_FUSED2_CASE_F(1, 3) QB_FUSED2_CASE_F(2, 3) QB_FUSED2_CASE_F(3, 0)
@@ -1785,9 +2039,9 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
 #define QB_FUSED_CASE(N)                                                                      \
     case N:                                                                                   \
         if (dtype == QB_C128)                                                                 \
-            hipLaunchKernelGGL((fused_kernel<double, N>), dim3(grid), dim3(256), 0, st, a, lo_tab, x_tab); \
+            hipLaunchKernelGGL((fused_kernel<double, N>), dim3(grid, 1, nz), dim3(256), 0, st, a, lo_tab, x_tab); \
         else                                                                                  \
-            hipLaunchKernelGGL((fused_kernel<float, N>), dim3(grid), dim3(256), 0, st, a, lo_tab, x_tab);  \
+            hipLaunchKernelGGL((fused_kernel<float, N>), dim3(grid, 1, nz), dim3(256), 0, st, a, lo_tab, x_tab);  \
         break;
     switch (n_in) {
         QB_FUSED_CASE(1)
@@ -2085,6 +2339,14 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
     const uint32_t result_loc = (uint32_t)w[4];
     const int64_t elem = d->dtype == QB_C128 ? 16 : 8;
     const int64_t cap = d->arena_elems;
+    // slices z = 0 .. nz-1 (assignments slice_bits + z) in lockstep: every
+    // launch covers all of them (grid z), slice z in arena region z
+    const int nz = d->n_slices > 0 ? d->n_slices : 1;
+    if (nz > 1024) return set_error(QB_ERR_INVALID, "n_slices %d > 1024", d->n_slices);
+    if ((uint64_t)d->slice_bits + (uint64_t)(nz - 1) > 0xffffffffull)
+        return set_error(QB_ERR_INVALID, "slice assignments beyond 32 bits");
+    if (cap < 0 || cap > ((int64_t)1 << 40)) return set_error(QB_ERR_INVALID, "arena_elems %lld", (long long)cap);
+    const int64_t zbytes = cap * elem;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if ((int64_t)gates_off + (int64_t)n_gates * kGateWords > d->plan_words)
         return set_error(QB_ERR_INVALID, "gate table outside the plan");
@@ -2098,11 +2360,11 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
     if (n_gates > 0) {
         const int blocks = (n_gates + 255) / 256;
         if (d->dtype == QB_C128)
-            hipLaunchKernelGGL(build_gates_global<double>, dim3(blocks), dim3(256), 0, st, d->plan_dev + gates_off,
-                               n_gates, d->params, d->slice_bits, d->arena);
+            hipLaunchKernelGGL(build_gates_global<double>, dim3(blocks, 1, nz), dim3(256), 0, st,
+                               d->plan_dev + gates_off, n_gates, d->params, d->slice_bits, d->arena, zbytes);
         else
-            hipLaunchKernelGGL(build_gates_global<float>, dim3(blocks), dim3(256), 0, st, d->plan_dev + gates_off,
-                               n_gates, d->params, d->slice_bits, d->arena);
+            hipLaunchKernelGGL(build_gates_global<float>, dim3(blocks, 1, nz), dim3(256), 0, st,
+                               d->plan_dev + gates_off, n_gates, d->params, d->slice_bits, d->arena, zbytes);
         QB_HIP(hipGetLastError());
     }
     char* arena = reinterpret_cast<char*>(d->arena);
@@ -2112,7 +2374,8 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
         const uint32_t w0 = (uint32_t)w[pos];
         if (w0 & (kOpGather | kOpReduce)) return set_error(QB_ERR_INVALID, "wide plans take step ops only");
         if (w0 & kOpFused) {
-            const int rc = launch_fused(w, d->plan_words, d->plan_dev, pos, k, arena, elem, cap, d->dtype, st);
+            const int rc = launch_fused(w, d->plan_words, d->plan_dev, pos, k, arena, elem, cap, d->dtype, st, nz,
+                                        zbytes);
             if (rc) return rc;
             pos += w0 >> 16;
             continue;
@@ -2152,10 +2415,10 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
         const int grid = (int)(want < gcap ? (want > 0 ? want : 1) : gcap);
         int rc;
         switch (n_in) {
-        case 1: rc = launch_step_k<1>(a, runs, d->dtype, grid, st); break;
-        case 2: rc = launch_step_k<2>(a, runs, d->dtype, grid, st); break;
-        case 3: rc = launch_step_k<3>(a, runs, d->dtype, grid, st); break;
-        default: rc = launch_step_k<4>(a, runs, d->dtype, grid, st); break;
+        case 1: rc = launch_step_k<1>(a, runs, d->dtype, grid, st, nz, zbytes); break;
+        case 2: rc = launch_step_k<2>(a, runs, d->dtype, grid, st, nz, zbytes); break;
+        case 3: rc = launch_step_k<3>(a, runs, d->dtype, grid, st, nz, zbytes); break;
+        default: rc = launch_step_k<4>(a, runs, d->dtype, grid, st, nz, zbytes); break;
         }
         if (rc) return rc;
         pos += w0 >> 16;
@@ -2167,12 +2430,15 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
         return set_error(QB_ERR_INVALID, "result outside the arena");
     const int64_t stride = d->out_stride > 0 ? d->out_stride : 1;
     const int rblocks = (int)((n_res + 255) / 256 < 1024 ? (n_res + 255) / 256 : 1024);
+    if (nz > 1 && n_res > 1 && stride < nz)
+        return set_error(QB_ERR_INVALID, "out_stride %lld < n_slices %d: slice results would overlap",
+                         (long long)stride, nz);
     if (d->dtype == QB_C128)
-        hipLaunchKernelGGL(store_result<double>, dim3(rblocks), dim3(256), 0, st, d->arena, result_loc & ~kGlobal,
-                           n_res, stride, d->out);
+        hipLaunchKernelGGL(store_result<double>, dim3(rblocks, 1, nz), dim3(256), 0, st, d->arena,
+                           result_loc & ~kGlobal, n_res, stride, d->out, zbytes);
     else
-        hipLaunchKernelGGL(store_result<float>, dim3(rblocks), dim3(256), 0, st, d->arena, result_loc & ~kGlobal,
-                           n_res, stride, d->out);
+        hipLaunchKernelGGL(store_result<float>, dim3(rblocks, 1, nz), dim3(256), 0, st, d->arena,
+                           result_loc & ~kGlobal, n_res, stride, d->out, zbytes);
     QB_HIP(hipGetLastError());
     return QB_OK;
 }

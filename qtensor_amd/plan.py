@@ -60,6 +60,9 @@ GLOBAL = 0x80000000
 OP_GATHER = 1 << 15
 # fused step (wide plans): several bucket steps contracted in one pass
 OP_FUSED = 1 << 14
+# fused step flag: a chain op (see chain_pairs)
+OP_CHAIN = 1 << 12
+CHAIN_MAX_X, CHAIN_MAX_Y, CHAIN_MAX_D = 4, 5, 10
 # reduction op (item plans): a chain of single-input sums in one op, one wave
 OP_REDUCE = 1 << 13
 REDUCE_HDR = 4
@@ -162,14 +165,17 @@ class SymStep:
 class FusedStep:
     """Several consecutive bucket steps of a wide plan contracted in one pass:
     out[o] = sum over the summed variables x of prod_j in_j[pext(u(o, x), mask_j)],
-    u = the union offset (variable a of the union at bit rank_u - 1 - a)."""
+    u = the union offset (variable a of the union at bit rank_u - 1 - a).
+    ``chain``: three inputs A, B, D where D carries none of the summed
+    variables of the first two's contraction (see ``chain_pairs``)."""
     fused = True
 
-    def __init__(self, inputs, out, union_dev, members):
+    def __init__(self, inputs, out, union_dev, members, chain=False):
         self.inputs = inputs
         self.out = out
         self.union = sorted(union_dev)
         self.members = members          # original step indices, last = the output step
+        self.chain = chain
 
     @property
     def rank_out(self):
@@ -401,6 +407,81 @@ def split_small_inputs(steps):
     return out
 
 
+def chain_pairs(steps, bw=4e12, rate=40e12, launch=4e-6):
+    """Merge a two-input contraction T = sum_x A B with the two-input
+    contraction out = sum_y T D that reads T into one chain op over A, B and D
+    (``FusedStep(chain=True)``), so that T, often the largest tensor of a wide
+    plan, is formed in registers and never written: the kernel sums
+    sum_y D (sum_x A B).  D carries no variable of x (bucket elimination
+    removes a variable from every tensor when it sums it).  Taken when the time
+    model (bytes at ``bw``, arithmetic at ``rate``; T's values recomputed for
+    each output bit beyond the first that only D carries) says one pass beats
+    the two; limits: at least one summed variable that D lacks (up to
+    CHAIN_MAX_X of them are summed innermost, over A B alone), at most
+    CHAIN_MAX_Y others, D of rank <= CHAIN_MAX_D, union rank <= 31, output
+    rank >= 8.
+    The value is the same sum as the two steps', up to rounding (the
+    reference forms a bucket's product in any order,
+    ``qtensor/ProcessingFrameworks.py:106-146``)."""
+    def union_of(st):
+        if getattr(st, "fused", False):
+            return set(st.union)
+        u = set()
+        for nd in st.inputs:
+            u |= set(nd.dev)
+        return u
+
+    producer = {id(st.out): i for i, st in enumerate(steps)}
+    merged, gone = {}, set()
+    for ci, c in enumerate(steps):
+        if isinstance(c, ReduceStep) or getattr(c, "chain", False) or len(c.inputs) != 2:
+            continue
+        big = max(c.inputs, key=lambda nd: len(nd.dev))
+        d = c.inputs[1] if big is c.inputs[0] else c.inputs[0]
+        pi = producer.get(id(big))
+        if pi is None or pi in gone or pi in merged:
+            continue
+        pst = steps[pi]
+        if isinstance(pst, ReduceStep) or getattr(pst, "chain", False) or len(pst.inputs) != 2:
+            continue
+        u_p, u_c = union_of(pst), union_of(c)
+        out_c = set(c.out.dev)
+        union = u_p | u_c
+        if (u_p - set(big.dev)) & set(d.dev):
+            continue
+        # the device sums up to CHAIN_MAX_X of the summed variables D lacks
+        # innermost (over A B only) and the rest in the outer loop
+        summed = union - out_c
+        kx = min(CHAIN_MAX_X, len(summed - set(d.dev)))
+        ky = len(summed) - kx
+        if not (kx >= 1 and ky <= CHAIN_MAX_Y and len(d.dev) <= CHAIN_MAX_D and len(union) <= 31
+                and len(out_c) >= 8 and len(summed) <= FUSED_MAX_SUM):
+            continue
+        # the kernel tiles up to 2 x 2 high output bits that only A (B) carries in
+        # registers; a large op without such bits would load A and B per product
+        a_set, b_set = set(pst.inputs[0].dev), set(pst.inputs[1].dev)
+        ordered = sorted(union)
+        high_out = [v for v in ordered if v in out_c][:max(len(out_c) - 8, 0)]
+        tile = min(sum(1 for v in high_out if v in a_set and v not in b_set),
+                   sum(1 for v in high_out if v in b_set and v not in a_set), 2)
+        if tile < 2 and len(union) > 20:
+            continue
+        d_only = len(out_c - u_p)
+        t_sep = _step_time([len(nd.dev) for nd in pst.inputs], len(big.dev), len(u_p), bw, rate, launch) + \
+            _step_time([len(nd.dev) for nd in c.inputs], len(out_c), len(u_c), bw, rate, launch)
+        nbytes = 8 * (sum(nd.n for nd in pst.inputs) + d.n + c.out.n)
+        # products A B: every union assignment, except that one output bit only D
+        # carries shares them; products with D: every (output, outer) assignment
+        work = _step_work(2, len(union) - min(d_only, 1)) + _step_work(2, len(out_c) + ky)
+        t_chain = max(nbytes / bw, work / rate) + launch
+        if t_chain < 0.9 * t_sep:
+            a_, b_ = pst.inputs
+            members = list(getattr(pst, "members", [])) + list(getattr(c, "members", []))
+            merged[ci] = FusedStep([a_, b_, d], c.out, union, members, chain=True)
+            gone.add(pi)
+    return [merged.get(i, st) for i, st in enumerate(steps) if i not in gone]
+
+
 @dataclass
 class Plan:
     words: np.ndarray
@@ -517,7 +598,7 @@ class PlanCompiler:
 
     def __init__(self, lds_tensor_limit=1 << 12, lds_capacity=None, elem_bytes=8, lanes=64,
                  gather=True, window=None, gate_space="lds", schedule="list", fuse=False, reduce=True,
-                 bake_angles=False):
+                 bake_angles=False, chain=True):
         self.lds_tensor_limit = lds_tensor_limit
         # gates keep the angle they were composed with instead of rebuilding it
         # from the parameter vector (circuits that carry no parameter terms)
@@ -532,8 +613,10 @@ class PlanCompiler:
         if schedule not in ("levels", "alap", "first_fit", "list"):
             raise ValueError(f"unknown schedule {schedule!r}")
         self.schedule = schedule
-        # wide plans: contract chains of steps in one pass (see fuse_steps)
+        # wide plans: contract chains of steps in one pass (see fuse_steps), and
+        # producer/consumer pairs without storing the intermediate (chain_pairs)
         self.fuse = fuse
+        self.chain = chain
         # item plans: chains of single-input sums become one reduction op each
         self.reduce = reduce
         # "global": gates and every intermediate live in global memory (wide plans)
@@ -690,6 +773,8 @@ class PlanCompiler:
         if wide:
             if self.fuse:
                 exec_steps = split_small_inputs(fuse_steps(steps))
+                if self.chain:
+                    exec_steps = chain_pairs(exec_steps)
             ops, op_kind = [[i] for i in range(len(exec_steps))], [False] * len(exec_steps)
         else:
             if self.reduce:
@@ -943,7 +1028,8 @@ class PlanCompiler:
         for x in range(1 << k):
             u = _pdep(x, smask)
             words += [_pext(u, m) for m in masks]
-        words[0] = OP_FUSED | n_in | (sync_glb << 11) | (len(words) << 16)
+        chain = OP_CHAIN if getattr(fs, "chain", False) else 0
+        words[0] = OP_FUSED | chain | n_in | (sync_glb << 11) | (len(words) << 16)
         return words
 
     @staticmethod

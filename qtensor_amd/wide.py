@@ -99,11 +99,14 @@ class WidePlan:
         self.n_slices = 1 << n_slice_vars
 
 
-def compile_wide(circuit, order=None, par_ids=(), lanes=256, fuse=True, batch_vars=0, bake_angles=False):
+def compile_wide(circuit, order=None, par_ids=(), lanes=256, fuse=True, batch_vars=0, bake_angles=False,
+                 chain=True):
     """Compile ``circuit`` with elimination ``order`` (variable ids of the line
     graph; greedy when None) and slice variables ``par_ids`` into a wide plan.
     With ``fuse``, chains of steps are contracted in one pass where the time
-    model favours it (``plan.fuse_steps``).  ``batch_vars`` = b keeps the
+    model favours it (``plan.fuse_steps``), and with ``chain`` also
+    producer/consumer pairs without storing the intermediate
+    (``plan.chain_pairs``).  ``batch_vars`` = b keeps the
     output wires of qubits 0..b-1 open, as ``QtreeSimulator.simulate_batch``
     does (``qtensor/Simulate.py:66-94``): the plan's result is the 2^b
     amplitudes, qubit 0 the most significant bit.  Their variables join the
@@ -132,7 +135,7 @@ def compile_wide(circuit, order=None, par_ids=(), lanes=256, fuse=True, batch_va
     peo = list(tn.ket_vars) + bra + _vars(graph, rest) + _vars(graph, par_ids) + free
     buckets, perm = reorder_buckets(tn.buckets, peo)
     pc = PlanCompiler(lds_tensor_limit=0, gather=False, gate_space="global", lanes=lanes, fuse=fuse,
-                      bake_angles=bake_angles)
+                      bake_angles=bake_angles, chain=chain)
     plan = pc.compile(buckets, tn.data_dict, [perm[v] for v in tn.ket_vars],
                       [perm[v] for v in bra], [perm[v] for v in _vars(graph, par_ids)], tw,
                       free_vars=[perm[v] for v in free])
@@ -156,26 +159,42 @@ class WideEngine:
             self._threads = (n, ThreadPoolExecutor(max_workers=n, thread_name_prefix="qb-wide"))
         return self._threads[1]
 
-    def run(self, wplan, params, stream=None, timing=False, n_streams=4, graph=False):
+    # device memory that the arenas of slices run in lockstep may take (bytes)
+    LOCKSTEP_ARENA_BYTES = 8 << 30
+
+    def lockstep_slices(self, wplan):
+        """Slices per launch by default: all of them, as far as their arenas
+        fit in LOCKSTEP_ARENA_BYTES."""
+        elem = 8 if self.dtype == "complex64" else 16
+        fit = self.LOCKSTEP_ARENA_BYTES // max(wplan.plan.glb_need * elem, 1)
+        return int(max(1, min(wplan.n_slices, fit)))
+
+    def run(self, wplan, params, stream=None, timing=False, n_streams=4, graph=False, slices_per_launch=None):
         """Sum of the plan over all its slices (complex, FP64 accumulation): a
         complex scalar, or for a plan with b open variables a complex128 numpy
         array of the 2^b amplitudes.
 
-        Slices are independent: they are spread over ``n_streams`` streams, each
-        with a private arena, so the latency-bound small steps of one slice
-        overlap the bandwidth-bound large steps of another.  With ``timing``
-        also returns the device time in seconds.  With ``graph``, each stream's
-        launches are captured in a HIP graph the first time the plan runs and
-        replayed afterwards (new angles go into the captured angle buffer), so
-        a re-evaluation costs one graph launch per stream instead of a host
-        pass over every op."""
+        Slices are independent.  ``slices_per_launch`` = S consecutive slices
+        run in lockstep, every launch covering all S (S arenas side by side):
+        S times fewer launches (default ``lockstep_slices``: all slices whose
+        arenas fit in LOCKSTEP_ARENA_BYTES).  The groups of S slices are spread
+        over ``n_streams`` streams, each with private arenas, so the
+        latency-bound small steps of one group overlap the bandwidth-bound
+        large steps of another.  With ``timing`` also returns the device time in seconds.
+        With ``graph``, each stream's launches are captured in a HIP graph the
+        first time the plan runs and replayed afterwards (new angles go into
+        the captured angle buffer), so a re-evaluation costs one graph launch
+        per stream instead of a host pass over every op."""
+        if slices_per_launch is None:
+            slices_per_launch = self.lockstep_slices(wplan)
+        per = max(1, min(int(slices_per_launch), wplan.n_slices))
         if graph:
-            return self._run_graph(wplan, params, stream, timing, n_streams)
+            return self._run_graph(wplan, params, stream, timing, n_streams, per)
         import torch
         lib = _lib.init(self.device_index)
         dev = torch.device("cuda", self.device_index)
-        n_streams = max(1, min(n_streams, wplan.n_slices))
-        b = self._buffers(wplan, n_streams, len(params))
+        n_streams = max(1, min(n_streams, -(-wplan.n_slices // per)))
+        b = self._buffers(wplan, n_streams, len(params), per)
         if len(params):
             b["prm"][:len(params)] = torch.tensor([float(x) for x in params], dtype=torch.float64)
         main = stream if stream is not None else torch.cuda.current_stream(dev)
@@ -219,35 +238,40 @@ class WideEngine:
             return val, ev0.elapsed_time(ev1) * 1e-3
         return val
 
-    def run_many(self, wplans, params, stream=None, timing=False, n_streams=4):
+    def run_many(self, wplans, params, stream=None, timing=False, n_streams=4, slices_per_launch=None):
         """Values of several scalar wide plans for one angle vector (a list of
         complex, and the device time with ``timing``).  Every (plan, slice) is
-        independent: they are spread over ``n_streams`` streams, longest first
-        onto the least loaded stream (cost: the plan's executed-op bytes), each
-        stream with one arena large enough for its plans, so the launch-bound
-        small steps of many networks overlap; one segmented compensated FP64
-        reduction then gives each plan's sum over its slices."""
+        independent; up to ``slices_per_launch`` consecutive slices of a plan
+        run in lockstep in one call (one launch per op for all of them;
+        default: each plan's ``lockstep_slices``).  The
+        calls are spread over ``n_streams`` streams, longest first onto the
+        least loaded stream (cost: the plan's arena size times its slices),
+        each stream with one arena large enough for its calls, so the
+        launch-bound small steps of many networks overlap; one segmented
+        compensated FP64 reduction then gives each plan's sum over its
+        slices."""
         import torch
         lib = _lib.init(self.device_index)
         dev = torch.device("cuda", self.device_index)
         tdtype = torch.complex64 if self.dtype == "complex64" else torch.complex128
         if not wplans:
             return ([], 0.0) if timing else []
-        works, seg = [], [0]
+        works, seg = [], [0]          # works: (plan, first slice, slices)
         for i, wp in enumerate(wplans):
             if int(wp.plan.words[11]) != 0:
                 raise ValueError("run_many takes scalar plans (no open variables)")
-            for s in range(wp.n_slices):
-                works.append((i, s))
-            seg.append(len(works))
+            per = self.lockstep_slices(wp) if slices_per_launch is None else max(1, int(slices_per_launch))
+            for s in range(0, wp.n_slices, per):
+                works.append((i, s, min(per, wp.n_slices - s)))
+            seg.append(seg[-1] + wp.n_slices)
         n_streams = max(1, min(n_streams, len(works)))
-        cost = [float(max(wp.plan.glb_need, 1)) for wp in wplans]
-        order = sorted(range(len(works)), key=lambda w: -cost[works[w][0]])
+        cost = [float(max(wplans[i].plan.glb_need, 1)) * c for i, _, c in works]
+        order = sorted(range(len(works)), key=lambda w: -cost[w])
         load, per_stream = [0.0] * n_streams, [[] for _ in range(n_streams)]
         for w in order:
             k = min(range(n_streams), key=load.__getitem__)
             per_stream[k].append(w)
-            load[k] += cost[works[w][0]]
+            load[k] += cost[w]
         words, dev_words = [], []
         for wp in wplans:
             cached = getattr(wp, "_dev_words", None)
@@ -257,12 +281,13 @@ class WideEngine:
                 wp._dev_words = cached
             words.append(cached[1])
             dev_words.append(cached[2])
-        arenas = [torch.empty(max([max(wplans[works[w][0]].plan.glb_need, 1) for w in ws] + [1]), dtype=tdtype,
-                              device=dev) for ws in per_stream]
+        # a stream's arena: room for its largest call (slices x the plan's need)
+        arenas = [torch.empty(max([max(wplans[works[w][0]].plan.glb_need, 1) * works[w][2] for w in ws] + [1]),
+                              dtype=tdtype, device=dev) for ws in per_stream]
         prm = torch.zeros(len(params) + 8, dtype=torch.float64)
         prm[:len(params)] = torch.tensor([float(x) for x in params], dtype=torch.float64)
         prm = prm.to(dev)
-        out = torch.zeros(2 * len(works), dtype=torch.float64, device=dev)
+        out = torch.zeros(2 * seg[-1], dtype=torch.float64, device=dev)
         seg_dev = torch.tensor(seg, dtype=torch.int32, device=dev)
         term = torch.zeros(2 * len(wplans), dtype=torch.float64, device=dev)
         total = torch.zeros(2, dtype=torch.float64, device=dev)
@@ -278,17 +303,18 @@ class WideEngine:
             d = _lib.WideDesc()
             d.params = prm.data_ptr()
             d.arena = arenas[k].data_ptr()
-            d.arena_elems = arenas[k].numel()
             d.dtype = qb_dtype
             d.out_stride = 1
             h = ctypes.c_void_p(streams[k].cuda_stream)
             for w in per_stream[k]:
-                i, s = works[w]
+                i, s, c = works[w]
                 d.plan_host = words[i].ctypes.data
                 d.plan_words = words[i].size
                 d.plan_dev = dev_words[i].data_ptr()
+                d.arena_elems = arenas[k].numel() // c        # each slice's region
                 d.slice_bits = s
-                d.out = out.data_ptr() + 16 * w
+                d.n_slices = c
+                d.out = out.data_ptr() + 16 * (seg[i] + s)
                 _lib.check(lib.qb_wide_execute(ctypes.byref(d), h))
 
         if n_streams == 1:
@@ -315,9 +341,9 @@ class WideEngine:
 
     _GRAPH_PARAMS = 256        # angles a captured plan can take
 
-    def _buffers(self, wplan, n_streams, n_params):
-        """Device buffers of one evaluation (arenas, angles, per-slice results,
-        segment table, terms, total)."""
+    def _buffers(self, wplan, n_streams, n_params, per=1):
+        """Device buffers of one evaluation (arenas for ``per`` slices in
+        lockstep, angles, per-slice results, segment table, terms, total)."""
         import torch
         dev = torch.device("cuda", self.device_index)
         tdtype = torch.complex64 if self.dtype == "complex64" else torch.complex128
@@ -328,8 +354,10 @@ class WideEngine:
         if rank > 20:
             raise ValueError(f"{rank} open variables: at most 20")
         n_res = 1 << rank
+        need = max(plan.glb_need, 1)
         return {"words": words, "plan_dev": torch.from_numpy(words).to(dev), "rank": rank, "n_res": n_res,
-                "arenas": [torch.empty(max(plan.glb_need, 1), dtype=tdtype, device=dev) for _ in range(n_streams)],
+                "per": per, "arena_elems": need,
+                "arenas": [torch.empty(need * per, dtype=tdtype, device=dev) for _ in range(n_streams)],
                 "prm": torch.zeros(n_params + 8, dtype=torch.float64, device=dev),
                 "out": torch.zeros(2 * n * n_res, dtype=torch.float64, device=dev),
                 "seg": torch.arange(0, n * n_res + 1, n, dtype=torch.int32, device=dev),
@@ -337,36 +365,38 @@ class WideEngine:
                 "total": torch.zeros(2, dtype=torch.float64, device=dev)}
 
     def _issue(self, lib, b, wplan, streams, k, n_streams):
-        """Launch the slices of stream k (k, k + n_streams, ...) on it."""
-        n = wplan.n_slices
+        """Launch stream k's groups of slices on it: group g holds slices
+        [g per, (g + 1) per), and stream k takes groups k, k + n_streams, ..."""
+        n, per = wplan.n_slices, b["per"]
         d = _lib.WideDesc()
         d.plan_host = b["words"].ctypes.data
         d.plan_words = b["words"].size
         d.plan_dev = b["plan_dev"].data_ptr()
         d.params = b["prm"].data_ptr()
         d.arena = b["arenas"][k].data_ptr()
-        d.arena_elems = b["arenas"][k].numel()
+        d.arena_elems = b["arena_elems"]
         d.dtype = _lib.QB_C64 if self.dtype == "complex64" else _lib.QB_C128
         d.out_stride = n
         h = ctypes.c_void_p(streams[k].cuda_stream)
-        for s in range(k, n, n_streams):
+        for s in range(k * per, n, n_streams * per):
             d.slice_bits = s
+            d.n_slices = min(per, n - s)
             d.out = b["out"].data_ptr() + 16 * s
             _lib.check(lib.qb_wide_execute(ctypes.byref(d), h))
 
-    def _run_graph(self, wplan, params, stream, timing, n_streams):
+    def _run_graph(self, wplan, params, stream, timing, n_streams, per):
         import torch
         lib = _lib.init(self.device_index)
         dev = torch.device("cuda", self.device_index)
         if len(params) > self._GRAPH_PARAMS:
             raise ValueError(f"{len(params)} angles: a captured plan takes at most {self._GRAPH_PARAMS}")
-        n_streams = max(1, min(n_streams, wplan.n_slices))
+        n_streams = max(1, min(n_streams, -(-wplan.n_slices // per)))
         if not hasattr(self, "_graphs"):
             self._graphs = {}
-        key = (id(wplan), n_streams)
+        key = (id(wplan), n_streams, per)
         st = self._graphs.get(key)
         if st is None or st["plan"] is not wplan:
-            b = self._buffers(wplan, n_streams, self._GRAPH_PARAMS)
+            b = self._buffers(wplan, n_streams, self._GRAPH_PARAMS, per)
             streams = [torch.cuda.Stream(device=dev) for _ in range(n_streams)]
             torch.cuda.synchronize(dev)
             for k in range(n_streams):          # an eager pass first: kernels are loaded outside capture

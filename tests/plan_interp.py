@@ -128,6 +128,16 @@ def run_plan(words, params, slice_bits=0, dtype=np.complex128, lanes=64):
             x_tab = np.array(w[tab + (n_in << lo_bits): tab + (n_in << lo_bits) + (n_in << k)],
                              dtype=np.int64).reshape(-1, n_in)
             assert pos + length == tab + (n_in << lo_bits) + (n_in << k)
+            if w0 & P.OP_CHAIN:
+                # the device's factorisation: sum over y of D times (sum over x
+                # of A B), x = the (at most CHAIN_MAX_X lowest) summed variables D
+                # lacks; the value is the same sum as any fused op's
+                not_d = smask & ~ins[2][1]
+                kx = min(bin(not_d).count("1"), P.CHAIN_MAX_X)
+                assert n_in == 3 and rank_out >= 8 and kx >= 1 and k - kx <= P.CHAIN_MAX_Y
+                assert bin(ins[2][1]).count("1") <= P.CHAIN_MAX_D and rank_u <= 31
+            else:
+                assert rank_u <= 30
             o = np.arange(1 << rank_out, dtype=np.int64)
             x = np.arange(1 << k, dtype=np.int64)
             # the device splits o into its low lo_bits (table) and the rest

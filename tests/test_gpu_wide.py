@@ -183,3 +183,110 @@ def test_wide_run_many_matches_run(gpu, dtype, tol):
     for wp, v, r in zip(plans, vals, refs):
         assert v == eng.run(wp, g + b)
         assert abs(v - r) <= tol
+
+
+def _chain_count(wp):
+    from qtensor_amd import plan as P
+    w = np.asarray(wp.plan.words)
+    pos, n = int(w[3]), 0
+    for _ in range(int(w[1])):
+        w0 = int(w[pos]) & 0xFFFFFFFF
+        if (w0 & P.OP_FUSED) and (w0 & P.OP_CHAIN):
+            n += 1
+        pos += w0 >> 16
+    return n
+
+
+def _forced_p3_plans(tw=20, n_slice=0):
+    """A p=3 lightcone of N=64 under a deliberately wide order, compiled with
+    and without chain ops."""
+    G = nx.random_regular_graph(3, 64, seed=3)
+    rng = np.random.default_rng(3)
+    g, b = list(rng.uniform(0, np.pi, 3)), list(rng.uniform(0, np.pi / 2, 3))
+    e = list(G.edges())[0]
+    comp = QtreeQAOAComposer(G, gamma=g, beta=b)
+    comp.energy_expectation_lightcone(e)
+    graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+    order = forced_width_order(graph, target_tw=tw, temp=2.0, seed=0)
+    par = slice_vars_for_order(graph, order, n_slice) if n_slice else []
+    chained = compile_wide(comp.circuit, order=order, par_ids=par)
+    plain = compile_wide(comp.circuit, order=order, par_ids=par, chain=False)
+    return G, e, g, b, chained, plain
+
+
+@pytest.mark.parametrize("n_slice", [0, 2])
+def test_chain_ops_match_separate_contractions(gpu, n_slice):
+    """Chain ops (a contraction and its consumer in one kernel, the
+    intermediate never stored) give the value of the same plan compiled
+    without them: complex128 to 1e-12, and both precisions match the oracle's
+    contraction of the edge's network."""
+    from oracle import qaoa_network
+    G, e, g, b, chained, plain = _forced_p3_plans(n_slice=n_slice)
+    assert _chain_count(chained) > 0 and _chain_count(plain) == 0
+    e128 = WideEngine(dtype="complex128")
+    v_chain, v_plain = e128.run(chained, g + b), e128.run(plain, g + b)
+    assert abs(v_chain - v_plain) < 1e-12
+    ref = qaoa_network.edge_zz(G, e, g, b)
+    assert abs(v_chain - ref) < 1e-11
+    assert abs(WideEngine(dtype="complex64").run(chained, g + b) - ref) < 1e-5
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex64", 1e-6), ("complex128", 1e-13)])
+def test_slices_per_launch_match_one_slice_per_launch(gpu, dtype, tol):
+    """Slices run in lockstep (one launch per op covering 2, 4 or all 8 slices,
+    their arenas side by side), eagerly and replayed from captured graphs,
+    equal one launch per slice; also for a plan with open variables (each
+    slice's amplitudes at their own stride) and for several plans at once."""
+    G, e, g, b, circ = _edge_circuit()
+    graph = QtreeTensorNet.from_qtree_gates(circ).get_line_graph()
+    order = forced_width_order(graph, target_tw=14, temp=3.0, seed=1)
+    wp = compile_wide(circ, order=order, par_ids=slice_vars_for_order(graph, order, 3))
+    assert wp.n_slices == 8
+    eng = WideEngine(dtype=dtype)
+    base = eng.run(wp, g + b, n_streams=4, slices_per_launch=1)
+    assert abs(base - statevector.zz_expectations(G, g, b)[e]) <= max(tol, 1e-11) * 10
+    for ns, per in ((1, 8), (2, 4), (4, 2), (1, 3)):
+        assert abs(eng.run(wp, g + b, n_streams=ns, slices_per_launch=per) - base) <= tol
+        assert abs(eng.run(wp, g + b, n_streams=ns, slices_per_launch=per, graph=True) - base) <= tol
+    from qtensor_amd.ordering import greedy_peo
+    G2 = nx.random_regular_graph(3, 12, seed=4)
+    comp = QtreeQAOAComposer(G2, gamma=[0.3, 0.5], beta=[0.2, 0.7])
+    comp.ansatz_state()
+    graph2 = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+    order2, _ = greedy_peo(graph2)
+    ws = compile_wide(comp.circuit, order=order2, par_ids=slice_vars_for_order(graph2, order2, 2), batch_vars=2)
+    amp1 = np.atleast_1d(eng.run(ws, [0.3, 0.5, 0.2, 0.7], slices_per_launch=1))
+    amp4 = np.atleast_1d(eng.run(ws, [0.3, 0.5, 0.2, 0.7], n_streams=1, slices_per_launch=4))
+    assert amp1.shape == (4,) and np.max(np.abs(amp4 - amp1)) <= tol
+    many1 = eng.run_many([wp, wp], g + b, slices_per_launch=1)
+    many4 = eng.run_many([wp, wp], g + b, slices_per_launch=4)
+    assert max(abs(x - base) for x in many1 + many4) <= tol
+
+
+def test_wide_rejects_malformed_chain_op(gpu):
+    """The chain flag on a fused op without a third input is refused on the host."""
+    import torch
+    from qtensor_amd import plan as P
+    G, e, g, b, chained, plain = _forced_p3_plans()
+    words = np.ascontiguousarray(plain.plan.words, dtype=np.int32).copy()
+    pos, tampered = int(words[3]), False
+    for _ in range(int(words[1])):
+        w0 = int(words[pos]) & 0xFFFFFFFF
+        if (w0 & P.OP_FUSED) and (w0 & 15) == 2:
+            words[pos] = np.int32(np.uint32(w0 | P.OP_CHAIN).view(np.int32))
+            tampered = True
+            break
+        pos += w0 >> 16
+    assert tampered
+    lib = _lib.init(0)
+    dev = torch.from_numpy(words).cuda()
+    arena = torch.empty(max(plain.plan.glb_need, 1), dtype=torch.complex64, device="cuda")
+    prm = torch.tensor(g + b, dtype=torch.float64, device="cuda")
+    out = torch.zeros(2, dtype=torch.float64, device="cuda")
+    d = _lib.WideDesc()
+    d.plan_host, d.plan_words, d.plan_dev = words.ctypes.data, words.size, dev.data_ptr()
+    d.params, d.arena, d.arena_elems = prm.data_ptr(), arena.data_ptr(), arena.numel()
+    d.slice_bits, d.dtype, d.out = 0, _lib.QB_C64, out.data_ptr()
+    rc = lib.qb_wide_execute(ctypes.byref(d), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    assert rc == _lib.QB_ERR_INVALID and b"chain op" in lib.qb_last_error()

@@ -231,6 +231,42 @@ def test_batch_vars_amplitudes_match_single_amplitude():
     assert np.allclose(amps, [sv[k << (nq - 2)] for k in range(4)], atol=1e-10)
 
 
+def test_chain_plan_program_vs_oracle():
+    """Wide plans with chain ops (a contraction and the contraction reading its
+    output merged into one op, the intermediate never stored) interpret to the
+    statevector value, sliced or not, and to the value of the same order
+    compiled without them; the interpreter checks each chain op against the
+    device kernel's structural limits."""
+    from qtensor_amd import plan as P
+    from qtensor_amd.wide import compile_wide, forced_width_order, slice_vars_for_order
+    from qtensor_amd.tensornet import QtreeTensorNet
+    G = nx.random_regular_graph(3, 14, seed=2)
+    g, b = [0.3, 0.7], [0.5, 0.1]
+    e = list(G.edges())[1]
+    comp = QtreeQAOAComposer(G, gamma=g, beta=b)
+    comp.energy_expectation_lightcone(e)
+    ref = statevector.zz_expectations(G, g, b)[e]
+    graph = QtreeTensorNet.from_qtree_gates(comp.circuit).get_line_graph()
+    tw0 = compile_wide(comp.circuit).treewidth
+    n_chain = 0
+    for extra, n_par in ((3, 0), (5, 0), (7, 0), (7, 2)):
+        order = forced_width_order(graph, target_tw=tw0 + extra, temp=3.0, seed=1)
+        par = slice_vars_for_order(graph, order, n_par) if n_par else []
+        for chain in (True, False):
+            wp = compile_wide(comp.circuit, order=order, par_ids=par, chain=chain)
+            w = np.asarray(wp.plan.words)
+            pos, found = int(w[3]), 0
+            for _ in range(int(w[1])):
+                w0 = int(w[pos]) & 0xFFFFFFFF
+                found += bool((w0 & P.OP_FUSED) and (w0 & P.OP_CHAIN))
+                pos += w0 >> 16
+            assert chain or found == 0
+            n_chain += found
+            total = sum(run_plan(wp.plan.words, g + b, slice_bits=s, lanes=256) for s in range(wp.n_slices))
+            assert abs(total - ref) < 1e-10
+    assert n_chain >= 3
+
+
 @pytest.mark.parametrize("fuse", [False, True], ids=["steps", "fused"])
 def test_wide_plan_program_vs_oracle(fuse):
     """Wide plans (all tensors global, step ops only, or with fused chains)
