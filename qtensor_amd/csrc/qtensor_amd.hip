@@ -528,56 +528,87 @@ __global__ __launch_bounds__(256) void fused2_kernel(const FusedArgs a, const in
 // out = sum T D consuming it).  With x = the (at most kChainMaxX lowest)
 // summed variables that D does not carry and y = the other summed variables,
 // the sum factorises as out[o] = sum_y D[d(o, y)] (sum_x A[a(o, y, x)] B[b(o, y, x)]),
-// so each inner sum is formed in registers and T is never stored.  Register tile: up to two high
-// output bits that A carries and B does not (RR), as many of B's, and one
-// output bit that only D carries (RD): per outer assignment y a thread forms
-// 2^RR x 2^RR values of T (2^RR loads of A and of B per inner assignment) and
-// multiplies each into 2^RD outputs; D (at most 2^kChainMaxD elements) is
-// staged in LDS.  Blocks visit the remaining high bits hk through deposit runs.
+// so each inner sum is formed in registers and T is never stored.
+//
+// Output bits are split three ways by the host.  Register bits: up to two
+// output bits that A carries and B does not (RR), as many of B's, and one that
+// only D carries (RD); per (y, x) a thread loads 2^RR values of A and of B and
+// forms their 2^2RR products, each multiplied into 2^RD outputs after the
+// inner sum.  With PAIR (complex64), A's and B's lowest variables are such
+// register bits, so each pair of register values is adjacent and one 16-byte
+// load fetches both: the address units take four lanes per cycle whatever the
+// width up to 16 bytes, so pairs halve their work, which bounds this kernel.
+// Lane bits: the lowest 8 other output bits (thread t's offsets come from
+// pdep(t, lane_mask)).  Block bits: the rest, visited through bit runs per
+// target.  D (at most 2^kChainMaxD elements) and the byte offsets of every
+// (y, x) are staged in LDS.
 constexpr uint32_t kFusedChain = 1u << 12;
-constexpr int kChainMaxX = 4, kChainMaxY = 5, kChainMaxD = 10;
+constexpr int kChainMaxX = 4, kChainMaxY = 5, kChainMaxD = 10, kChainRuns = 8;
 
 struct ChainArgs {
     uint32_t xoff[2][1 << kChainMaxX];   // byte offset in A, B of inner summed assignment x
     uint32_t yoff[3][1 << kChainMaxY];   // byte offset in A, B (element offset in D) of outer assignment y
-    uint32_t roff_a[4], roff_b[4];       // byte offset in A (B) of register pattern i of A's (B's) bits
-    uint32_t droff_a[4], droff_b[4], droff_d[2];  // offset in D of each register pattern
-    uint32_t rdep_a[4], rdep_b[4], rdep_d[2];     // high-bit pattern of each register pattern
-    uint32_t dep_run[kFusedDepRuns];
-    uint32_t n_dep, hk_bits, kx, ky, d_elems;
+    uint32_t roff_a[4], roff_b[4];       // byte offset in A (B) of register pattern p
+    uint32_t droff_a[4], droff_b[4], droff_d[2];  // element offset in D of each register pattern
+    uint32_t oroff_a[4], oroff_b[4], oroff_d[2];  // element offset in the output of each register pattern
+    uint32_t hk_run[4][kChainRuns];      // block bits -> offset bits in A, B, D, out (src | w << 5 | dst << 10)
+    uint32_t hk_nrun[4];
+    uint32_t lane_mask;                  // union bits of the 8 lane bits
+    uint32_t hk_bits, kx, ky, d_elems;
 };
 
-template <typename R, int RR, int RD>
-__global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const int32_t* __restrict__ lo_tab,
-                                                    const ChainArgs e) {
+__device__ __forceinline__ uint32_t chain_runs(const uint32_t* run, uint32_t n, uint32_t hk) {
+    uint32_t off = 0;
+#pragma unroll
+    for (int r = 0; r < kChainRuns; ++r) {
+        if ((uint32_t)r >= n) break;
+        const uint32_t f = run[r];
+        off |= bfe(hk, f & 31u, (f >> 5) & 31u) << (f >> 10);
+    }
+    return off;
+}
+
+template <typename R, int RR, int RD, bool PAIR>
+__global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const ChainArgs e) {
     using V = typename Cx<R>::V;
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    static_assert(!PAIR || (sizeof(V) == 8 && RR >= 1), "pairs: complex64 with a register bit per input");
     constexpr int NR = 1 << RR, ND = 1 << RD;
     __shared__ V dsh[1 << kChainMaxD];
-    const uint32_t tid = threadIdx.x;           // lo_bits = 8: every thread owns an output
+    // byte offsets in A and B of summed assignment it = y << kx | x
+    __shared__ uint2 xy_off[1 << (kChainMaxX + kChainMaxY)];
+    const uint32_t tid = threadIdx.x;
     const int64_t zo = (int64_t)blockIdx.z * a.zbytes;       // this block's slice
     const V* __restrict__ dsrc = reinterpret_cast<const V*>(a.in[2] + zo);
     for (uint32_t i = tid; i < e.d_elems; i += 256u) dsh[i] = dsrc[i];
+    const uint32_t n_x = 1u << e.kx, n_y = 1u << e.ky, n_hk = 1u << e.hk_bits;
+    const uint32_t n_it = n_y << e.kx;
+    for (uint32_t it = tid; it < n_it; it += 256u) {
+        const uint32_t x = it & (n_x - 1u), y = it >> e.kx;
+        xy_off[it] = make_uint2(e.yoff[0][y] + e.xoff[0][x], e.yoff[1][y] + e.xoff[1][x]);
+    }
     __syncthreads();
-    const uint32_t lo_a = (uint32_t)lo_tab[tid * 3], lo_b = (uint32_t)lo_tab[tid * 3 + 1],
-                   lo_d = (uint32_t)lo_tab[tid * 3 + 2];
-    // A and B (each below 4 GiB) are read through 32-bit byte offsets from a
+    constexpr uint32_t EB = sizeof(V);
+    // this lane's part of every offset
+    const uint32_t ul = pdep_loop(tid, e.lane_mask);
+    const uint32_t la = pext_loop(ul, a.mask[0]) * EB, lb = pext_loop(ul, a.mask[1]) * EB;
+    const uint32_t ld = pext_loop(ul, a.mask[2]), lo = pext_loop(ul, a.omask);
+    // A and B (each below 2 GiB) are read through 32-bit byte offsets from a
     // block-uniform base: one vector add per load
     const char* __restrict__ in_a = a.in[0] + zo;
     const char* __restrict__ in_b = a.in[1] + zo;
     V* out = reinterpret_cast<V*>(a.out + zo);
-    constexpr uint32_t EB = sizeof(V);
-    const uint32_t n_x = 1u << e.kx, n_y = 1u << e.ky, n_hk = 1u << e.hk_bits;
-    for (uint32_t hk = blockIdx.x; hk < n_hk; hk += gridDim.x) {
-        uint32_t hi = 0;
+    uint32_t ra[NR], rb[NR];            // register patterns' byte offsets (uniform)
 #pragma unroll
-        for (int r = 0; r < kFusedDepRuns; ++r) {
-            if ((uint32_t)r >= e.n_dep) break;
-            const uint32_t f = e.dep_run[r];
-            hi |= bfe(hk, f & 31u, (f >> 5) & 31u) << (f >> 10);
-        }
-        const uint32_t oa = (lo_a + fused_hi_offset<3>(a, 0, hi)) * EB;
-        const uint32_t ob = (lo_b + fused_hi_offset<3>(a, 1, hi)) * EB;
-        const uint32_t od = lo_d + fused_hi_offset<3>(a, 2, hi);
+    for (int i = 0; i < NR; ++i) {
+        ra[i] = e.roff_a[i];
+        rb[i] = e.roff_b[i];
+    }
+    for (uint32_t hk = blockIdx.x; hk < n_hk; hk += gridDim.x) {
+        const uint32_t oa = la + chain_runs(e.hk_run[0], e.hk_nrun[0], hk) * EB;
+        const uint32_t ob = lb + chain_runs(e.hk_run[1], e.hk_nrun[1], hk) * EB;
+        const uint32_t od = ld + chain_runs(e.hk_run[2], e.hk_nrun[2], hk);
+        const uint32_t oo = lo + chain_runs(e.hk_run[3], e.hk_nrun[3], hk);
         V acc[NR][NR][ND];
 #pragma unroll
         for (int i = 0; i < NR; ++i)
@@ -585,33 +616,54 @@ __global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const int
             for (int j = 0; j < NR; ++j)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) acc[i][j][d] = V{0, 0};
-        for (uint32_t y = 0; y < n_y; ++y) {
-            V t[NR][NR];
+        V t[NR][NR];
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+#pragma unroll
+            for (int j = 0; j < NR; ++j) t[i][j] = V{0, 0};
+        for (uint32_t it = 0; it < n_it; ++it) {
+            const uint2 xo = xy_off[it];
+            const uint32_t xa = oa + xo.x, xb = ob + xo.y;
+            V va[NR], vb[NR];
+            if constexpr (PAIR) {
+                // register pattern bit 0 is the input's lowest variable: p and
+                // p + 1 are adjacent elements
+#pragma unroll
+                for (int i = 0; i < NR; i += 2) {
+                    const f32x4 q = *reinterpret_cast<const f32x4*>(in_a + (xa + ra[i]));
+                    va[i] = V{q[0], q[1]};
+                    va[i + 1] = V{q[2], q[3]};
+                }
+#pragma unroll
+                for (int j = 0; j < NR; j += 2) {
+                    const f32x4 q = *reinterpret_cast<const f32x4*>(in_b + (xb + rb[j]));
+                    vb[j] = V{q[0], q[1]};
+                    vb[j + 1] = V{q[2], q[3]};
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < NR; ++i) va[i] = *reinterpret_cast<const V*>(in_a + (xa + ra[i]));
+#pragma unroll
+                for (int j = 0; j < NR; ++j) vb[j] = *reinterpret_cast<const V*>(in_b + (xb + rb[j]));
+            }
 #pragma unroll
             for (int i = 0; i < NR; ++i)
 #pragma unroll
-                for (int j = 0; j < NR; ++j) t[i][j] = V{0, 0};
-            const uint32_t ya = oa + e.yoff[0][y], yb = ob + e.yoff[1][y];
-            for (uint32_t x = 0; x < n_x; ++x) {
-                const uint32_t xa = ya + e.xoff[0][x], xb = yb + e.xoff[1][x];
-                V va[NR], vb[NR];
-#pragma unroll
-                for (int i = 0; i < NR; ++i) va[i] = *reinterpret_cast<const V*>(in_a + (xa + e.roff_a[i]));
-#pragma unroll
-                for (int j = 0; j < NR; ++j) vb[j] = *reinterpret_cast<const V*>(in_b + (xb + e.roff_b[j]));
+                for (int j = 0; j < NR; ++j) t[i][j] = cfma(t[i][j], va[i], vb[j]);
+            if (((it + 1u) & (n_x - 1u)) == 0u) {
+                // the inner sum of outer assignment y is complete: multiply into D
+                const uint32_t yd = od + e.yoff[2][it >> e.kx];
 #pragma unroll
                 for (int i = 0; i < NR; ++i)
 #pragma unroll
-                    for (int j = 0; j < NR; ++j) t[i][j] = cfma(t[i][j], va[i], vb[j]);
+                    for (int j = 0; j < NR; ++j) {
+#pragma unroll
+                        for (int d = 0; d < ND; ++d)
+                            acc[i][j][d] = cfma(acc[i][j][d], t[i][j],
+                                                dsh[yd + e.droff_a[i] + e.droff_b[j] + e.droff_d[d]]);
+                        t[i][j] = V{0, 0};
+                    }
             }
-            const uint32_t yd = od + e.yoff[2][y];
-#pragma unroll
-            for (int i = 0; i < NR; ++i)
-#pragma unroll
-                for (int j = 0; j < NR; ++j)
-#pragma unroll
-                    for (int d = 0; d < ND; ++d)
-                        acc[i][j][d] = cfma(acc[i][j][d], t[i][j], dsh[yd + e.droff_a[i] + e.droff_b[j] + e.droff_d[d]]);
         }
 #pragma unroll
         for (int i = 0; i < NR; ++i)
@@ -619,7 +671,7 @@ __global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const int
             for (int j = 0; j < NR; ++j)
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
-                    out[((uint64_t)(hi | e.rdep_a[i] | e.rdep_b[j] | e.rdep_d[d]) << 8) | tid] = acc[i][j][d];
+                    out[oo + e.oroff_a[i] + e.oroff_b[j] + e.oroff_d[d]] = acc[i][j][d];
     }
 }
 
@@ -1732,111 +1784,176 @@ static bool deposit_runs(uint32_t keep, uint32_t hi_bits, uint32_t* dep_run, uin
     return true;
 }
 
+// Bit runs mapping the block bits (the set bits of hmask, in order, as the
+// bits of hk) to offset bits of a tensor over `mask`: the kernel's chain_runs
+// must equal pext(pdep(hk, hmask), mask); checked for every bit and all of them.
+static bool chain_hk_runs(uint32_t hmask, uint32_t mask, uint32_t* run, uint32_t* n_run) {
+    uint32_t n = 0, k = 0, prev_k = 0, prev_t = 0;
+    for (uint32_t m = hmask; m; m &= m - 1, ++k) {
+        const uint32_t u = (uint32_t)__builtin_ctz(m);
+        if (!((mask >> u) & 1u)) continue;
+        const uint32_t t = (uint32_t)__builtin_popcount(mask & ((1u << u) - 1u));
+        if (n > 0 && k == prev_k + 1 && t == prev_t + 1) {
+            run[n - 1] += 1u << 5;
+        } else {
+            if (n >= (uint32_t)kChainRuns) return false;
+            run[n++] = k | (1u << 5) | (t << 10);
+        }
+        prev_k = k;
+        prev_t = t;
+    }
+    const uint32_t bits = (uint32_t)__builtin_popcount(hmask);
+    for (uint32_t b = 0; b <= bits; ++b) {
+        const uint32_t hk = b < bits ? (1u << b) : (bits >= 32 ? 0xffffffffu : (1u << bits) - 1u);
+        uint32_t got = 0;
+        for (uint32_t r = 0; r < n; ++r) {
+            const uint32_t f = run[r], w = (f >> 5) & 31u;
+            got |= ((hk >> (f & 31u)) & ((1u << w) - 1u)) << (f >> 10);
+        }
+        if (got != pext_host(pdep_host(hk, hmask), mask)) return false;
+    }
+    *n_run = n;
+    return true;
+}
+
 // Launch a chain op (kFusedChain): inputs A, B, D; the inner summed variables
-// are those D lacks.  `a` holds the checked inputs, masks and high-bit runs.
-static int launch_chain(const FusedArgs& a, const int32_t* lo_tab, uint32_t n_in, uint32_t smask, int32_t k_op,
-                        int32_t dtype, hipStream_t st, int nz) {
-    const uint32_t rank_out = a.rank_out, lo_bits = a.lo_bits, omask = a.omask;
-    // inner: the lowest (at most kChainMaxX) summed variables D lacks; outer: the rest
+// are the (at most kChainMaxX lowest) summed variables D lacks.  `a` holds the
+// checked inputs and masks.
+static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32_t k_op, int32_t dtype,
+                        hipStream_t st, int nz) {
+    const uint32_t rank_out = a.rank_out, omask = a.omask;
+    const uint32_t ma = a.mask[0], mb = a.mask[1], md = a.mask[2];
     uint32_t xbits = 0;
-    for (uint32_t m = smask & ~a.mask[2], n = 0; m && n < (uint32_t)kChainMaxX; m &= m - 1, ++n) xbits |= m & (0u - m);
+    for (uint32_t m = smask & ~md, n = 0; m && n < (uint32_t)kChainMaxX; m &= m - 1, ++n) xbits |= m & (0u - m);
     const uint32_t ybits = smask & ~xbits;
     const uint32_t kx = (uint32_t)__builtin_popcount(xbits), ky = (uint32_t)__builtin_popcount(ybits);
-    const uint32_t rd = (uint32_t)__builtin_popcount(a.mask[2]);
-    if (n_in != 3 || lo_bits != 8 || kx < 1 || kx > (uint32_t)kChainMaxX || ky > (uint32_t)kChainMaxY ||
+    const uint32_t rd = (uint32_t)__builtin_popcount(md);
+    if (n_in != 3 || rank_out < 8 || kx < 1 || kx > (uint32_t)kChainMaxX || ky > (uint32_t)kChainMaxY ||
         rd > (uint32_t)kChainMaxD)
         return set_error(QB_ERR_INVALID, "chain op %d: %u inputs, rank %u, %u inner and %u outer summed variables, "
                          "third input of rank %u", k_op, n_in, rank_out, kx, ky, rd);
+    const uint32_t eb = dtype == QB_C128 ? 16u : 8u;
+    // A and B are addressed with 32-bit byte offsets
+    if (__builtin_popcount(ma) + (eb == 16u ? 4 : 3) > 31 || __builtin_popcount(mb) + (eb == 16u ? 4 : 3) > 31)
+        return set_error(QB_ERR_INVALID, "chain op %d: an input of 2 GiB or more", k_op);
     ChainArgs e;
     memset(&e, 0, sizeof(e));
     e.kx = kx;
     e.ky = ky;
     e.d_elems = 1u << rd;
-    const uint32_t eb = dtype == QB_C128 ? 16u : 8u;
-    // A and B are addressed with 32-bit byte offsets
-    if (__builtin_popcount(a.mask[0]) + (eb == 16u ? 4 : 3) > 31 || __builtin_popcount(a.mask[1]) + (eb == 16u ? 4 : 3) > 31)
-        return set_error(QB_ERR_INVALID, "chain op %d: an input of 2 GiB or more", k_op);
     for (uint32_t x = 0; x < (1u << kx); ++x) {
         const uint32_t u = pdep_host(x, xbits);
-        e.xoff[0][x] = pext_host(u, a.mask[0]) * eb;
-        e.xoff[1][x] = pext_host(u, a.mask[1]) * eb;
+        e.xoff[0][x] = pext_host(u, ma) * eb;
+        e.xoff[1][x] = pext_host(u, mb) * eb;
     }
     for (uint32_t y = 0; y < (1u << ky); ++y) {
         const uint32_t u = pdep_host(y, ybits);
-        e.yoff[0][y] = pext_host(u, a.mask[0]) * eb;
-        e.yoff[1][y] = pext_host(u, a.mask[1]) * eb;
-        e.yoff[2][y] = pext_host(u, a.mask[2]);
+        e.yoff[0][y] = pext_host(u, ma) * eb;
+        e.yoff[1][y] = pext_host(u, mb) * eb;
+        e.yoff[2][y] = pext_host(u, md);
     }
-    const uint32_t hi_bits = rank_out - lo_bits;
-    uint32_t ubit[32], n_ob = 0;
-    for (uint32_t u = 0; u < 32; ++u)
-        if ((omask >> u) & 1u) ubit[n_ob++] = u;
-    // register bits (highest first): A's and B's in equal numbers (at most 2
-    // each), one that only D carries; fewer when the rest would split into more
-    // deposit runs than the kernel takes
-    uint32_t rr = 0, rdb = 0, reg_a[2], reg_b[2], reg_d[1];
+    // output bits that only A (B, D) carries, highest first
     uint32_t ca[32], cb[32], cd[32], na = 0, nb = 0, nd = 0;
-    for (uint32_t b = hi_bits; b-- > 0;) {
-        const uint32_t u = ubit[b + lo_bits];
-        const bool in_a = (a.mask[0] >> u) & 1u, in_b = (a.mask[1] >> u) & 1u, in_d = (a.mask[2] >> u) & 1u;
-        if (in_a && !in_b) ca[na++] = b;
-        else if (in_b && !in_a) cb[nb++] = b;
-        else if (in_d && !in_a && !in_b) cd[nd++] = b;
+    for (int u = 31; u >= 0; --u) {
+        if (!((omask >> u) & 1u)) continue;
+        const bool in_a = (ma >> u) & 1u, in_b = (mb >> u) & 1u, in_d = (md >> u) & 1u;
+        if (in_a && !in_b) ca[na++] = (uint32_t)u;
+        else if (in_b && !in_a) cb[nb++] = (uint32_t)u;
+        else if (in_d && !in_a && !in_b) cd[nd++] = (uint32_t)u;
     }
+    // pairs: A's and B's lowest variables are output bits only they carry
+    const uint32_t pa = ma ? (uint32_t)__builtin_ctz(ma) : 32u, pb = mb ? (uint32_t)__builtin_ctz(mb) : 32u;
+    // (16-byte loads: A, B and every slice's region 16-byte aligned)
+    const bool aligned = ((reinterpret_cast<uintptr_t>(a.in[0]) | reinterpret_cast<uintptr_t>(a.in[1]) |
+                           (uintptr_t)a.zbytes) & 15u) == 0;
+    const bool can_pair = dtype == QB_C64 && aligned && pa < 32u && pb < 32u && ((omask >> pa) & 1u) &&
+                          !((mb >> pa) & 1u) && ((omask >> pb) & 1u) && !((ma >> pb) & 1u);
     bool ok = false;
-    for (int want_r = 2; want_r >= 0 && !ok; --want_r) {
-        for (int want_d = 1; want_d >= 0 && !ok; --want_d) {
-            rr = (uint32_t)want_r;
-            if (rr > na) rr = na;
-            if (rr > nb) rr = nb;
-            rdb = (uint32_t)want_d < nd ? (uint32_t)want_d : nd;
-            uint32_t regmask = 0;
-            for (uint32_t i = 0; i < rr; ++i) {
-                reg_a[i] = ca[i];
-                reg_b[i] = cb[i];
-                regmask |= (1u << ca[i]) | (1u << cb[i]);
-            }
-            for (uint32_t i = 0; i < rdb; ++i) {
-                reg_d[i] = cd[i];
-                regmask |= 1u << cd[i];
-            }
-            const uint32_t keep = (hi_bits >= 32 ? 0xffffffffu : ((1u << hi_bits) - 1u)) & ~regmask;
-            ok = deposit_runs(keep, hi_bits, e.dep_run, &e.n_dep, &e.hk_bits);
+    uint32_t rr = 0, rdb = 0, ramask = 0, rbmask = 0, rdmask = 0;
+    bool pair = false;
+    for (int cfg = 0; cfg < 8 && !ok; ++cfg) {
+        // configurations by preference: pairs with 2 or 1 register bits per
+        // input, then without pairs 2, 1, 0; each with and without a D bit
+        const bool want_pair = cfg < 4;
+        const uint32_t want_r = cfg < 4 ? (cfg < 2 ? 2u : 1u) : (cfg < 6 ? 2u : 1u);
+        const uint32_t want_d = (cfg & 1) ? 0u : 1u;
+        if (want_pair && !can_pair) continue;
+        pair = want_pair;
+        ramask = rbmask = rdmask = 0;
+        uint32_t ga = 0, gb = 0;
+        if (pair) {
+            ramask = 1u << pa;
+            rbmask = 1u << pb;
+            ga = gb = 1;
         }
+        for (uint32_t i = 0; i < na && ga < want_r; ++i)
+            if (!((ramask >> ca[i]) & 1u)) { ramask |= 1u << ca[i]; ++ga; }
+        for (uint32_t i = 0; i < nb && gb < want_r; ++i)
+            if (!((rbmask >> cb[i]) & 1u)) { rbmask |= 1u << cb[i]; ++gb; }
+        rr = ga < gb ? ga : gb;
+        if (pair && rr < 1) continue;
+        // equal counts: drop the extra non-pair bits
+        while ((uint32_t)__builtin_popcount(ramask) > rr) {
+            const uint32_t hi = 31u - (uint32_t)__builtin_clz(ramask);
+            ramask &= ~(1u << (hi == pa && pair ? 31u - (uint32_t)__builtin_clz(ramask & ~(1u << pa)) : hi));
+        }
+        while ((uint32_t)__builtin_popcount(rbmask) > rr) {
+            const uint32_t hi = 31u - (uint32_t)__builtin_clz(rbmask);
+            rbmask &= ~(1u << (hi == pb && pair ? 31u - (uint32_t)__builtin_clz(rbmask & ~(1u << pb)) : hi));
+        }
+        rdb = (want_d && nd > 0) ? 1u : 0u;
+        rdmask = rdb ? (1u << cd[0]) : 0u;
+        const uint32_t regs = ramask | rbmask | rdmask;
+        if ((uint32_t)__builtin_popcount(omask & ~regs) < 8u) continue;
+        // lane bits: the lowest 8 other output bits; block bits: the rest
+        uint32_t lane = 0;
+        for (uint32_t m = omask & ~regs, n = 0; n < 8u; m &= m - 1, ++n) lane |= m & (0u - m);
+        const uint32_t hmask = omask & ~regs & ~lane;
+        const uint32_t tm[4] = {ma, mb, md, omask};
+        ok = true;
+        for (int j = 0; j < 4 && ok; ++j) ok = chain_hk_runs(hmask, tm[j], e.hk_run[j], &e.hk_nrun[j]);
+        if (!ok) continue;
+        e.lane_mask = lane;
+        e.hk_bits = (uint32_t)__builtin_popcount(hmask);
     }
-    if (!ok) return set_error(QB_ERR_UNSUPPORTED, "chain op %d: deposit runs", k_op);
+    if (!ok) return set_error(QB_ERR_UNSUPPORTED, "chain op %d: no register and lane split fits", k_op);
+    if (pair && ((ramask & (0u - ramask)) != (1u << pa) || (rbmask & (0u - rbmask)) != (1u << pb)))
+        return set_error(QB_ERR_UNSUPPORTED, "chain op %d: pair bits", k_op);
     for (uint32_t p = 0; p < (1u << rr); ++p) {
-        uint32_t ha = 0, hb = 0;
-        for (uint32_t i = 0; i < rr; ++i)
-            if ((p >> i) & 1u) {
-                ha |= 1u << reg_a[i];
-                hb |= 1u << reg_b[i];
-            }
-        const uint32_t ua = pdep_host(ha << lo_bits, omask), ub = pdep_host(hb << lo_bits, omask);
-        e.rdep_a[p] = ha;
-        e.rdep_b[p] = hb;
-        e.roff_a[p] = pext_host(ua, a.mask[0]) * eb;
-        e.roff_b[p] = pext_host(ub, a.mask[1]) * eb;
-        e.droff_a[p] = pext_host(ua, a.mask[2]);
-        e.droff_b[p] = pext_host(ub, a.mask[2]);
+        const uint32_t ua = pdep_host(p, ramask), ub = pdep_host(p, rbmask);
+        e.roff_a[p] = pext_host(ua, ma) * eb;
+        e.roff_b[p] = pext_host(ub, mb) * eb;
+        e.droff_a[p] = pext_host(ua, md);
+        e.droff_b[p] = pext_host(ub, md);
+        e.oroff_a[p] = pext_host(ua, omask);
+        e.oroff_b[p] = pext_host(ub, omask);
     }
-    for (uint32_t p = 0; p < (1u << rdb); ++p) {
-        const uint32_t hd = p ? (1u << reg_d[0]) : 0u;
-        e.rdep_d[p] = hd;
-        e.droff_d[p] = pext_host(pdep_host(hd << lo_bits, omask), a.mask[2]);
+    if (pair)
+        for (uint32_t p = 0; p < (1u << rr); p += 2)
+            if (e.roff_a[p + 1] != e.roff_a[p] + eb || e.roff_b[p + 1] != e.roff_b[p] + eb)
+                return set_error(QB_ERR_UNSUPPORTED, "chain op %d: pair offsets", k_op);
+    for (uint32_t q = 0; q < (1u << rdb); ++q) {
+        const uint32_t ud = pdep_host(q, rdmask);
+        e.droff_d[q] = pext_host(ud, md);
+        e.oroff_d[q] = pext_host(ud, omask);
     }
     const uint64_t gcap = (uint64_t)g_num_cus * 16;
     const uint64_t want = 1ull << e.hk_bits;
     const int grid = (int)(want < gcap ? want : gcap);
 #define QB_CHAIN_CASE(RR_, RD_)                                                                           \
-    if (rr == RR_ && rdb == RD_) {                                                                        \
+    if (!pair && rr == RR_ && rdb == RD_) {                                                               \
         if (dtype == QB_C128)                                                                             \
-            hipLaunchKernelGGL((chain_kernel<double, RR_, RD_>), dim3(grid, 1, nz), dim3(256), 0, st, a, lo_tab, e); \
+            hipLaunchKernelGGL((chain_kernel<double, RR_, RD_, false>), dim3(grid, 1, nz), dim3(256), 0, st, a, e); \
         else                                                                                              \
-            hipLaunchKernelGGL((chain_kernel<float, RR_, RD_>), dim3(grid, 1, nz), dim3(256), 0, st, a, lo_tab, e);  \
+            hipLaunchKernelGGL((chain_kernel<float, RR_, RD_, false>), dim3(grid, 1, nz), dim3(256), 0, st, a, e);  \
     }
+#define QB_CHAIN_PAIR_CASE(RR_, RD_)                                                                      \
+    if (pair && rr == RR_ && rdb == RD_)                                                                  \
+        hipLaunchKernelGGL((chain_kernel<float, RR_, RD_, true>), dim3(grid, 1, nz), dim3(256), 0, st, a, e);
     QB_CHAIN_CASE(0, 0) QB_CHAIN_CASE(0, 1) QB_CHAIN_CASE(1, 0) QB_CHAIN_CASE(1, 1) QB_CHAIN_CASE(2, 0)
     QB_CHAIN_CASE(2, 1)
+    QB_CHAIN_PAIR_CASE(1, 0) QB_CHAIN_PAIR_CASE(1, 1) QB_CHAIN_PAIR_CASE(2, 0) QB_CHAIN_PAIR_CASE(2, 1)
+#undef QB_CHAIN_PAIR_CASE
 #undef QB_CHAIN_CASE
     QB_HIP(hipGetLastError());
     return QB_OK;
@@ -1935,7 +2052,7 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
     a.k = k;
     a.lo_bits = lo_bits;
     a.zbytes = zbytes;
-    if (chain) return launch_chain(a, lo_tab, n_in, smask, k_op, dtype, st, nz);
+    if (chain) return launch_chain(a, n_in, smask, k_op, dtype, st, nz);
     const uint64_t gcap = (uint64_t)g_num_cus * 16;
     if (n_in == 2 && rank_out >= lo_bits + 1) {
         // register bits: high output bits carried by one input only

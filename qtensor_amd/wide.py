@@ -281,8 +281,10 @@ class WideEngine:
                 wp._dev_words = cached
             words.append(cached[1])
             dev_words.append(cached[2])
-        # a stream's arena: room for its largest call (slices x the plan's need)
-        arenas = [torch.empty(max([max(wplans[works[w][0]].plan.glb_need, 1) * works[w][2] for w in ws] + [1]),
+        # a stream's arena: room for its largest call (slices x the plan's need,
+        # each slice's region 64-element aligned)
+        need = [-(-max(wp.plan.glb_need, 1) // 64) * 64 for wp in wplans]
+        arenas = [torch.empty(max([need[works[w][0]] * works[w][2] for w in ws] + [1]),
                               dtype=tdtype, device=dev) for ws in per_stream]
         prm = torch.zeros(len(params) + 8, dtype=torch.float64)
         prm[:len(params)] = torch.tensor([float(x) for x in params], dtype=torch.float64)
@@ -311,7 +313,7 @@ class WideEngine:
                 d.plan_host = words[i].ctypes.data
                 d.plan_words = words[i].size
                 d.plan_dev = dev_words[i].data_ptr()
-                d.arena_elems = arenas[k].numel() // c        # each slice's region
+                d.arena_elems = need[i] if c > 1 else arenas[k].numel()   # each slice's region
                 d.slice_bits = s
                 d.n_slices = c
                 d.out = out.data_ptr() + 16 * (seg[i] + s)
@@ -354,7 +356,7 @@ class WideEngine:
         if rank > 20:
             raise ValueError(f"{rank} open variables: at most 20")
         n_res = 1 << rank
-        need = max(plan.glb_need, 1)
+        need = -(-max(plan.glb_need, 1) // 64) * 64       # slices' regions stay aligned
         return {"words": words, "plan_dev": torch.from_numpy(words).to(dev), "rank": rank, "n_res": n_res,
                 "per": per, "arena_elems": need,
                 "arenas": [torch.empty(need * per, dtype=tdtype, device=dev) for _ in range(n_streams)],
