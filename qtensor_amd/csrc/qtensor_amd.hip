@@ -540,21 +540,22 @@ __global__ __launch_bounds__(256) void fused2_kernel(const FusedArgs a, const in
 // width up to 16 bytes, so pairs halve their work, which bounds this kernel.
 // Lane bits: the lowest 8 other output bits (thread t's offsets come from
 // pdep(t, lane_mask)).  Block bits: the rest, visited through bit runs per
-// target.  D (at most 2^kChainMaxD elements) and the byte offsets of every
-// (y, x) are staged in LDS.
+// target.  D (at most 2^kChainMaxD elements) and the offsets of every (y, x)
+// are staged in LDS.  Without D (has_d = 0) the kernel runs a plain two-input
+// op out = sum_x A B (D is the scalar 1 and y is empty): the same register
+// tiles and pair loads for fused two-input ops.
 constexpr uint32_t kFusedChain = 1u << 12;
-constexpr int kChainMaxX = 4, kChainMaxY = 5, kChainMaxD = 10, kChainRuns = 8;
+constexpr int kChainMaxX = 4, kChainMaxY = 5, kChainMaxXY = 9, kChainMaxD = 10, kChainRuns = 8;
 
 struct ChainArgs {
-    uint32_t xoff[2][1 << kChainMaxX];   // byte offset in A, B of inner summed assignment x
-    uint32_t yoff[3][1 << kChainMaxY];   // byte offset in A, B (element offset in D) of outer assignment y
     uint32_t roff_a[4], roff_b[4];       // byte offset in A (B) of register pattern p
     uint32_t droff_a[4], droff_b[4], droff_d[2];  // element offset in D of each register pattern
     uint32_t oroff_a[4], oroff_b[4], oroff_d[2];  // element offset in the output of each register pattern
     uint32_t hk_run[4][kChainRuns];      // block bits -> offset bits in A, B, D, out (src | w << 5 | dst << 10)
     uint32_t hk_nrun[4];
     uint32_t lane_mask;                  // union bits of the 8 lane bits
-    uint32_t hk_bits, kx, ky, d_elems;
+    uint32_t xbits, ybits;               // union bits of the inner (x) and outer (y) summed variables
+    uint32_t hk_bits, kx, ky, d_elems, has_d;
 };
 
 __device__ __forceinline__ uint32_t chain_runs(const uint32_t* run, uint32_t n, uint32_t hk) {
@@ -575,20 +576,27 @@ __global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const Cha
     static_assert(!PAIR || (sizeof(V) == 8 && RR >= 1), "pairs: complex64 with a register bit per input");
     constexpr int NR = 1 << RR, ND = 1 << RD;
     __shared__ V dsh[1 << kChainMaxD];
-    // byte offsets in A and B of summed assignment it = y << kx | x
-    __shared__ uint2 xy_off[1 << (kChainMaxX + kChainMaxY)];
+    // byte offsets in A and B of summed assignment it = y << kx | x, and D's
+    // element offset of its outer assignment y
+    __shared__ uint4 xy_off[1 << kChainMaxXY];
+    constexpr uint32_t EB = sizeof(V);
     const uint32_t tid = threadIdx.x;
     const int64_t zo = (int64_t)blockIdx.z * a.zbytes;       // this block's slice
-    const V* __restrict__ dsrc = reinterpret_cast<const V*>(a.in[2] + zo);
-    for (uint32_t i = tid; i < e.d_elems; i += 256u) dsh[i] = dsrc[i];
+    if (e.has_d) {
+        const V* __restrict__ dsrc = reinterpret_cast<const V*>(a.in[2] + zo);
+        for (uint32_t i = tid; i < e.d_elems; i += 256u) dsh[i] = dsrc[i];
+    } else if (tid == 0) {
+        dsh[0] = V{1, 0};
+    }
     const uint32_t n_x = 1u << e.kx, n_y = 1u << e.ky, n_hk = 1u << e.hk_bits;
     const uint32_t n_it = n_y << e.kx;
     for (uint32_t it = tid; it < n_it; it += 256u) {
-        const uint32_t x = it & (n_x - 1u), y = it >> e.kx;
-        xy_off[it] = make_uint2(e.yoff[0][y] + e.xoff[0][x], e.yoff[1][y] + e.xoff[1][x]);
+        const uint32_t uy = pdep_loop(it >> e.kx, e.ybits);
+        const uint32_t u = pdep_loop(it & (n_x - 1u), e.xbits) | uy;
+        xy_off[it] = make_uint4(pext_loop(u, a.mask[0]) * EB, pext_loop(u, a.mask[1]) * EB,
+                                e.has_d ? pext_loop(uy, a.mask[2]) : 0u, 0u);
     }
     __syncthreads();
-    constexpr uint32_t EB = sizeof(V);
     // this lane's part of every offset
     const uint32_t ul = pdep_loop(tid, e.lane_mask);
     const uint32_t la = pext_loop(ul, a.mask[0]) * EB, lb = pext_loop(ul, a.mask[1]) * EB;
@@ -621,10 +629,9 @@ __global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const Cha
         for (int i = 0; i < NR; ++i)
 #pragma unroll
             for (int j = 0; j < NR; ++j) t[i][j] = V{0, 0};
-        for (uint32_t it = 0; it < n_it; ++it) {
-            const uint2 xo = xy_off[it];
+        // the tiles of summed assignment xo: 2^RR values of A and of B
+        auto load = [&](const uint4& xo, V* va, V* vb) {
             const uint32_t xa = oa + xo.x, xb = ob + xo.y;
-            V va[NR], vb[NR];
             if constexpr (PAIR) {
                 // register pattern bit 0 is the input's lowest variable: p and
                 // p + 1 are adjacent elements
@@ -646,13 +653,16 @@ __global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const Cha
 #pragma unroll
                 for (int j = 0; j < NR; ++j) vb[j] = *reinterpret_cast<const V*>(in_b + (xb + rb[j]));
             }
+        };
+        // products into T; after the last inner assignment of an outer one,
+        // T times D into the outputs
+        auto step = [&](uint32_t it, const uint4& xo, const V* va, const V* vb) {
 #pragma unroll
             for (int i = 0; i < NR; ++i)
 #pragma unroll
                 for (int j = 0; j < NR; ++j) t[i][j] = cfma(t[i][j], va[i], vb[j]);
             if (((it + 1u) & (n_x - 1u)) == 0u) {
-                // the inner sum of outer assignment y is complete: multiply into D
-                const uint32_t yd = od + e.yoff[2][it >> e.kx];
+                const uint32_t yd = od + xo.z;
 #pragma unroll
                 for (int i = 0; i < NR; ++i)
 #pragma unroll
@@ -664,6 +674,23 @@ __global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const Cha
                         t[i][j] = V{0, 0};
                     }
             }
+        };
+        // two summed assignments per iteration: both tiles' loads are issued
+        // before the first products
+        uint32_t it = 0;
+        for (; it + 1u < n_it; it += 2u) {
+            const uint4 xo0 = xy_off[it], xo1 = xy_off[it + 1u];
+            V a0[NR], b0[NR], a1[NR], b1[NR];
+            load(xo0, a0, b0);
+            load(xo1, a1, b1);
+            step(it, xo0, a0, b0);
+            step(it + 1u, xo1, a1, b1);
+        }
+        if (it < n_it) {
+            const uint4 xo0 = xy_off[it];
+            V a0[NR], b0[NR];
+            load(xo0, a0, b0);
+            step(it, xo0, a0, b0);
         }
 #pragma unroll
         for (int i = 0; i < NR; ++i)
@@ -1816,42 +1843,33 @@ static bool chain_hk_runs(uint32_t hmask, uint32_t mask, uint32_t* run, uint32_t
     return true;
 }
 
-// Launch a chain op (kFusedChain): inputs A, B, D; the inner summed variables
-// are the (at most kChainMaxX lowest) summed variables D lacks.  `a` holds the
-// checked inputs and masks.
-static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32_t k_op, int32_t dtype,
-                        hipStream_t st, int nz) {
-    const uint32_t rank_out = a.rank_out, omask = a.omask;
-    const uint32_t ma = a.mask[0], mb = a.mask[1], md = a.mask[2];
-    uint32_t xbits = 0;
-    for (uint32_t m = smask & ~md, n = 0; m && n < (uint32_t)kChainMaxX; m &= m - 1, ++n) xbits |= m & (0u - m);
-    const uint32_t ybits = smask & ~xbits;
-    const uint32_t kx = (uint32_t)__builtin_popcount(xbits), ky = (uint32_t)__builtin_popcount(ybits);
-    const uint32_t rd = (uint32_t)__builtin_popcount(md);
-    if (n_in != 3 || rank_out < 8 || kx < 1 || kx > (uint32_t)kChainMaxX || ky > (uint32_t)kChainMaxY ||
-        rd > (uint32_t)kChainMaxD)
-        return set_error(QB_ERR_INVALID, "chain op %d: %u inputs, rank %u, %u inner and %u outer summed variables, "
-                         "third input of rank %u", k_op, n_in, rank_out, kx, ky, rd);
-    const uint32_t eb = dtype == QB_C128 ? 16u : 8u;
-    // A and B are addressed with 32-bit byte offsets
-    if (__builtin_popcount(ma) + (eb == 16u ? 4 : 3) > 31 || __builtin_popcount(mb) + (eb == 16u ? 4 : 3) > 31)
-        return set_error(QB_ERR_INVALID, "chain op %d: an input of 2 GiB or more", k_op);
+// Register, lane and block bits of a chain kernel launch (see chain_kernel)
+// for inputs A, B (and D when has_d) of `a`, inner summed variables xbits and
+// outer ybits.  Returns false when no split fits.
+struct ChainPlan {
     ChainArgs e;
+    uint32_t rr, rdb;
+    bool pair;
+};
+
+static bool chain_config(const FusedArgs& a, uint32_t xbits, uint32_t ybits, bool has_d, int32_t dtype,
+                         ChainPlan* cp) {
+    const uint32_t omask = a.omask;
+    const uint32_t ma = a.mask[0], mb = a.mask[1], md = has_d ? a.mask[2] : 0u;
+    const uint32_t kx = (uint32_t)__builtin_popcount(xbits), ky = (uint32_t)__builtin_popcount(ybits);
+    const uint32_t eb = dtype == QB_C128 ? 16u : 8u;
+    const int lg = eb == 16u ? 4 : 3;
+    if (a.rank_out < 8 || ky > (uint32_t)kChainMaxY || kx + ky > (uint32_t)kChainMaxXY ||
+        __builtin_popcount(md) > kChainMaxD || __builtin_popcount(ma) + lg > 31 || __builtin_popcount(mb) + lg > 31)
+        return false;
+    ChainArgs& e = cp->e;
     memset(&e, 0, sizeof(e));
     e.kx = kx;
     e.ky = ky;
-    e.d_elems = 1u << rd;
-    for (uint32_t x = 0; x < (1u << kx); ++x) {
-        const uint32_t u = pdep_host(x, xbits);
-        e.xoff[0][x] = pext_host(u, ma) * eb;
-        e.xoff[1][x] = pext_host(u, mb) * eb;
-    }
-    for (uint32_t y = 0; y < (1u << ky); ++y) {
-        const uint32_t u = pdep_host(y, ybits);
-        e.yoff[0][y] = pext_host(u, ma) * eb;
-        e.yoff[1][y] = pext_host(u, mb) * eb;
-        e.yoff[2][y] = pext_host(u, md);
-    }
+    e.xbits = xbits;
+    e.ybits = ybits;
+    e.has_d = has_d ? 1u : 0u;
+    e.d_elems = 1u << __builtin_popcount(md);
     // output bits that only A (B, D) carries, highest first
     uint32_t ca[32], cb[32], cd[32], na = 0, nb = 0, nd = 0;
     for (int u = 31; u >= 0; --u) {
@@ -1862,24 +1880,23 @@ static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32
         else if (in_d && !in_a && !in_b) cd[nd++] = (uint32_t)u;
     }
     // pairs: A's and B's lowest variables are output bits only they carry
-    const uint32_t pa = ma ? (uint32_t)__builtin_ctz(ma) : 32u, pb = mb ? (uint32_t)__builtin_ctz(mb) : 32u;
     // (16-byte loads: A, B and every slice's region 16-byte aligned)
+    const uint32_t pa = ma ? (uint32_t)__builtin_ctz(ma) : 32u, pb = mb ? (uint32_t)__builtin_ctz(mb) : 32u;
     const bool aligned = ((reinterpret_cast<uintptr_t>(a.in[0]) | reinterpret_cast<uintptr_t>(a.in[1]) |
                            (uintptr_t)a.zbytes) & 15u) == 0;
     const bool can_pair = dtype == QB_C64 && aligned && pa < 32u && pb < 32u && ((omask >> pa) & 1u) &&
                           !((mb >> pa) & 1u) && ((omask >> pb) & 1u) && !((ma >> pb) & 1u);
-    bool ok = false;
     uint32_t rr = 0, rdb = 0, ramask = 0, rbmask = 0, rdmask = 0;
-    bool pair = false;
-    for (int cfg = 0; cfg < 8 && !ok; ++cfg) {
-        // configurations by preference: pairs with 2 or 1 register bits per
-        // input, then without pairs 2, 1, 0; each with and without a D bit
+    bool pair = false, ok = false;
+    for (int cfg = 0; cfg < 10 && !ok; ++cfg) {
+        // by preference: pairs with 2 or 1 register bits per input, then
+        // without pairs 2, 1, 0; each with and without a D bit
         const bool want_pair = cfg < 4;
-        const uint32_t want_r = cfg < 4 ? (cfg < 2 ? 2u : 1u) : (cfg < 6 ? 2u : 1u);
+        const uint32_t want_r = cfg < 2 ? 2u : cfg < 4 ? 1u : cfg < 6 ? 2u : cfg < 8 ? 1u : 0u;
         const uint32_t want_d = (cfg & 1) ? 0u : 1u;
         if (want_pair && !can_pair) continue;
         pair = want_pair;
-        ramask = rbmask = rdmask = 0;
+        ramask = rbmask = 0;
         uint32_t ga = 0, gb = 0;
         if (pair) {
             ramask = 1u << pa;
@@ -1892,15 +1909,9 @@ static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32
             if (!((rbmask >> cb[i]) & 1u)) { rbmask |= 1u << cb[i]; ++gb; }
         rr = ga < gb ? ga : gb;
         if (pair && rr < 1) continue;
-        // equal counts: drop the extra non-pair bits
-        while ((uint32_t)__builtin_popcount(ramask) > rr) {
-            const uint32_t hi = 31u - (uint32_t)__builtin_clz(ramask);
-            ramask &= ~(1u << (hi == pa && pair ? 31u - (uint32_t)__builtin_clz(ramask & ~(1u << pa)) : hi));
-        }
-        while ((uint32_t)__builtin_popcount(rbmask) > rr) {
-            const uint32_t hi = 31u - (uint32_t)__builtin_clz(rbmask);
-            rbmask &= ~(1u << (hi == pb && pair ? 31u - (uint32_t)__builtin_clz(rbmask & ~(1u << pb)) : hi));
-        }
+        // equal counts: drop the highest extra bits (never a pair bit, the lowest)
+        while ((uint32_t)__builtin_popcount(ramask) > rr) ramask &= ~(1u << (31u - (uint32_t)__builtin_clz(ramask)));
+        while ((uint32_t)__builtin_popcount(rbmask) > rr) rbmask &= ~(1u << (31u - (uint32_t)__builtin_clz(rbmask)));
         rdb = (want_d && nd > 0) ? 1u : 0u;
         rdmask = rdb ? (1u << cd[0]) : 0u;
         const uint32_t regs = ramask | rbmask | rdmask;
@@ -1916,9 +1927,8 @@ static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32
         e.lane_mask = lane;
         e.hk_bits = (uint32_t)__builtin_popcount(hmask);
     }
-    if (!ok) return set_error(QB_ERR_UNSUPPORTED, "chain op %d: no register and lane split fits", k_op);
-    if (pair && ((ramask & (0u - ramask)) != (1u << pa) || (rbmask & (0u - rbmask)) != (1u << pb)))
-        return set_error(QB_ERR_UNSUPPORTED, "chain op %d: pair bits", k_op);
+    if (!ok) return false;
+    if (pair && ((ramask & (0u - ramask)) != (1u << pa) || (rbmask & (0u - rbmask)) != (1u << pb))) return false;
     for (uint32_t p = 0; p < (1u << rr); ++p) {
         const uint32_t ua = pdep_host(p, ramask), ub = pdep_host(p, rbmask);
         e.roff_a[p] = pext_host(ua, ma) * eb;
@@ -1930,16 +1940,25 @@ static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32
     }
     if (pair)
         for (uint32_t p = 0; p < (1u << rr); p += 2)
-            if (e.roff_a[p + 1] != e.roff_a[p] + eb || e.roff_b[p + 1] != e.roff_b[p] + eb)
-                return set_error(QB_ERR_UNSUPPORTED, "chain op %d: pair offsets", k_op);
+            if (e.roff_a[p + 1] != e.roff_a[p] + eb || e.roff_b[p + 1] != e.roff_b[p] + eb) return false;
     for (uint32_t q = 0; q < (1u << rdb); ++q) {
         const uint32_t ud = pdep_host(q, rdmask);
         e.droff_d[q] = pext_host(ud, md);
         e.oroff_d[q] = pext_host(ud, omask);
     }
+    cp->rr = rr;
+    cp->rdb = rdb;
+    cp->pair = pair;
+    return true;
+}
+
+static int launch_chain_plan(const FusedArgs& a, const ChainPlan& cp, int32_t dtype, hipStream_t st, int nz) {
     const uint64_t gcap = (uint64_t)g_num_cus * 16;
-    const uint64_t want = 1ull << e.hk_bits;
+    const uint64_t want = 1ull << cp.e.hk_bits;
     const int grid = (int)(want < gcap ? want : gcap);
+    const uint32_t rr = cp.rr, rdb = cp.rdb;
+    const bool pair = cp.pair;
+    const ChainArgs& e = cp.e;
 #define QB_CHAIN_CASE(RR_, RD_)                                                                           \
     if (!pair && rr == RR_ && rdb == RD_) {                                                               \
         if (dtype == QB_C128)                                                                             \
@@ -1957,6 +1976,27 @@ static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32
 #undef QB_CHAIN_CASE
     QB_HIP(hipGetLastError());
     return QB_OK;
+}
+
+// Launch a chain op (kFusedChain): inputs A, B, D; the inner summed variables
+// are the (at most kChainMaxX lowest) summed variables D lacks.  `a` holds the
+// checked inputs and masks.
+static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32_t k_op, int32_t dtype,
+                        hipStream_t st, int nz) {
+    const uint32_t md = a.mask[2];
+    uint32_t xbits = 0;
+    for (uint32_t m = smask & ~md, n = 0; m && n < (uint32_t)kChainMaxX; m &= m - 1, ++n) xbits |= m & (0u - m);
+    const uint32_t ybits = smask & ~xbits;
+    const uint32_t kx = (uint32_t)__builtin_popcount(xbits), ky = (uint32_t)__builtin_popcount(ybits);
+    const uint32_t rd = (uint32_t)__builtin_popcount(md);
+    if (n_in != 3 || a.rank_out < 8 || kx < 1 || ky > (uint32_t)kChainMaxY || rd > (uint32_t)kChainMaxD)
+        return set_error(QB_ERR_INVALID, "chain op %d: %u inputs, rank %u, %u inner and %u outer summed variables, "
+                         "third input of rank %u", k_op, n_in, a.rank_out, kx, ky, rd);
+    ChainPlan cp;
+    if (!chain_config(a, xbits, ybits, true, dtype, &cp))
+        return set_error(QB_ERR_UNSUPPORTED, "chain op %d: no register and lane split fits (or an input of 2 GiB "
+                         "or more)", k_op);
+    return launch_chain_plan(a, cp, dtype, st, nz);
 }
 
 // Parse, check and launch one fused op of a wide plan (host copy `w`, device
@@ -2085,6 +2125,18 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
             for (uint32_t b = 0; b < hi_bits; ++b)
                 if (((keep >> b) & 1u) && (b == 0 || !((keep >> (b - 1)) & 1u))) ++runs;
             if (runs <= (uint32_t)kFusedDepRuns) break;
+        }
+        {
+            // the chain kernel without D: lane bits of its own choosing, so
+            // register bits may come from the low output bits too, and 16-byte
+            // pair loads; taken when it loads less per product
+            ChainPlan cp;
+            if (chain_config(a, smask, 0u, false, dtype, &cp)) {
+                const double nr = (double)(1u << cp.rr);
+                const double cost_c = 2.0 * nr / (cp.pair ? 2.0 : 1.0) / (nr * nr);
+                const double cost_f = (double)((1u << ra) + (1u << rb)) / (double)(1u << (ra + rb));
+                if (cost_c < cost_f) return launch_chain_plan(a, cp, dtype, st, nz);
+            }
         }
         if (ra + rb > 0) {
             Fused2Args e;
