@@ -296,9 +296,15 @@ def _step_work(n_in, union_rank):
     return (6 * max(n_in - 1, 0) + 2 + 4 * n_in) * (1 << union_rank)
 
 
-def _step_time(in_ranks, out_rank, union_rank, bw=4e12, rate=40e12, launch=4e-6):
+# the wide path's time model: seconds per launch of a grid kernel
+MODEL_LAUNCH_S = 4e-6
+
+
+def _step_time(in_ranks, out_rank, union_rank, bw=4e12, rate=40e12, launch=None):
     """Time model of one step or fused step on the device (memory, arithmetic
     including its loads, launch)."""
+    if launch is None:
+        launch = MODEL_LAUNCH_S
     nbytes = 8 * (sum(1 << r for r in in_ranks) + (1 << out_rank))
     return max(nbytes / bw, _step_work(len(in_ranks), union_rank) / rate) + launch
 
@@ -407,7 +413,7 @@ def split_small_inputs(steps):
     return out
 
 
-def chain_pairs(steps, bw=4e12, rate=40e12, launch=4e-6):
+def chain_pairs(steps, bw=4e12, rate=40e12, launch=None):
     """Merge a two-input contraction T = sum_x A B with the two-input
     contraction out = sum_y T D that reads T into one chain op over A, B and D
     (``FusedStep(chain=True)``), so that T, often the largest tensor of a wide
@@ -473,7 +479,7 @@ def chain_pairs(steps, bw=4e12, rate=40e12, launch=4e-6):
         # products A B: every union assignment, except that one output bit only D
         # carries shares them; products with D: every (output, outer) assignment
         work = _step_work(2, len(union) - min(d_only, 1)) + _step_work(2, len(out_c) + ky)
-        t_chain = max(nbytes / bw, work / rate) + launch
+        t_chain = max(nbytes / bw, work / rate) + (MODEL_LAUNCH_S if launch is None else launch)
         if t_chain < 0.9 * t_sep:
             a_, b_ = pst.inputs
             members = list(getattr(pst, "members", [])) + list(getattr(c, "members", []))
