@@ -374,6 +374,15 @@ __global__ __launch_bounds__(256) void step_kernel_k(const StepArgsK<K> a) {
 // Fused steps (wide plans): several bucket steps contracted in one pass
 // ---------------------------------------------------------------------------
 
+// Fused op record (int32 words, written by plan.PlanCompiler._encode_fused):
+//   w0 = kOpFused | chain flag (1 << 12, kFusedChain below) | n_in | sync << 11 | len << 16
+//   w1 = rank_u | rank_out << 8 | k << 16 | lo_bits << 24, w2 = out_loc, w3 = 0,
+//   w4 = omask, w5 = smask (output and summed variables as bits of the union
+//   offset), w6 = w7 = 0; per input j: loc, mask, 0, 0; then LO[o][j] for
+//   o < 2^lo_bits and X[x][j] for x < 2^k, the offsets pext(pdep(o, omask), mask_j)
+//   and pext(pdep(x, smask), mask_j), which the host checks against the masks.
+// out[o] = sum over x of prod_j in_j[LO/high part of o + X[x][j]].  A chain op
+// (three inputs) has the same value; the flag selects the factorised kernel.
 constexpr int kFusedMaxIn = 8;
 constexpr int kFusedHdr = 8;
 constexpr uint32_t kOpFused = 1u << 14;
