@@ -711,6 +711,130 @@ __global__ __launch_bounds__(256) void chain_kernel(const FusedArgs a, const Cha
     }
 }
 
+// Chain op on the matrix cores (complex64): per wave a 16 x 16 tile of the
+// inner sum T[m][n] = sum_x A[m, x] B[x, n] over 4 output bits only A carries
+// (m) and 4 only B carries (n), formed by v_mfma_f32_16x16x4f32 in chunks of 4
+// inner assignments (four real products per complex chunk: Re += Ar Br - Ai Bi,
+// Im += Ar Bi + Ai Br), then multiplied by D on the vector units into 2^RD
+// outputs per T value (RD: an output bit only D carries).  Operand layout of
+// the instruction (checked on the device): lane k 16 + i supplies A[i][k] and
+// B[k][i]; lane (i / 4) 16 + j holds result element i % 4 of row i, column j.
+// Each wave of a block takes its own block-bit value hk; D and the offsets of
+// every (y, x) are staged in LDS as for chain_kernel.
+struct ChainMfmaArgs {
+    uint32_t m_mask, n_mask, d_mask;     // union bits of m, n and the D register bit
+    uint32_t xbits, ybits;               // inner and outer summed variables
+    uint32_t hk_run[4][kChainRuns];      // block bits -> offset bits in A, B, D, out
+    uint32_t hk_nrun[4];
+    uint32_t hk_bits, kx, ky, d_elems, has_d;
+};
+
+template <int RD>
+__global__ __launch_bounds__(256) void chain_mfma_kernel(const FusedArgs a, const ChainMfmaArgs e) {
+    using V = typename Cx<float>::V;
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    constexpr int ND = 1 << RD;
+    constexpr uint32_t EB = sizeof(V);
+    __shared__ V dsh[1 << kChainMaxD];
+    __shared__ uint4 xy_off[1 << kChainMaxXY];
+    const uint32_t tid = threadIdx.x;
+    const int64_t zo = (int64_t)blockIdx.z * a.zbytes;       // this block's slice
+    if (e.has_d) {
+        const V* __restrict__ dsrc = reinterpret_cast<const V*>(a.in[2] + zo);
+        for (uint32_t i = tid; i < e.d_elems; i += 256u) dsh[i] = dsrc[i];
+    } else if (tid == 0) {
+        dsh[0] = V{1, 0};
+    }
+    const uint32_t n_x = 1u << e.kx, n_y = 1u << e.ky, n_hk = 1u << e.hk_bits;
+    const uint32_t n_it = n_y << e.kx;
+    for (uint32_t it = tid; it < n_it; it += 256u) {
+        const uint32_t uy = pdep_loop(it >> e.kx, e.ybits);
+        const uint32_t u = pdep_loop(it & (n_x - 1u), e.xbits) | uy;
+        xy_off[it] = make_uint4(pext_loop(u, a.mask[0]) * EB, pext_loop(u, a.mask[1]) * EB,
+                                e.has_d ? pext_loop(uy, a.mask[2]) : 0u, 0u);
+    }
+    __syncthreads();
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    const uint32_t r = lane & 15u, kk = lane >> 4;
+    // this lane's operand rows (A: m = r, B: n = r) and result rows m = 4 kk + i
+    const uint32_t un = pdep_loop(r, e.n_mask);
+    const uint32_t la = pext_loop(pdep_loop(r, e.m_mask), a.mask[0]) * EB;
+    const uint32_t lb = pext_loop(un, a.mask[1]) * EB;
+    const uint32_t dn = pext_loop(un, a.mask[2]), on = pext_loop(un, a.omask);
+    uint32_t dm[4], om[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t um = pdep_loop(kk * 4u + (uint32_t)i, e.m_mask);
+        dm[i] = pext_loop(um, a.mask[2]);
+        om[i] = pext_loop(um, a.omask);
+    }
+    uint32_t dd[ND], od_[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const uint32_t ud = d ? e.d_mask : 0u;
+        dd[d] = pext_loop(ud, a.mask[2]);
+        od_[d] = pext_loop(ud, a.omask);
+    }
+    const char* __restrict__ in_a = a.in[0] + zo;
+    const char* __restrict__ in_b = a.in[1] + zo;
+    V* out = reinterpret_cast<V*>(a.out + zo);
+    const uint32_t n_ch = n_x >> 2;         // chunks of 4 inner assignments (kx >= 2)
+    for (uint32_t hk = blockIdx.x * 4u + wave; hk < n_hk; hk += gridDim.x * 4u) {
+        const uint32_t oa = la + chain_runs(e.hk_run[0], e.hk_nrun[0], hk) * EB;
+        const uint32_t ob = lb + chain_runs(e.hk_run[1], e.hk_nrun[1], hk) * EB;
+        const uint32_t od = dn + chain_runs(e.hk_run[2], e.hk_nrun[2], hk);
+        const uint32_t oo = on + chain_runs(e.hk_run[3], e.hk_nrun[3], hk);
+        V acc[4][ND];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) acc[i][d] = V{0, 0};
+        // the operands of outer assignment y + 1 are loaded while y's matrix
+        // products and D step run
+        V av[4], bv[4], an[4], bn[4];
+        uint32_t yd = 0, ydn = 0;
+        auto load = [&](uint32_t y, V* pa, V* pb, uint32_t& ydo) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((uint32_t)c >= n_ch) break;
+                const uint4 xo = xy_off[(y << e.kx) | ((uint32_t)c * 4u + kk)];
+                ydo = xo.z;
+                pa[c] = *reinterpret_cast<const V*>(in_a + (oa + xo.x));
+                pb[c] = *reinterpret_cast<const V*>(in_b + (ob + xo.y));
+            }
+        };
+        load(0u, av, bv, yd);
+        for (uint32_t y = 0; y < n_y; ++y) {
+            if (y + 1u < n_y) load(y + 1u, an, bn, ydn);
+            f32x4 tr = {0.f, 0.f, 0.f, 0.f}, ti = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((uint32_t)c >= n_ch) break;
+                tr = __builtin_amdgcn_mfma_f32_16x16x4f32(av[c].x, bv[c].x, tr, 0, 0, 0);
+                ti = __builtin_amdgcn_mfma_f32_16x16x4f32(av[c].x, bv[c].y, ti, 0, 0, 0);
+                tr = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[c].y, bv[c].y, tr, 0, 0, 0);
+                ti = __builtin_amdgcn_mfma_f32_16x16x4f32(av[c].y, bv[c].x, ti, 0, 0, 0);
+            }
+            const uint32_t base = od + yd;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                    acc[i][d] = cfma(acc[i][d], V{tr[i], ti[i]}, dsh[base + dm[i] + dd[d]]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                av[c] = an[c];
+                bv[c] = bn[c];
+            }
+            yd = ydn;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) out[oo + om[i] + od_[d]] = acc[i][d];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched executor
 // ---------------------------------------------------------------------------
@@ -1987,6 +2111,60 @@ static int launch_chain_plan(const FusedArgs& a, const ChainPlan& cp, int32_t dt
     return QB_OK;
 }
 
+// The matrix-core form of a chain launch (complex64): 4 output bits only A
+// carries (the lowest: consecutive lanes read consecutive elements), 4 only B
+// carries, one only D carries when there is one, 4 to 16 inner assignments
+// (kx 2..4); false when the op does not have them.
+static bool chain_mfma_config(const FusedArgs& a, uint32_t xbits, uint32_t ybits, bool has_d, int32_t dtype,
+                              ChainMfmaArgs* e, uint32_t* rd_out) {
+    const uint32_t omask = a.omask;
+    const uint32_t ma = a.mask[0], mb = a.mask[1], md = has_d ? a.mask[2] : 0u;
+    const uint32_t kx = (uint32_t)__builtin_popcount(xbits), ky = (uint32_t)__builtin_popcount(ybits);
+    if (dtype != QB_C64 || kx < 2 || kx > (uint32_t)kChainMaxX || ky > (uint32_t)kChainMaxY ||
+        kx + ky > (uint32_t)kChainMaxXY || __builtin_popcount(md) > kChainMaxD ||
+        __builtin_popcount(ma) + 3 > 31 || __builtin_popcount(mb) + 3 > 31)
+        return false;
+    uint32_t mm = 0, nm = 0, dmask = 0, na = 0, nb = 0;
+    for (uint32_t u = 0; u < 32; ++u) {
+        if (!((omask >> u) & 1u)) continue;
+        const bool in_a = (ma >> u) & 1u, in_b = (mb >> u) & 1u, in_d = (md >> u) & 1u;
+        if (in_a && !in_b && na < 4) { mm |= 1u << u; ++na; }
+        else if (in_b && !in_a && nb < 4) { nm |= 1u << u; ++nb; }
+        else if (in_d && !in_a && !in_b && !dmask) dmask = 1u << u;
+    }
+    if (na < 4 || nb < 4) return false;
+    memset(e, 0, sizeof(*e));
+    const uint32_t hmask = omask & ~(mm | nm | dmask);
+    const uint32_t tm[4] = {ma, mb, md, omask};
+    for (int j = 0; j < 4; ++j)
+        if (!chain_hk_runs(hmask, tm[j], e->hk_run[j], &e->hk_nrun[j])) return false;
+    e->m_mask = mm;
+    e->n_mask = nm;
+    e->d_mask = dmask;
+    e->xbits = xbits;
+    e->ybits = ybits;
+    e->hk_bits = (uint32_t)__builtin_popcount(hmask);
+    e->kx = kx;
+    e->ky = ky;
+    e->d_elems = 1u << __builtin_popcount(md);
+    e->has_d = has_d ? 1u : 0u;
+    *rd_out = dmask ? 1u : 0u;
+    return true;
+}
+
+static int launch_chain_mfma(const FusedArgs& a, const ChainMfmaArgs& e, uint32_t rd, hipStream_t st, int nz) {
+    const uint64_t waves = 1ull << e.hk_bits;
+    const uint64_t want = (waves + 3) / 4;
+    const uint64_t gcap = (uint64_t)g_num_cus * 16;
+    const int grid = (int)(want < gcap ? want : gcap);
+    if (rd)
+        hipLaunchKernelGGL(chain_mfma_kernel<1>, dim3(grid, 1, nz), dim3(256), 0, st, a, e);
+    else
+        hipLaunchKernelGGL(chain_mfma_kernel<0>, dim3(grid, 1, nz), dim3(256), 0, st, a, e);
+    QB_HIP(hipGetLastError());
+    return QB_OK;
+}
+
 // Launch a chain op (kFusedChain): inputs A, B, D; the inner summed variables
 // are the (at most kChainMaxX lowest) summed variables D lacks.  `a` holds the
 // checked inputs and masks.
@@ -2001,6 +2179,9 @@ static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32
     if (n_in != 3 || a.rank_out < 8 || kx < 1 || ky > (uint32_t)kChainMaxY || rd > (uint32_t)kChainMaxD)
         return set_error(QB_ERR_INVALID, "chain op %d: %u inputs, rank %u, %u inner and %u outer summed variables, "
                          "third input of rank %u", k_op, n_in, a.rank_out, kx, ky, rd);
+    ChainMfmaArgs em;
+    uint32_t rdm = 0;
+    if (chain_mfma_config(a, xbits, ybits, true, dtype, &em, &rdm)) return launch_chain_mfma(a, em, rdm, st, nz);
     ChainPlan cp;
     if (!chain_config(a, xbits, ybits, true, dtype, &cp))
         return set_error(QB_ERR_UNSUPPORTED, "chain op %d: no register and lane split fits (or an input of 2 GiB "
