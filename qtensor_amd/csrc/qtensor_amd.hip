@@ -768,12 +768,14 @@ __global__ __launch_bounds__(256) void chain_mfma_kernel(const FusedArgs a, cons
         dm[i] = pext_loop(um, a.mask[2]);
         om[i] = pext_loop(um, a.omask);
     }
-    uint32_t dd[ND], od_[ND];
+    uint32_t dmd[4][ND], od_[ND];        // D offsets of (result row i, D bit d)
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const uint32_t ud = d ? e.d_mask : 0u;
-        dd[d] = pext_loop(ud, a.mask[2]);
+        const uint32_t dd = pext_loop(ud, a.mask[2]);
         od_[d] = pext_loop(ud, a.omask);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dmd[i][d] = dm[i] + dd;
     }
     const char* __restrict__ in_a = a.in[0] + zo;
     const char* __restrict__ in_b = a.in[1] + zo;
@@ -789,23 +791,17 @@ __global__ __launch_bounds__(256) void chain_mfma_kernel(const FusedArgs a, cons
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int d = 0; d < ND; ++d) acc[i][d] = V{0, 0};
-        // the operands of outer assignment y + 1 are loaded while y's matrix
-        // products and D step run
-        V av[4], bv[4], an[4], bn[4];
-        uint32_t yd = 0, ydn = 0;
-        auto load = [&](uint32_t y, V* pa, V* pb, uint32_t& ydo) {
+        for (uint32_t y = 0; y < n_y; ++y) {
+            V av[4], bv[4];
+            uint32_t yd = 0;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 if ((uint32_t)c >= n_ch) break;
                 const uint4 xo = xy_off[(y << e.kx) | ((uint32_t)c * 4u + kk)];
-                ydo = xo.z;
-                pa[c] = *reinterpret_cast<const V*>(in_a + (oa + xo.x));
-                pb[c] = *reinterpret_cast<const V*>(in_b + (ob + xo.y));
+                yd = xo.z;
+                av[c] = *reinterpret_cast<const V*>(in_a + (oa + xo.x));
+                bv[c] = *reinterpret_cast<const V*>(in_b + (ob + xo.y));
             }
-        };
-        load(0u, av, bv, yd);
-        for (uint32_t y = 0; y < n_y; ++y) {
-            if (y + 1u < n_y) load(y + 1u, an, bn, ydn);
             f32x4 tr = {0.f, 0.f, 0.f, 0.f}, ti = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -815,18 +811,11 @@ __global__ __launch_bounds__(256) void chain_mfma_kernel(const FusedArgs a, cons
                 tr = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[c].y, bv[c].y, tr, 0, 0, 0);
                 ti = __builtin_amdgcn_mfma_f32_16x16x4f32(av[c].y, bv[c].x, ti, 0, 0, 0);
             }
-            const uint32_t base = od + yd;
+            const V* dy = dsh + (od + yd);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int d = 0; d < ND; ++d)
-                    acc[i][d] = cfma(acc[i][d], V{tr[i], ti[i]}, dsh[base + dm[i] + dd[d]]);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                av[c] = an[c];
-                bv[c] = bn[c];
-            }
-            yd = ydn;
+                for (int d = 0; d < ND; ++d) acc[i][d] = cfma(acc[i][d], V{tr[i], ti[i]}, dy[dmd[i][d]]);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
