@@ -197,9 +197,12 @@ int qb_check_program(const int32_t* program, int64_t program_words, const int32_
  * Wide plan: one compiled lightcone plan whose tensors (gates included) all
  * live in `arena` (global memory), for networks too large for the executor's
  * LDS tiers.  Builds the gate tensors from `params` for the slice assignment
- * `slice_bits`, launches one grid-wide step kernel per step op in order on
- * `stream`, and writes the result as complex128: the scalar to out[0..1], or,
- * for a plan with b open (batch) variables (plan word 11; the amplitudes of
+ * `slice_bits`, launches one grid-wide kernel per op in order on `stream` (a
+ * bucket step, a fused chain of steps, or a chain op: a contraction and the
+ * contraction reading its output, whose intermediate is never stored; the
+ * matrix cores form its inner sums when it has the tile shape they need),
+ * and writes the result as complex128: the scalar to out[0..1], or, for a
+ * plan with b open (batch) variables (plan word 11; the amplitudes of
  * `QtreeSimulator.simulate_batch(batch_vars=b)`, qtensor/Simulate.py:66-94),
  * element e of the 2^b result to out[2 e out_stride .. + 1] (first open
  * variable = most significant bit of e).  The host copy of the plan is parsed
