@@ -824,6 +824,56 @@ __global__ __launch_bounds__(256) void chain_mfma_kernel(const FusedArgs a, cons
     }
 }
 
+// Runs of gate-only ops (wide plans): fused ops that read gate tensors only
+// are independent of each other (their outputs lie past the gate area), and
+// the plan compiler puts them first.  One launch runs a run of them, one
+// block per op (grid x) and slice (grid z): each op is decoded from its record
+// in the device copy of the plan (checked on the host before the launch) and
+// evaluated from its offset tables, thread t taking outputs t, t + 256, ...;
+// the part of an offset above the table's low output bits is a pext of the
+// output's high bits.
+constexpr int kRunMaxOps = 64;
+constexpr uint32_t kRunMaxTerms = 1u << 14;    // outputs x summed assignments x inputs of one op
+constexpr uint32_t kLocBits = 0x7fffffffu;     // a location without its global-memory flag
+
+struct OpRun {
+    int32_t pos[kRunMaxOps];    // record positions in the plan
+    int32_t n;
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void gate_ops_kernel(const int32_t* __restrict__ plan, const OpRun run,
+                                                       char* __restrict__ arena, int64_t zbytes) {
+    using V = typename Cx<R>::V;
+    char* base = arena + (int64_t)blockIdx.z * zbytes;       // this block's slice
+    const int32_t* r = plan + run.pos[blockIdx.x];
+    const uint32_t n_in = (uint32_t)r[0] & 15u, meta = (uint32_t)r[1];
+    const uint32_t rank_out = (meta >> 8) & 255u, k = (meta >> 16) & 255u, lo_bits = meta >> 24;
+    const uint32_t omask = (uint32_t)r[4];
+    const int32_t* lo_tab = r + kFusedHdr + 4 * n_in;
+    const int32_t* x_tab = lo_tab + (n_in << lo_bits);
+    V* out = reinterpret_cast<V*>(base) + ((uint32_t)r[2] & kLocBits);
+    for (uint32_t o = threadIdx.x; o < (1u << rank_out); o += 256u) {
+        const uint32_t olo = o & ((1u << lo_bits) - 1u), ohi = o >> lo_bits;
+        const uint32_t uhi = ohi ? pdep_loop(ohi << lo_bits, omask) : 0u;
+        const V* in[kFusedMaxIn];
+        for (uint32_t j = 0; j < n_in; ++j) {
+            const uint32_t loc = (uint32_t)r[kFusedHdr + 4 * j] & kLocBits;
+            const uint32_t mask = (uint32_t)r[kFusedHdr + 4 * j + 1];
+            in[j] = reinterpret_cast<const V*>(base) + loc + (uint32_t)lo_tab[olo * n_in + j] +
+                    (ohi ? pext_loop(uhi, mask) : 0u);
+        }
+        V acc = {0, 0};
+        for (uint32_t x = 0; x < (1u << k); ++x) {
+            const int32_t* xt = x_tab + x * n_in;
+            V p = in[0][xt[0]];
+            for (uint32_t j = 1; j < n_in; ++j) p = cmul(p, in[j][xt[j]]);
+            acc = cadd(acc, p);
+        }
+        out[o] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched executor
 // ---------------------------------------------------------------------------
@@ -2183,7 +2233,8 @@ static int launch_chain(const FusedArgs& a, uint32_t n_in, uint32_t smask, int32
 // by an input's mask: the tables are recomputed here from the masks and must
 // match, and each input and the output must lie inside the arena.
 int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, int64_t pos, int32_t k_op,
-                 char* arena, int64_t elem, int64_t cap, int32_t dtype, hipStream_t st, int nz, int64_t zbytes) {
+                 char* arena, int64_t elem, int64_t cap, int32_t dtype, hipStream_t st, int nz, int64_t zbytes,
+                 bool check_only) {
     if (pos + kFusedHdr > plan_words) return set_error(QB_ERR_INVALID, "fused op %d outside the plan", k_op);
     const uint32_t w0 = (uint32_t)w[pos];
     const uint32_t n_in = w0 & 15u;
@@ -2263,6 +2314,7 @@ int launch_fused(const int32_t* w, int64_t plan_words, const int32_t* plan_dev, 
             }
         }
     }
+    if (check_only) return QB_OK;
     const int32_t* lo_tab = plan_dev + tab;                        // (1 << lo_bits) x n_in
     const int32_t* x_tab = plan_dev + tab + ((int64_t)n_in << lo_bits);  // (1 << k) x n_in
     a.out = arena + (int64_t)(out_loc & ~kGlobal) * elem;
@@ -2716,17 +2768,68 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
         QB_HIP(hipGetLastError());
     }
     char* arena = reinterpret_cast<char*>(d->arena);
+    // the gate area ends where the last gate tensor does; consecutive fused ops
+    // that read it only (each checked as any fused op) run in one launch
+    int64_t gate_end = 0;
+    for (int32_t g = 0; g < n_gates; ++g) {
+        const int32_t* e = w + gates_off + (size_t)g * kGateWords;
+        if ((int64_t)e[1] + e[2] > gate_end) gate_end = (int64_t)e[1] + e[2];
+    }
+    OpRun run;
+    run.n = 0;
+    auto flush_run = [&]() -> int {
+        if (run.n == 0) return QB_OK;
+        if (d->dtype == QB_C128)
+            hipLaunchKernelGGL(gate_ops_kernel<double>, dim3(run.n, 1, nz), dim3(256), 0, st, d->plan_dev, run,
+                               arena, zbytes);
+        else
+            hipLaunchKernelGGL(gate_ops_kernel<float>, dim3(run.n, 1, nz), dim3(256), 0, st, d->plan_dev, run,
+                               arena, zbytes);
+        run.n = 0;
+        QB_HIP(hipGetLastError());
+        return QB_OK;
+    };
     int64_t pos = ops_off;
     for (int32_t k = 0; k < n_ops; ++k) {
         if (pos + kStepHdr > d->plan_words) return set_error(QB_ERR_INVALID, "op %d outside the plan", k);
         const uint32_t w0 = (uint32_t)w[pos];
         if (w0 & (kOpGather | kOpReduce)) return set_error(QB_ERR_INVALID, "wide plans take step ops only");
         if (w0 & kOpFused) {
-            const int rc = launch_fused(w, d->plan_words, d->plan_dev, pos, k, arena, elem, cap, d->dtype, st, nz,
-                                        zbytes);
-            if (rc) return rc;
+            // a candidate for a run of gate-only ops, from the record as it
+            // stands (launch_fused then checks it like any fused op)
+            const uint32_t n_in = w0 & 15u;
+            bool gate_only = !(w0 & kFusedChain) && n_in >= 1 && n_in <= (uint32_t)kFusedMaxIn &&
+                             pos + kFusedHdr + 4 * (int64_t)n_in <= d->plan_words && pos <= INT32_MAX;
+            if (gate_only) {
+                const uint32_t meta = (uint32_t)w[pos + 1];
+                const uint32_t ro = (meta >> 8) & 255u, kk = (meta >> 16) & 255u;
+                gate_only = ro + kk < 32u && ((uint64_t)n_in << (ro + kk)) <= (uint64_t)kRunMaxTerms &&
+                            (int64_t)((uint32_t)w[pos + 2] & kLocBits) >= gate_end;
+            }
+            for (uint32_t j = 0; j < n_in && gate_only; ++j) {
+                const uint32_t loc = (uint32_t)w[pos + kFusedHdr + 4 * j] & kLocBits;
+                const uint32_t mask = (uint32_t)w[pos + kFusedHdr + 4 * j + 1];
+                gate_only = (int64_t)loc + (1ll << __builtin_popcount(mask)) <= gate_end;
+            }
+            int rc;
+            if (gate_only) {
+                if ((rc = launch_fused(w, d->plan_words, d->plan_dev, pos, k, arena, elem, cap, d->dtype, st, nz,
+                                       zbytes, true)))            // checks only
+                    return rc;
+                run.pos[run.n++] = (int32_t)pos;
+                if (run.n == kRunMaxOps && (rc = flush_run())) return rc;
+            } else {
+                if ((rc = flush_run())) return rc;
+                if ((rc = launch_fused(w, d->plan_words, d->plan_dev, pos, k, arena, elem, cap, d->dtype, st, nz,
+                                       zbytes, false)))
+                    return rc;
+            }
             pos += w0 >> 16;
             continue;
+        }
+        {
+            const int rc = flush_run();
+            if (rc) return rc;
         }
         const uint32_t n_in = w0 & 15u, rank_out = (w0 >> 4) & 31u, sum = (w0 >> 9) & 1u;
         const uint32_t out_loc = (uint32_t)w[pos + 2];
@@ -2770,6 +2873,10 @@ int qb_wide_execute(const qb_wide_desc* d, void* stream) {
         }
         if (rc) return rc;
         pos += w0 >> 16;
+    }
+    {
+        const int rc = flush_run();
+        if (rc) return rc;
     }
     const int32_t rank = w[11];
     if (rank < 0 || rank > 24) return set_error(QB_ERR_INVALID, "result rank %d", rank);

@@ -488,6 +488,27 @@ def chain_pairs(steps, bw=4e12, rate=40e12, launch=None):
     return [merged.get(i, st) for i, st in enumerate(steps) if i not in gone]
 
 
+def hoist_gate_only(steps):
+    """Wide plans: steps whose inputs are all gate tensors go first (in their
+    order, as fused records), so that they form one run of mutually
+    independent ops at the start of the plan, which ``qb_wide_execute``
+    launches together (one block per op).  Moving them earlier keeps every
+    dependency: they read gates only."""
+    def gate_only(st):
+        return not isinstance(st, ReduceStep) and all(nd.buf is not None and nd.buf.leaf for nd in st.inputs)
+
+    def as_fused(st):
+        # a plain step is the fused op over its inputs' variables
+        if getattr(st, "fused", False):
+            return st
+        union = set()
+        for nd in st.inputs:
+            union |= set(nd.dev)
+        return FusedStep(list(st.inputs), st.out, union, [])
+    first = [as_fused(st) for st in steps if gate_only(st)]
+    return first + [st for st in steps if not gate_only(st)]
+
+
 @dataclass
 class Plan:
     words: np.ndarray
@@ -781,6 +802,7 @@ class PlanCompiler:
                 exec_steps = split_small_inputs(fuse_steps(steps))
                 if self.chain:
                     exec_steps = chain_pairs(exec_steps)
+                exec_steps = hoist_gate_only(exec_steps)
             ops, op_kind = [[i] for i in range(len(exec_steps))], [False] * len(exec_steps)
         else:
             if self.reduce:
