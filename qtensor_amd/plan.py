@@ -361,6 +361,11 @@ def fuse_steps(steps):
     return out
 
 
+# split_small_inputs: the small inputs' product is taken when its union is at
+# least this many variables below the contraction's
+SPLIT_MARGIN = 6
+
+
 def split_small_inputs(steps):
     """Within a step or fused step of three or more inputs, multiply all inputs
     but the largest into one tensor first (summing the summed variables only
@@ -402,7 +407,7 @@ def split_small_inputs(steps):
         t_one = _step_time(ranks(st.inputs), len(out_set), len(union))
         t_two = _step_time(ranks(rest), len(p_dev), len(u_rest)) + \
             _step_time([len(big.dev), len(p_dev)], len(out_set), len(main_union))
-        small = len(u_rest) + 6 <= len(main_union) and \
+        small = len(u_rest) + SPLIT_MARGIN <= len(main_union) and \
             _step_work(2, len(main_union)) < 0.9 * _step_work(len(st.inputs), len(union))
         if not (t_two < 0.9 * t_one or small):
             out.append(st)
